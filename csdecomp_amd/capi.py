@@ -1,0 +1,218 @@
+"""ctypes binding of the C-ABI (include/csdecomp_amd.h) for callers that
+hold device pointers (e.g. torch tensors on ``cuda:0``): the batched checker,
+FK, sampler, edge validation, occupancy and k-NN with inputs already resident
+in HBM.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import LIB_PATH
+
+_VP, _I32, _I64, _F, _U32, _U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_uint32, C.c_uint64
+
+
+class PlantDesc(C.Structure):
+    _fields_ = [
+        ("num_links", _I32), ("num_joints", _I32), ("num_configuration_variables", _I32),
+        ("joint_type", _VP), ("joint_parent_link", _VP), ("joint_child_link", _VP),
+        ("joint_config_idx", _VP), ("traversal", _VP), ("joint_X_PL_J", _VP), ("joint_axis", _VP),
+        ("num_geometries", _I32), ("geom_type", _VP), ("geom_link", _VP), ("geom_group", _VP),
+        ("geom_dims", _VP), ("geom_X_L_B", _VP), ("num_collision_pairs", _I32),
+        ("collision_pairs", _VP),
+    ]
+
+
+_SIGS = {
+    "csd_status_string": (C.c_char_p, [C.c_int]),
+    "csd_last_error": (C.c_char_p, []),
+    "csd_init": (C.c_int, [C.c_int]),
+    "csd_plant_create": (C.c_int, [C.POINTER(PlantDesc), C.POINTER(_VP)]),
+    "csd_plant_destroy": (C.c_int, [_VP]),
+    "csd_plant_info": (C.c_int, [_VP, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
+    "csd_forward_kinematics": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _VP]),
+    "csd_check_collision_free": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _VP]),
+    "csd_check_collision_free_voxels": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _F, _VP, _I32, _VP, _VP]),
+    "csd_voxel_flags": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _F, _VP, _I32, _VP, _VP]),
+    "csd_hit_and_run": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _I32, _VP, _I64, _I32, _U64, _U32, _VP, _VP]),
+    "csd_bisect": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _I32, _I32, _VP, _I32, _F, _VP, _I32, _VP, _VP, _VP]),
+    "csd_place_faces": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _VP, _VP, _I32, _VP, _VP, _F, _I32, _VP, _VP]),
+    "csd_validate_edges": (C.c_int, [_VP, _VP, _I32, _VP, _I64, _F, _I32, _VP, _VP]),
+    "csd_occupancy": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _VP, _VP, _F, _VP, _VP]),
+    "csd_knn": (C.c_int, [_VP, _I32, _I32, _I32, _VP, _VP, _VP]),
+    "csd_stream_synchronize": (C.c_int, [_VP]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded C-ABI library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing; build with `python csdecomp_amd/build.py`")
+        _lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+class CsdError(RuntimeError):
+    def __init__(self, status, what):
+        l = lib()
+        super().__init__(f"{what}: {l.csd_status_string(status).decode()} "
+                         f"({l.csd_last_error().decode()})")
+        self.status = status
+
+
+def check(status, what="csd call"):
+    if status != 0:
+        raise CsdError(status, what)
+
+
+def init(device=0):
+    check(lib().csd_init(device), "csd_init")
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):  # torch tensor
+        return C.c_void_p(a.data_ptr())
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(int(a))
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if hasattr(s, "cuda_stream"):
+        return C.c_void_p(s.cuda_stream)
+    return C.c_void_p(int(s))
+
+
+class DevicePlant:
+    """Device image of a MinimalPlant, owned by this object."""
+
+    def __init__(self, minimal_plant):
+        mp = minimal_plant
+        t = mp.kin_tree
+        J = t.num_joints
+        joints = t.joints
+        geoms = mp.scene_collision_geometries
+        G = len(geoms)
+        self._keep = [
+            np.array([int(joints[j].type) for j in range(J)], np.int32),
+            np.array([joints[j].parent_link for j in range(J)], np.int32),
+            np.array([joints[j].child_link for j in range(J)], np.int32),
+            np.array(list(t.joint_idx_to_config_idx)[:J], np.int32),
+            np.array(list(t.joint_traversal_sequence)[:J], np.int32),
+            np.ascontiguousarray([np.asarray(joints[j].X_PL_J, np.float32).T.reshape(16) for j in range(J)],
+                                 np.float32).reshape(-1),
+            np.ascontiguousarray([np.asarray(joints[j].axis, np.float32) for j in range(J)], np.float32).reshape(-1),
+            np.array([int(g.type) for g in geoms], np.int32),
+            np.array([g.link_index for g in geoms], np.int32),
+            np.array([int(g.group) for g in geoms], np.int32),
+            np.ascontiguousarray([np.asarray(g.dimensions, np.float32) for g in geoms], np.float32).reshape(-1),
+            np.ascontiguousarray([np.asarray(g.X_L_B, np.float32).T.reshape(16) for g in geoms],
+                                 np.float32).reshape(-1),
+            np.ascontiguousarray(mp.collision_pairs_flat, np.int32),
+        ]
+        k = self._keep
+        d = PlantDesc(t.num_links, J, t.num_configuration_variables,
+                      *[x.ctypes.data for x in k[:7]], G, *[x.ctypes.data for x in k[7:12]],
+                      mp.num_collision_pairs, k[12].ctypes.data)
+        h = C.c_void_p()
+        check(lib().csd_plant_create(C.byref(d), C.byref(h)), "csd_plant_create")
+        self.handle = h
+        self.dof = t.num_configuration_variables
+        self.num_links = t.num_links
+
+    def info(self):
+        L, dof, P, fast = _I32(), _I32(), _I32(), _I32()
+        check(lib().csd_plant_info(self.handle, C.byref(L), C.byref(dof), C.byref(P), C.byref(fast)))
+        return {"num_links": L.value, "dof": dof.value, "device_pairs": P.value, "fast_path": bool(fast.value)}
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().csd_plant_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def check_collision_free(plant, q, n, stride_n, stride_j, out, stream=None):
+    check(lib().csd_check_collision_free(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+                                         _stream(stream)), "csd_check_collision_free")
+
+
+def check_collision_free_voxels(plant, q, n, stride_n, stride_j, voxels, num_voxels, radius,
+                                robot_geoms, out, stream=None):
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    check(lib().csd_check_collision_free_voxels(plant.handle, _ptr(q), n, stride_n, stride_j,
+                                                _ptr(voxels), num_voxels, radius, _ptr(rg), len(rg),
+                                                _ptr(out), _stream(stream)),
+          "csd_check_collision_free_voxels")
+
+
+def voxel_flags(plant, q, n, stride_n, stride_j, voxels, num_voxels, radius, robot_geoms, out,
+                stream=None):
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    check(lib().csd_voxel_flags(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(voxels),
+                                num_voxels, radius, _ptr(rg), len(rg), _ptr(out), _stream(stream)),
+          "csd_voxel_flags")
+
+
+def forward_kinematics(plant, q, n, stride_n, stride_j, out, stream=None):
+    check(lib().csd_forward_kinematics(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+                                       _stream(stream)), "csd_forward_kinematics")
+
+
+def hit_and_run(A, b, nf, face_stride, dim, num_polytopes, starts, n, mixing, seed, round_, out,
+                stream=None):
+    check(lib().csd_hit_and_run(_ptr(A), _ptr(b), _ptr(nf), face_stride, dim, num_polytopes,
+                                _ptr(starts), n, mixing, seed, round_, _ptr(out), _stream(stream)),
+          "csd_hit_and_run")
+
+
+def bisect(plant, particles, ls, le, seg_idx, n, steps, scol, proj, voxels=None, num_voxels=0,
+           radius=0.0, robot_geoms=(), stream=None):
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    check(lib().csd_bisect(plant.handle, _ptr(particles), _ptr(ls), _ptr(le), _ptr(seg_idx), n,
+                           steps, _ptr(voxels), num_voxels, radius, _ptr(rg), len(rg), _ptr(scol),
+                           _ptr(proj), _stream(stream)), "csd_bisect")
+
+
+def place_faces(A, b, nf, cap, dim, scol, proj, n, ls, le, margin, max_faces, added, stream=None):
+    check(lib().csd_place_faces(_ptr(A), _ptr(b), _ptr(nf), cap, dim, _ptr(scol), _ptr(proj), n,
+                                _ptr(ls), _ptr(le), margin, max_faces, _ptr(added), _stream(stream)),
+          "csd_place_faces")
+
+
+def validate_edges(plant, nodes, num_nodes, edges, num_edges, step, rule, out, stream=None):
+    check(lib().csd_validate_edges(plant.handle, _ptr(nodes), num_nodes, _ptr(edges), num_edges,
+                                   step, rule, _ptr(out), _stream(stream)), "csd_validate_edges")
+
+
+def occupancy(plant, q, n, stride_n, stride_j, robot_geoms, center, size, resolution, grid,
+              stream=None):
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    c = np.ascontiguousarray(center, np.float32)
+    s = np.ascontiguousarray(size, np.float32)
+    check(lib().csd_occupancy(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(rg), len(rg),
+                              _ptr(c), _ptr(s), resolution, _ptr(grid), _stream(stream)),
+          "csd_occupancy")
+
+
+def knn(nodes, num_nodes, dim, k, out_idx, out_d2=None, stream=None):
+    check(lib().csd_knn(_ptr(nodes), num_nodes, dim, k, _ptr(out_idx), _ptr(out_d2),
+                        _stream(stream)), "csd_knn")
+
+
+def synchronize(stream=None):
+    check(lib().csd_stream_synchronize(_stream(stream)), "csd_stream_synchronize")

@@ -1,0 +1,224 @@
+// csd_check.hip -- fused forward kinematics + collision checking for gfx950.
+//
+// One lane owns one configuration end to end: FK of the rigid bodies, world
+// positions of the moving spheres in VGPRs, then every collision pair of the
+// plant, with a wave-level early exit once all 64 lanes have found a
+// collision. Nothing is materialised in HBM except the 1-byte flag per
+// configuration (the reference writes 64 B/link of transforms and 1 B per
+// (config, pair): plant/cuda_collision_checker.cu:16-85).
+#include <string.h>
+
+#include "csd_collide.h"
+
+namespace csd {
+
+constexpr int kCheckBlock = 256;
+
+template <int VAR>
+__global__ __launch_bounds__(kCheckBlock) void k_check(PlantImage P, const float *__restrict__ q,
+                                                       int64_t n, int64_t sn, int64_t sj,
+                                                       VoxelArgs V, RobotGeomList R,
+                                                       uint8_t *__restrict__ out) {
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = gi < n;
+  const int64_t i = valid ? gi : n - 1;  // idle tail lanes re-read the last config
+  const QMem qa{q, i * sn, sj};
+  // idle lanes start "done" so they never hold the wave back
+  const bool col = collides_variant<VAR>(P, qa, V, R, !valid);
+  if (valid) out[gi] = col ? 0 : 1;
+}
+
+// per (config, voxel): 1 iff no robot geometry touches the voxel
+template <int NB>
+__global__ __launch_bounds__(kCheckBlock) void k_voxel_flags(PlantImage P,
+                                                             const float *__restrict__ q,
+                                                             int64_t n, int64_t sn, int64_t sj,
+                                                             VoxelArgs V, RobotGeomList R,
+                                                             uint8_t *__restrict__ out) {
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= n) return;
+  CSD_BODY_FRAMES(B, NB);
+  fk_bodies<NB>(P, QMem{q, gi * sn, sj}, B);
+  for (int v = 0; v < V.num; ++v) {
+    float Y[12];
+    aff_identity(Y);
+    Y[3] = V.vox[3 * v];
+    Y[7] = V.vox[3 * v + 1];
+    Y[11] = V.vox[3 * v + 2];
+    const float vd[4] = {V.radius, 0.f, 0.f, 0.f};
+    bool hit = false;
+    for (int r = 0; r < R.n; ++r) {
+      const DevGeom &g = P.geoms[R.ids[r]];
+      float X[12];
+      geom_world<NB>(g, B, X);
+      hit |= hit_posed(0, Y, vd, geom_rank(g.type), X, g.dims);
+    }
+    out[gi * V.num + v] = hit ? 0 : 1;
+  }
+}
+
+// standalone FK: transforms [N][L][16] column-major (the reference layout,
+// plant/cuda_forward_kinematics.cu:59-77)
+template <int NB>
+__global__ __launch_bounds__(kCheckBlock) void k_fk(PlantImage P, const float *__restrict__ q,
+                                                    int64_t n, int64_t sn, int64_t sj,
+                                                    float *__restrict__ tf) {
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= n) return;
+  CSD_BODY_FRAMES(B, NB);
+  fk_bodies<NB>(P, QMem{q, gi * sn, sj}, B);
+  for (int l = 0; l < P.num_links; ++l) {
+    const DevLink &lk = P.links[l];
+    float X[12];
+    if (lk.body < 0) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) X[k] = lk.O[k];
+    } else {
+      float Y[12];
+      B.get(lk.body, Y);
+      aff_mul(Y, lk.O, X);
+    }
+    float4 *o4 = reinterpret_cast<float4 *>(tf + (gi * P.num_links + l) * 16);
+    o4[0] = make_float4(X[0], X[4], X[8], 0.f);
+    o4[1] = make_float4(X[1], X[5], X[9], 0.f);
+    o4[2] = make_float4(X[2], X[6], X[10], 0.f);
+    o4[3] = make_float4(X[3], X[7], X[11], 1.f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side dispatch
+// ---------------------------------------------------------------------------
+namespace {
+int fail(int code, const char *msg) {
+  set_error(msg);
+  return code;
+}
+
+int check_args(const csd_plant *pl, const float *q, int64_t n, int64_t sn, int64_t sj,
+               const void *out) {
+  if (!pl) return fail(CSD_ERR_INVALID_ARGUMENT, "null plant");
+  if (n < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative configuration count");
+  if (n > 0 && (!q || !out)) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  if (n > 0 && (sn < 0 || sj < 0)) return fail(CSD_ERR_INVALID_ARGUMENT, "negative stride");
+  if (pl->img.num_bodies > 16) return fail(CSD_ERR_CAPACITY, "more than 16 moving joints");
+  return CSD_OK;
+}
+
+dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kCheckBlock - 1) / kCheckBlock)); }
+}  // namespace
+
+bool fill_robot_list(const csd_plant *pl, const int32_t *robot, int32_t nr, RobotGeomList &R,
+                     VoxelArgs &V, int *status) {
+  memset(&R, 0, sizeof(R));
+  V.mask0 = V.mask1 = 0;
+  *status = CSD_OK;
+  if (nr < 0 || nr > kMaxRobotGeoms) {
+    *status = fail(CSD_ERR_CAPACITY, "robot geometry list longer than 128");
+    return false;
+  }
+  if (nr > 0 && !robot) {
+    *status = fail(CSD_ERR_INVALID_ARGUMENT, "null robot geometry list");
+    return false;
+  }
+  bool all_slots = true;
+  R.n = nr;
+  for (int k = 0; k < nr; ++k) {
+    int g = robot[k];
+    if (g < 0 || g >= (int)pl->geom_type.size()) {
+      *status = fail(CSD_ERR_INVALID_ARGUMENT, "robot geometry index out of range");
+      return false;
+    }
+    R.ids[k] = g;
+    int s = pl->geom_slot[g];
+    if (s < 0)
+      all_slots = false;
+    else if (s < 64)
+      V.mask0 |= (uint64_t)1 << s;
+    else
+      V.mask1 |= (uint64_t)1 << (s - 64);
+  }
+  return all_slots;
+}
+
+int launch_check(const csd_plant *pl, const float *q, int64_t n, int64_t sn, int64_t sj,
+                 const float *vox, int32_t nvox, float vr, const int32_t *robot, int32_t nr,
+                 uint8_t *out, hipStream_t st) {
+  int rc = check_args(pl, q, n, sn, sj, out);
+  if (rc) return rc;
+  if (n == 0) return CSD_OK;
+  if (nvox < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative voxel count");
+  if (nvox > 0 && !vox) return fail(CSD_ERR_INVALID_ARGUMENT, "null voxel array");
+  RobotGeomList R;
+  VoxelArgs V{vox, nvox, vr, 0, 0};
+  bool all_slots = fill_robot_list(pl, robot, nvox > 0 ? nr : 0, R, V, &rc);
+  if (rc) return rc;
+  const int var = check_variant(pl->img, nvox > 0 && !all_slots);
+  CSD_DISPATCH_VARIANT(var, k_check, grid_for(n), dim3(kCheckBlock), 0, st, pl->img, q, n, sn,
+                       sj, V, R, out);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+}  // namespace csd
+
+using namespace csd;
+
+extern "C" {
+
+int csd_check_collision_free(const csd_plant *plant, const float *q, int64_t n,
+                             int64_t stride_n, int64_t stride_j, uint8_t *out, void *stream) {
+  return launch_check(plant, q, n, stride_n, stride_j, nullptr, 0, 0.f, nullptr, 0, out,
+                      (hipStream_t)stream);
+}
+
+int csd_check_collision_free_voxels(const csd_plant *plant, const float *q, int64_t n,
+                                    int64_t stride_n, int64_t stride_j, const float *voxels,
+                                    int32_t num_voxels, float voxel_radius,
+                                    const int32_t *robot_geoms, int32_t num_robot_geoms,
+                                    uint8_t *out, void *stream) {
+  return launch_check(plant, q, n, stride_n, stride_j, voxels, num_voxels, voxel_radius,
+                      robot_geoms, num_robot_geoms, out, (hipStream_t)stream);
+}
+
+int csd_voxel_flags(const csd_plant *plant, const float *q, int64_t n, int64_t stride_n,
+                    int64_t stride_j, const float *voxels, int32_t num_voxels,
+                    float voxel_radius, const int32_t *robot_geoms, int32_t num_robot_geoms,
+                    uint8_t *out, void *stream) {
+  int rc = check_args(plant, q, n, stride_n, stride_j, out);
+  if (rc) return rc;
+  if (num_voxels < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative voxel count");
+  if (n == 0 || num_voxels == 0) return CSD_OK;
+  if (!voxels) return fail(CSD_ERR_INVALID_ARGUMENT, "null voxel array");
+  RobotGeomList R;
+  VoxelArgs V{voxels, num_voxels, voxel_radius, 0, 0};
+  fill_robot_list(plant, robot_geoms, num_robot_geoms, R, V, &rc);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (plant->img.num_bodies <= 8)
+    hipLaunchKernelGGL((k_voxel_flags<8>), grid_for(n), dim3(kCheckBlock), 0, st, plant->img, q,
+                       n, stride_n, stride_j, V, R, out);
+  else
+    hipLaunchKernelGGL((k_voxel_flags<16>), grid_for(n), dim3(kCheckBlock), 0, st, plant->img,
+                       q, n, stride_n, stride_j, V, R, out);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+int csd_forward_kinematics(const csd_plant *plant, const float *q, int64_t n, int64_t stride_n,
+                           int64_t stride_j, float *transforms, void *stream) {
+  int rc = check_args(plant, q, n, stride_n, stride_j, transforms);
+  if (rc) return rc;
+  if (n == 0) return CSD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (plant->img.num_bodies <= 8)
+    hipLaunchKernelGGL((k_fk<8>), grid_for(n), dim3(kCheckBlock), 0, st, plant->img, q, n,
+                       stride_n, stride_j, transforms);
+  else
+    hipLaunchKernelGGL((k_fk<16>), grid_for(n), dim3(kCheckBlock), 0, st, plant->img, q, n,
+                       stride_n, stride_j, transforms);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+}  // extern "C"

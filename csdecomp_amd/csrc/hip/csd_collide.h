@@ -1,0 +1,222 @@
+// csd_collide.h -- "is this configuration in collision?" as device functions,
+// shared by the batched checker, the EI-ZO sampler/bisection kernels and the
+// edge validator. One lane evaluates one configuration.
+#pragma once
+#include "csd_device.h"
+
+namespace csd {
+
+constexpr int kMaxRobotGeoms = 128;
+
+// robot geometries checked against voxels, passed by value (kernarg segment)
+struct RobotGeomList {
+  int32_t n;
+  int32_t pad[3];
+  int32_t ids[kMaxRobotGeoms];
+};
+
+struct VoxelArgs {
+  const float *vox;  // [V][3] device
+  int32_t num;
+  float radius;
+  uint64_t mask0, mask1;  // fast path: robot sphere slots
+};
+
+// Wave-level early exit. ANY = false: every lane of the wave has found a
+// collision (independent configurations). ANY = true: some lane has (the
+// wave evaluates points of one edge and only "any collision" matters; the
+// per-lane results are then incomplete by design).
+template <bool ANY>
+__device__ __forceinline__ bool wave_done(bool col) {
+  if constexpr (ANY) return __any(col);
+  else return __all(col);
+}
+
+// Fast path: every moving geometry is a sphere (PlantImage::fast).
+// `col` enters as the lane's initial state (true = nothing left to decide,
+// e.g. an idle tail lane) and is returned OR-ed with the result.
+template <int NB, int KM, bool ANY = false, class Q>
+__device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
+                                              const VoxelArgs &V, bool col) {
+  constexpr int NS = NB * KM;
+  CSD_BODY_FRAMES(B, NB);
+  fk_bodies<NB>(P, qa, B);
+
+  CSD_SLOT_ARRAY(cx, NS);
+  CSD_SLOT_ARRAY(cy, NS);
+  CSD_SLOT_ARRAY(cz, NS);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    cx.at(s) = cy.at(s) = cz.at(s) = 0.f;
+    if (s / KM < P.num_bodies) {
+      const float4 sp = P.slot_sphere[s];
+      if (sp.w >= 0.f) {
+        float X[12];
+        B.get(s / KM, X);
+        aff_point(X, sp.x, sp.y, sp.z, cx.at(s), cy.at(s), cz.at(s));
+      }
+    }
+  }
+  col = col || P.always_colliding;
+
+  // moving sphere x moving sphere (self collision): cheap, test first
+  for (int p = 0; p < P.num_self_pairs; ++p) {
+    if ((p & 31) == 0 && wave_done<ANY>(col)) break;
+    const int4 sp = P.self_pairs[p];
+    col |= hit_sphere_sphere(cx.at(sp.x), cy.at(sp.x), cz.at(sp.x), cx.at(sp.y), cy.at(sp.y),
+                             cz.at(sp.y), __int_as_float(sp.z));
+  }
+
+  // moving sphere x world-fixed obstacle
+  for (int o = 0; o < P.num_obstacles; ++o) {
+    if (wave_done<ANY>(col)) break;
+    const int t = P.obstacle_type[o];
+    const float *ov = P.obstacles[o].v;
+    const uint64_t m0 = P.obstacle_mask[2 * o], m1 = P.obstacle_mask[2 * o + 1];
+    if (t == CSD_BOX) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
+        if (on) {
+          const float r = P.slot_sphere[s].w;
+          col |= hit_sphere_obox(cx.at(s), cy.at(s), cz.at(s), r * r, ov);
+        }
+      }
+    } else if (t == CSD_SPHERE) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
+        if (on) {
+          const float rs = P.slot_sphere[s].w + ov[3];
+          col |= hit_sphere_sphere(cx.at(s), cy.at(s), cz.at(s), ov[0], ov[1], ov[2], rs * rs);
+        }
+      }
+    } else {  // capsule
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
+        if (on) {
+          const float rs = P.slot_sphere[s].w + ov[7];
+          col |= seg_point_d2(cx.at(s), cy.at(s), cz.at(s), ov[0], ov[1], ov[2], ov[3], ov[4],
+                              ov[5], ov[6]) < rs * rs;
+        }
+      }
+    }
+  }
+
+  // robot spheres x voxel spheres (executeCollisionFreeVoxelsKernel)
+  for (int v = 0; v < V.num; ++v) {
+    if (wave_done<ANY>(col)) break;
+    const float vx = V.vox[3 * v], vy = V.vox[3 * v + 1], vz = V.vox[3 * v + 2];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const bool on = s < 64 ? ((V.mask0 >> s) & 1) : ((V.mask1 >> (s - 64)) & 1);
+      if (on) {
+        const float rs = V.radius + P.slot_sphere[s].w;
+        col |= hit_sphere_sphere(vx, vy, vz, cx.at(s), cy.at(s), cz.at(s), rs * rs);
+      }
+    }
+  }
+  return col;
+}
+
+// Generic path: any sphere / box / capsule pair, full 3x4 geometry poses.
+template <int NB, bool ANY = false, class Q>
+__device__ __forceinline__ bool collides_generic(const PlantImage &P, const Q &qa,
+                                                 const VoxelArgs &V, const RobotGeomList &R,
+                                                 bool col) {
+  CSD_BODY_FRAMES(B, NB);
+  fk_bodies<NB>(P, qa, B);
+  col = col || P.always_colliding;
+  for (int p = 0; p < P.num_pairs; ++p) {
+    if ((p & 7) == 0 && wave_done<ANY>(col)) break;
+    const DevGeom &ga = P.geoms[P.pairs[2 * p]];
+    const DevGeom &gb = P.geoms[P.pairs[2 * p + 1]];
+    float XA[12], XB[12];
+    geom_world<NB>(ga, B, XA);
+    geom_world<NB>(gb, B, XB);
+    col |= hit_posed(geom_rank(ga.type), XA, ga.dims, geom_rank(gb.type), XB, gb.dims);
+  }
+  for (int r = 0; r < R.n; ++r) {
+    if (wave_done<ANY>(col)) break;
+    const DevGeom &g = P.geoms[R.ids[r]];
+    float X[12];
+    geom_world<NB>(g, B, X);
+    const int rk = geom_rank(g.type);
+    for (int v = 0; v < V.num; ++v) {
+      float Y[12];
+      aff_identity(Y);
+      Y[3] = V.vox[3 * v];
+      Y[7] = V.vox[3 * v + 1];
+      Y[11] = V.vox[3 * v + 2];
+      const float vd[4] = {V.radius, 0.f, 0.f, 0.f};
+      col |= hit_posed(0, Y, vd, rk, X, g.dims);
+    }
+  }
+  return col;
+}
+
+// Kernel-variant selector shared by every launcher: which template the plant
+// needs. 0..3 fast <8,4>,<8,8>,<16,4>,<16,8>; 4,5 generic <8>,<16>.
+enum CheckVariant { kFast8x4 = 0, kFast8x8, kFast16x4, kFast16x8, kGeneric8, kGeneric16 };
+
+inline int check_variant(const PlantImage &P, bool need_generic) {
+  if (P.fast && !need_generic) {
+    const int nbt = P.nslots / P.kmax;
+    if (nbt == 8) return P.kmax == 4 ? kFast8x4 : kFast8x8;
+    return P.kmax == 4 ? kFast16x4 : kFast16x8;
+  }
+  return P.num_bodies <= 8 ? kGeneric8 : kGeneric16;
+}
+
+// dispatch a device-side call on the variant (VAR is a compile-time value
+// inside the generic lambda-free macro body)
+template <int VAR, bool ANY = false, class Q>
+__device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &qa,
+                                                 const VoxelArgs &V, const RobotGeomList &R,
+                                                 bool col) {
+  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY>(P, qa, V, col);
+  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY>(P, qa, V, col);
+  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY>(P, qa, V, col);
+  else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY>(P, qa, V, col);
+  else if constexpr (VAR == kGeneric8) return collides_generic<8, ANY>(P, qa, V, R, col);
+  else return collides_generic<16, ANY>(P, qa, V, R, col);
+}
+
+// host: launch KERNEL<VAR> for the runtime variant
+#define CSD_DISPATCH_VARIANT(var, KERNEL, grid, block, shmem, stream, ...)                 \
+  do {                                                                                    \
+    switch (var) {                                                                        \
+      case ::csd::kFast8x4:                                                               \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast8x4>), grid, block, shmem, stream,        \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kFast8x8:                                                               \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast8x8>), grid, block, shmem, stream,        \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kFast16x4:                                                              \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast16x4>), grid, block, shmem, stream,       \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kFast16x8:                                                              \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast16x8>), grid, block, shmem, stream,       \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kGeneric8:                                                              \
+        hipLaunchKernelGGL((KERNEL<::csd::kGeneric8>), grid, block, shmem, stream,       \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      default:                                                                            \
+        hipLaunchKernelGGL((KERNEL<::csd::kGeneric16>), grid, block, shmem, stream,      \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+    }                                                                                     \
+  } while (0)
+
+// host: robot geometry list and voxel slot masks; returns false if a robot
+// geometry is not a fast-path sphere slot
+bool fill_robot_list(const csd_plant *pl, const int32_t *robot, int32_t nr, RobotGeomList &R,
+                     VoxelArgs &V, int *status);
+
+}  // namespace csd

@@ -1,0 +1,284 @@
+// csd_edges.hip -- consumers of the collision kernel for roadmaps:
+// edge validation (configurations generated in-lane, never stored), swept
+// voxel occupancy, brute-force k nearest neighbours.
+#include <math.h>
+#include <string.h>
+
+#include "csd_collide.h"
+
+namespace csd {
+
+constexpr int kEdgeBlock = 256;
+constexpr int kEdgeDim = 16;
+
+// sequential float sum of squares without contraction, the order the CPU
+// restatement uses (so step counts agree bit for bit)
+__device__ __forceinline__ float norm_diff_rn(const float *a, const float *b, int dim) {
+  float s = 0.f;
+#pragma unroll
+  for (int d = 0; d < kEdgeDim; ++d)
+    if (d < dim) {
+      const float e = __fsub_rn(b[d], a[d]);
+      s = __fadd_rn(s, __fmul_rn(e, e));
+    }
+  return sqrtf(s);
+}
+
+// One wave per edge (grid-stride over edges); lane k checks point k + 64 j.
+// rule 0: checkEdgeCollisionFree, n = ceil(|q2-q1|/step), t = k/n, k < n
+//         (plant/collision_checker.cpp:41-53)
+// rule 1: VisibilityGraph, n = (uint)(|q2-q1|/step + 2) points: q1, q2 and
+//         t = (k+1)/(n-1) for k < n-2 (planning/cuda_visibility_graph.cu:58-172)
+template <int VAR>
+__global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
+    PlantImage P, const float *__restrict__ nodes, int dim, const int32_t *__restrict__ edges,
+    int64_t E, float step, int rule, uint8_t *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const VoxelArgs V{nullptr, 0, 0.f, 0, 0};
+  RobotGeomList R;
+  R.n = 0;
+  for (int64_t e = wave; e < E; e += nwaves) {
+    const int a = edges[2 * e], b = edges[2 * e + 1];
+    float q1[kEdgeDim], q2[kEdgeDim];
+#pragma unroll
+    for (int d = 0; d < kEdgeDim; ++d) {
+      q1[d] = d < dim ? nodes[(size_t)a * dim + d] : 0.f;
+      q2[d] = d < dim ? nodes[(size_t)b * dim + d] : 0.f;
+    }
+    const float dist = norm_diff_rn(q1, q2, dim);
+    int64_t npts;
+    float tstep = 0.f;
+    if (rule == 0) {
+      npts = (int64_t)ceilf(dist / step);
+    } else {
+      npts = (int64_t)(uint32_t)(dist / step + 2);
+      tstep = (float)(1.0 / (double)(npts - 1));
+    }
+    bool any = false;
+    for (int64_t base = 0; base < npts && !any; base += 64) {
+      const int64_t k = base + lane;
+      const bool valid = k < npts;
+      float t;
+      if (rule == 0) {
+        t = (float)((double)k / (1.0 * (double)npts));
+      } else {
+        t = k == 0 ? 0.f : (k == 1 ? 1.f : __fmul_rn(tstep, (float)(k - 1)));
+      }
+      float q[kEdgeDim];
+#pragma unroll
+      for (int d = 0; d < kEdgeDim; ++d)
+        q[d] = __fadd_rn(__fmul_rn(t, q2[d]), __fmul_rn(__fsub_rn(1.f, t), q1[d]));
+      const bool col = collides_variant<VAR, true>(P, QReg{q}, V, R, !valid) && valid;
+      any = __any(col);
+    }
+    if (lane == 0) out[e] = any ? 0 : 1;
+  }
+}
+
+// Swept occupancy: every robot geometry at every configuration marks the
+// voxels it touches (same voxel test as geomToVoxelCollisionFree, GPU rule).
+// Voxel centres follow GetCollisionVoxelCenter (roadmap_builder.cpp:709-741).
+struct GridArgs {
+  float center[3], size[3];
+  float res, radius;
+  int nx, ny, nz;
+};
+
+__device__ __forceinline__ float voxel_coord(float c, float size, int i, float res) {
+  return (float)((double)c - (double)size / 2.0 + (double)(float)(i * 2 * res) + (double)res);
+}
+
+template <int NB>
+__global__ __launch_bounds__(kEdgeBlock) void k_occupancy(PlantImage P, const float *__restrict__ q,
+                                                          int64_t n, int64_t sn, int64_t sj,
+                                                          RobotGeomList R, GridArgs G,
+                                                          uint8_t *__restrict__ grid) {
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= n) return;
+  CSD_BODY_FRAMES(B, NB);
+  fk_bodies<NB>(P, QMem{q, gi * sn, sj}, B);
+  const float two_res = 2.f * G.res;
+  const float vd[4] = {G.radius, 0.f, 0.f, 0.f};
+  for (int r = 0; r < R.n; ++r) {
+    const DevGeom &g = P.geoms[R.ids[r]];
+    float X[12];
+    geom_world<NB>(g, B, X);
+    float ext[3];
+    const int rk = geom_rank(g.type);
+    for (int k = 0; k < 3; ++k) {
+      if (rk == 0) ext[k] = g.dims[0];
+      else if (rk == 1)
+        ext[k] = fabsf(X[k * 4 + 0]) * g.dims[0] + fabsf(X[k * 4 + 1]) * g.dims[1] +
+                 fabsf(X[k * 4 + 2]) * g.dims[2];
+      else ext[k] = g.dims[0] + fabsf(X[k * 4 + 2]) * g.dims[1];
+      ext[k] = ext[k] * 1.0001f + G.radius + 1e-6f;
+    }
+    int lo[3], hi[3];
+    const int nn[3] = {G.nx, G.ny, G.nz};
+    for (int k = 0; k < 3; ++k) {
+      const float origin = G.center[k] - 0.5f * G.size[k] + G.res;
+      const float p = X[k * 4 + 3];
+      lo[k] = max(0, (int)floorf((p - ext[k] - origin) / two_res) - 1);
+      hi[k] = min(nn[k] - 1, (int)ceilf((p + ext[k] - origin) / two_res) + 1);
+    }
+    for (int iz = lo[2]; iz <= hi[2]; ++iz)
+      for (int iy = lo[1]; iy <= hi[1]; ++iy)
+        for (int ix = lo[0]; ix <= hi[0]; ++ix) {
+          float Y[12];
+          aff_identity(Y);
+          Y[3] = voxel_coord(G.center[0], G.size[0], ix, G.res);
+          Y[7] = voxel_coord(G.center[1], G.size[1], iy, G.res);
+          Y[11] = voxel_coord(G.center[2], G.size[2], iz, G.res);
+          if (hit_posed(0, Y, vd, rk, X, g.dims))
+            grid[(size_t)ix + (size_t)iy * G.nx + (size_t)iz * G.nx * G.ny] = 1;
+        }
+  }
+}
+
+// k nearest neighbours by squared L2 distance (sequential, uncontracted
+// float sum); candidates with d2 <= FLT_EPSILON (the node itself and exact
+// duplicates) are skipped as in roadmap_builder.cpp:190-197; ties by index.
+constexpr int kKnnMax = 16;
+__global__ __launch_bounds__(256) void k_knn(const float *__restrict__ nodes, int N, int dim,
+                                             int k, int32_t *__restrict__ out_idx,
+                                             float *__restrict__ out_d2) {
+  __shared__ float tile[256 * kEdgeDim];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = i < N;
+  float qv[kEdgeDim];
+#pragma unroll
+  for (int d = 0; d < kEdgeDim; ++d) qv[d] = (valid && d < dim) ? nodes[(size_t)i * dim + d] : 0.f;
+  float bd[kKnnMax];
+  int bi[kKnnMax];
+#pragma unroll
+  for (int s = 0; s < kKnnMax; ++s) {
+    bd[s] = INFINITY;
+    bi[s] = 0x7fffffff;
+  }
+  for (int t0 = 0; t0 < N; t0 += 256) {
+    const int cnt = min(256, N - t0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < cnt * dim; e += 256) tile[e] = nodes[(size_t)t0 * dim + e];
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      float s = 0.f;
+#pragma unroll
+      for (int d = 0; d < kEdgeDim; ++d)
+        if (d < dim) {
+          const float e = __fsub_rn(tile[j * dim + d], qv[d]);
+          s = __fadd_rn(s, __fmul_rn(e, e));
+        }
+      const int jj = t0 + j;
+      const float kth = bd[kKnnMax - 1];
+      if (s > 1.1920928955078125e-07f && (s < kth || (s == kth && jj < bi[kKnnMax - 1]))) {
+        // insertion into the sorted list (static indices)
+        float cd = s;
+        int ci = jj;
+#pragma unroll
+        for (int p = 0; p < kKnnMax; ++p) {
+          const bool lt = cd < bd[p] || (cd == bd[p] && ci < bi[p]);
+          const float td = bd[p];
+          const int ti = bi[p];
+          bd[p] = lt ? cd : td;
+          bi[p] = lt ? ci : ti;
+          cd = lt ? td : cd;
+          ci = lt ? ti : ci;
+        }
+      }
+    }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int s = 0; s < kKnnMax; ++s)
+    if (s < k) {
+      out_idx[(size_t)i * k + s] = bi[s] == 0x7fffffff ? -1 : bi[s];
+      if (out_d2) out_d2[(size_t)i * k + s] = bd[s];
+    }
+}
+
+namespace {
+int fail(int code, const char *msg) {
+  set_error(msg);
+  return code;
+}
+}  // namespace
+
+}  // namespace csd
+
+using namespace csd;
+
+extern "C" {
+
+int csd_validate_edges(const csd_plant *plant, const float *nodes, int32_t num_nodes,
+                       const int32_t *edges, int64_t num_edges, float step, int32_t rule,
+                       uint8_t *out, void *stream) {
+  if (!plant) return fail(CSD_ERR_INVALID_ARGUMENT, "null plant");
+  const int dim = plant->img.dof;
+  if (dim > kEdgeDim) return fail(CSD_ERR_CAPACITY, "edge validation supports up to 16 dims");
+  if (plant->img.num_bodies > 16) return fail(CSD_ERR_CAPACITY, "more than 16 moving joints");
+  if (num_edges < 0 || num_nodes < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative size");
+  if (!(step > 0.f)) return fail(CSD_ERR_INVALID_ARGUMENT, "step must be positive");
+  if (rule != 0 && rule != 1) return fail(CSD_ERR_INVALID_ARGUMENT, "rule must be 0 or 1");
+  if (num_edges == 0) return CSD_OK;
+  if (!nodes || !edges || !out) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  const int var = check_variant(plant->img, false);
+  // persistent grid: enough waves to fill 256 CUs several times over
+  const int64_t waves = std::min<int64_t>(num_edges, 256 * 64);
+  dim3 g((unsigned)((waves * 64 + kEdgeBlock - 1) / kEdgeBlock));
+  CSD_DISPATCH_VARIANT(var, k_validate_edges, g, dim3(kEdgeBlock), 0, (hipStream_t)stream,
+                       plant->img, nodes, dim, edges, num_edges, step, rule, out);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+int csd_occupancy(const csd_plant *plant, const float *q, int64_t n, int64_t stride_n,
+                  int64_t stride_j, const int32_t *robot_geoms, int32_t num_robot_geoms,
+                  const float *map_center, const float *map_size, float resolution,
+                  uint8_t *grid, void *stream) {
+  if (!plant || !map_center || !map_size) return fail(CSD_ERR_INVALID_ARGUMENT, "null argument");
+  if (plant->img.num_bodies > 16) return fail(CSD_ERR_CAPACITY, "more than 16 moving joints");
+  if (n < 0 || !(resolution > 0.f)) return fail(CSD_ERR_INVALID_ARGUMENT, "bad size");
+  if (n == 0 || num_robot_geoms == 0) return CSD_OK;
+  if (!q || !grid) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  RobotGeomList R;
+  VoxelArgs V{nullptr, 0, 0.f, 0, 0};
+  int rc = CSD_OK;
+  fill_robot_list(plant, robot_geoms, num_robot_geoms, R, V, &rc);
+  if (rc) return rc;
+  GridArgs G;
+  for (int k = 0; k < 3; ++k) {
+    G.center[k] = map_center[k];
+    G.size[k] = map_size[k];
+  }
+  G.res = resolution;
+  G.radius = (float)(sqrt(3.0) * resolution);  // GetOfflineVoxelRadius, roadmap.h:68
+  G.nx = (int)ceilf(map_size[0] / (2 * resolution));
+  G.ny = (int)ceilf(map_size[1] / (2 * resolution));
+  G.nz = (int)ceilf(map_size[2] / (2 * resolution));
+  dim3 g((unsigned)((n + kEdgeBlock - 1) / kEdgeBlock));
+  if (plant->img.num_bodies <= 8)
+    hipLaunchKernelGGL((k_occupancy<8>), g, dim3(kEdgeBlock), 0, (hipStream_t)stream,
+                       plant->img, q, n, stride_n, stride_j, R, G, grid);
+  else
+    hipLaunchKernelGGL((k_occupancy<16>), g, dim3(kEdgeBlock), 0, (hipStream_t)stream,
+                       plant->img, q, n, stride_n, stride_j, R, G, grid);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+int csd_knn(const float *nodes, int32_t num_nodes, int32_t dim, int32_t k, int32_t *out_idx,
+            float *out_d2, void *stream) {
+  if (dim < 1 || dim > kEdgeDim) return fail(CSD_ERR_CAPACITY, "k-NN supports 1..16 dims");
+  if (k < 1 || k > kKnnMax) return fail(CSD_ERR_CAPACITY, "k must be in 1..16");
+  if (num_nodes < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative size");
+  if (num_nodes == 0) return CSD_OK;
+  if (!nodes || !out_idx) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  hipLaunchKernelGGL(k_knn, dim3((num_nodes + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     nodes, num_nodes, dim, k, out_idx, out_d2);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+}  // extern "C"

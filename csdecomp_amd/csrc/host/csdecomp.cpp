@@ -1,0 +1,1038 @@
+// csdecomp.cpp -- host side of the csdecomp-compatible API. Plant assembly,
+// polytope bookkeeping and argument checking live here; every numerical
+// evaluation of the hot path is a call through include/csdecomp_amd.h.
+#include "csdecomp.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <queue>
+#include <stack>
+
+namespace csdecomp {
+
+Matrix4f identity4() {
+  Matrix4f m{};
+  m[0] = m[5] = m[10] = m[15] = 1.f;
+  return m;
+}
+
+void check_status(int rc, const char *what) {
+  if (rc == CSD_OK) return;
+  std::string msg = std::string("[csdecomp_amd] ") + what + ": " + csd_status_string(rc);
+  const char *detail = csd_last_error();
+  if (detail && *detail) msg += std::string(" (") + detail + ")";
+  if (rc == CSD_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+void ensure_device() {
+  static std::once_flag once;
+  static int rc = CSD_OK;
+  static std::string err;
+  std::call_once(once, [] {
+    rc = csd_init(0);
+    if (rc != CSD_OK) err = csd_last_error();
+  });
+  if (rc != CSD_OK)
+    throw std::runtime_error("[csdecomp_amd] no usable gfx950 GPU: " + err +
+                             " (this library has no CPU fallback)");
+}
+
+namespace {
+
+// ---- device scratch: grows, never shrinks, one set per host thread ----
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t cap = 0;
+  ~DevBuf() {
+    if (ptr) csd_free(ptr);
+  }
+  void *get(size_t bytes) {
+    if (bytes > cap) {
+      if (ptr) csd_free(ptr);
+      ptr = nullptr;
+      cap = 0;
+      check_status(csd_malloc(&ptr, bytes), "device allocation");
+      cap = bytes;
+    }
+    return ptr;
+  }
+};
+struct Scratch {
+  DevBuf q, out, aux, aux2, aux3;
+};
+Scratch &scratch() {
+  static thread_local Scratch s;
+  return s;
+}
+
+void require(bool ok, const std::string &msg) {
+  if (!ok) throw std::invalid_argument("[csdecomp_amd] " + msg);
+}
+
+size_t fnv(size_t h, const void *p, size_t n) {
+  const unsigned char *c = (const unsigned char *)p;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+}  // namespace
+
+// ===========================================================================
+// KinematicTree
+// ===========================================================================
+void KinematicTree::addLink(const Link &link) {
+  if (is_finalized_) throw std::runtime_error("Cannot add link to finalized tree.");
+  if (link_name_to_index_.count(link.name))
+    throw std::runtime_error("Duplicate link name detected in URDF. Name : " + link.name);
+  link_name_to_index_[link.name] = (int)links_.size();
+  links_.push_back(link);
+}
+
+void KinematicTree::addJoint(const Joint &joint) {
+  if (is_finalized_) throw std::runtime_error("Cannot add joint to finalized tree.");
+  if (joint_name_to_index_.count(joint.name))
+    throw std::runtime_error("Duplicate joint name detected in URDF.");
+  if (joint.parent_link < 0 || joint.child_link < 0 || joint.parent_link >= (int)links_.size() ||
+      joint.child_link >= (int)links_.size())
+    throw std::runtime_error("Parent or child link not found for joint: " + joint.name);
+  int idx = (int)joints_.size();
+  joint_name_to_index_[joint.name] = idx;
+  joints_.push_back(joint);
+  links_[joint.parent_link].child_joints.push_back(idx);
+  links_[joint.child_link].parent_joint = idx;
+}
+
+void KinematicTree::finalize() {
+  is_finalized_ = true;
+  leaf_link_indices_.clear();
+  joint_idx_to_config_idx_.assign(joints_.size(), 0);
+  traversal_.clear();
+  lower_.clear();
+  upper_.clear();
+  num_configuration_variables_ = 0;
+  for (int i = 0; i < (int)links_.size(); ++i)
+    if (links_[i].child_joints.empty()) leaf_link_indices_.push_back(i);
+  for (size_t j = 0; j < joints_.size(); ++j) {
+    if (joints_[j].type != FIXED) {
+      joint_idx_to_config_idx_[j] = num_configuration_variables_++;
+      lower_.push_back(joints_[j].position_lower_limit);
+      upper_.push_back(joints_[j].position_upper_limit);
+    } else {
+      joint_idx_to_config_idx_[j] = -1;
+    }
+  }
+  if (links_.empty()) return;
+  // depth first from the root with an explicit stack: children are pushed in
+  // order, so the last child is visited first (kinematic_tree.cpp:117-132)
+  std::stack<int> todo;
+  for (int cj : links_[0].child_joints) todo.push(cj);
+  while (!todo.empty()) {
+    int j = todo.top();
+    todo.pop();
+    traversal_.push_back(j);
+    for (int cj : links_[joints_[j].child_link].child_joints) todo.push(cj);
+  }
+}
+
+int KinematicTree::getLinkIndexByName(const std::string &name) const {
+  auto it = link_name_to_index_.find(name);
+  if (it == link_name_to_index_.end())
+    throw std::runtime_error("[KinematicTree] Invalid link name " + name);
+  return it->second;
+}
+
+std::string KinematicTree::getLinkNameByIndex(int index) const {
+  if (index < 0 || index >= (int)links_.size()) throw std::runtime_error("link index not found");
+  return links_[index].name;
+}
+
+const Link &KinematicTree::getLink(int index) const { return links_.at(index); }
+
+int KinematicTree::getJointIndex(const std::string &name) const {
+  auto it = joint_name_to_index_.find(name);
+  return it == joint_name_to_index_.end() ? -1 : it->second;
+}
+
+const Joint &KinematicTree::getJoint(int index) const { return joints_.at(index); }
+
+MinimalKinematicTree KinematicTree::getMyMinimalKinematicTreeStruct() const {
+  MinimalKinematicTree t;
+  t.num_configuration_variables = num_configuration_variables_;
+  t.num_joints = (int)joints_.size();
+  t.num_links = (int)links_.size();
+  t.joint_traversal_sequence = traversal_;
+  t.joint_idx_to_config_idx = joint_idx_to_config_idx_;
+  for (size_t l = 0; l < links_.size(); ++l) {
+    MinimalLink ml;
+    ml.index = (int)l;
+    ml.parent_joint_index = links_[l].parent_joint;
+    ml.child_joint_index_array = links_[l].child_joints;
+    ml.num_child_joints = (int)links_[l].child_joints.size();
+    t.links.push_back(ml);
+  }
+  for (size_t j = 0; j < joints_.size(); ++j) {
+    MinimalJoint mj;
+    mj.index = (int)j;
+    mj.type = joints_[j].type;
+    mj.X_PL_J = joints_[j].X_PL_J;
+    mj.parent_link = joints_[j].parent_link;
+    mj.child_link = joints_[j].child_link;
+    mj.axis = joints_[j].axis;
+    t.joints.push_back(mj);
+  }
+  return t;
+}
+
+std::vector<Matrix4f> KinematicTree::computeLinkFrameToWorldTransforms(
+    const std::vector<float> &q) const {
+  if (!is_finalized_) throw std::runtime_error("KinematicTree is not finalized");
+  if ((int)q.size() != num_configuration_variables_)
+    throw std::invalid_argument("Incorrect number of configuration variables");
+  return computeLinkFrameToWorldTransformsMinimal(q, getMyMinimalKinematicTreeStruct());
+}
+
+// ===========================================================================
+// scene
+// ===========================================================================
+std::vector<int32_t> SceneInspector::getCollisionPairMatrix() const {
+  std::vector<int32_t> m;
+  m.reserve(2 * robot_to_world_collision_pairs.size());
+  for (auto &p : robot_to_world_collision_pairs) {
+    m.push_back(p.first);
+    m.push_back(p.second);
+  }
+  return m;
+}
+
+namespace {
+// breadth-first path between two links over the undirected joint graph
+// (urdf_parser.cpp:38-71); returns link indices, empty if disconnected
+std::vector<int> link_path(const KinematicTree &t, int a, int b) {
+  int n = t.numLinks();
+  std::vector<int> parent(n, -1);
+  std::vector<char> seen(n, 0);
+  std::vector<std::vector<int>> adj(n);
+  for (const Joint &j : t.getJoints()) {
+    adj[j.parent_link].push_back(j.child_link);
+    adj[j.child_link].push_back(j.parent_link);
+  }
+  for (auto &v : adj) std::sort(v.begin(), v.end());
+  std::queue<int> q;
+  q.push(a);
+  seen[a] = 1;
+  while (!q.empty()) {
+    int c = q.front();
+    q.pop();
+    if (c == b) {
+      std::vector<int> path;
+      for (int v = b; v != -1; v = parent[v]) path.push_back(v);
+      std::reverse(path.begin(), path.end());
+      return path;
+    }
+    for (int nb : adj[c])
+      if (!seen[nb]) {
+        seen[nb] = 1;
+        parent[nb] = c;
+        q.push(nb);
+      }
+  }
+  return {};
+}
+
+int joint_between(const KinematicTree &t, int la, int lb) {
+  const auto &js = t.getJoints();
+  for (size_t j = 0; j < js.size(); ++j)
+    if ((js[j].parent_link == la && js[j].child_link == lb) ||
+        (js[j].parent_link == lb && js[j].child_link == la))
+      return (int)j;
+  return -1;
+}
+}  // namespace
+
+SceneInspector finalizeScene(const KinematicTree &tree, std::vector<CollisionGeometry> &geoms,
+                             const std::unordered_map<std::string, int> &names) {
+  SceneInspector insp;
+  insp.link_index_to_scene_collision_geometry_indices.assign(tree.numLinks(), {});
+  for (size_t g = 0; g < geoms.size(); ++g) {
+    if (geoms[g].link_index < 0 || geoms[g].link_index >= tree.numLinks())
+      throw std::runtime_error("collision geometry attached to an unknown link");
+    insp.link_index_to_scene_collision_geometry_indices[geoms[g].link_index].push_back((int)g);
+  }
+  // WORLD iff the link reaches the root through fixed joints only
+  for (auto &co : geoms) {
+    const Link &lk = tree.getLink(co.link_index);
+    if (lk.parent_joint < 0) {
+      co.group = WORLD;  // geometry on the root link itself
+      continue;
+    }
+    const Joint *prev = &tree.getJoint(lk.parent_joint);
+    while (true) {
+      if (prev->type != FIXED) {
+        co.group = ROBOT;
+        break;
+      }
+      if (prev->parent_link == 0) {
+        co.group = WORLD;
+        break;
+      }
+      int pj = tree.getLink(prev->parent_link).parent_joint;
+      if (pj == -1)
+        throw std::runtime_error("The kinematic tree appears to be disconnected.");
+      prev = &tree.getJoint(pj);
+    }
+  }
+  for (size_t g = 0; g < geoms.size(); ++g)
+    if (geoms[g].group == ROBOT) insp.robot_geometry_ids.push_back((GeometryIndex)g);
+
+  auto parent_link_of = [&](int link) {
+    int pj = tree.getLink(link).parent_joint;
+    return pj < 0 ? -1 : tree.getJoint(pj).parent_link;
+  };
+  for (size_t a = 0; a + 1 < geoms.size(); ++a) {
+    for (size_t b = a + 1; b < geoms.size(); ++b) {
+      if (geoms[a].group == WORLD && geoms[b].group == WORLD) continue;
+      int la = geoms[a].link_index, lb = geoms[b].link_index;
+      bool not_successive = parent_link_of(la) != lb && parent_link_of(lb) != la;
+      bool not_same = la != lb;
+      std::vector<int> path = link_path(tree, la, lb);
+      bool fixed_only = true;
+      for (size_t s = 0; s + 1 < path.size(); ++s) {
+        int j = joint_between(tree, path[s], path[s + 1]);
+        if (tree.getJoint(j).type != FIXED) {
+          fixed_only = false;
+          break;
+        }
+      }
+      if (not_successive && not_same && !fixed_only)
+        insp.robot_to_world_collision_pairs.emplace_back((int)a, (int)b);
+    }
+  }
+  insp.scene_collision_geometry_name_to_geometry_index = names;
+  return insp;
+}
+
+csd_plant *MinimalPlant::device() const {
+  ensure_device();
+  const MinimalKinematicTree &t = kin_tree;
+  require(t.num_links >= 1 && (int)t.links.size() >= t.num_links, "plant has no links");
+  require((int)t.joints.size() >= t.num_joints && (int)t.joint_traversal_sequence.size() >= t.num_joints &&
+              (int)t.joint_idx_to_config_idx.size() >= t.num_joints,
+          "kinematic tree arrays are shorter than num_joints");
+  require((int)scene_collision_geometries.size() >= num_scene_geometries,
+          "fewer geometries than num_scene_geometries");
+  require((int)collision_pairs_flat.size() >= 2 * num_collision_pairs,
+          "collision_pairs_flat shorter than 2*num_collision_pairs");
+  const int J = t.num_joints, G = num_scene_geometries, P = num_collision_pairs;
+  std::vector<int32_t> jt(J), jp(J), jc(J), jq(J), tr(J), gt(G), gl(G), gg(G);
+  std::vector<float> jx(16 * (size_t)J), ja(3 * (size_t)J), gd(3 * (size_t)G), gx(16 * (size_t)G);
+  for (int j = 0; j < J; ++j) {
+    const MinimalJoint &m = t.joints[j];
+    jt[j] = m.type;
+    jp[j] = m.parent_link;
+    jc[j] = m.child_link;
+    jq[j] = t.joint_idx_to_config_idx[j];
+    tr[j] = t.joint_traversal_sequence[j];
+    std::memcpy(&jx[16 * j], m.X_PL_J.data(), 64);
+    std::memcpy(&ja[3 * j], m.axis.data(), 12);
+  }
+  for (int g = 0; g < G; ++g) {
+    const CollisionGeometry &c = scene_collision_geometries[g];
+    gt[g] = c.type;
+    gl[g] = c.link_index;
+    gg[g] = c.group;
+    std::memcpy(&gd[3 * g], c.dimensions.data(), 12);
+    std::memcpy(&gx[16 * g], c.X_L_B.data(), 64);
+  }
+  size_t h = 1469598103934665603ull;
+  int hdr[5] = {t.num_links, J, t.num_configuration_variables, G, P};
+  h = fnv(h, hdr, sizeof(hdr));
+  for (auto *v : {&jt, &jp, &jc, &jq, &tr, &gt, &gl, &gg}) h = fnv(h, v->data(), v->size() * 4);
+  for (auto *v : {&jx, &ja, &gd, &gx}) h = fnv(h, v->data(), v->size() * 4);
+  h = fnv(h, collision_pairs_flat.data(), 8 * (size_t)P);
+  if (dev_ && h == dev_hash_) return dev_.get();
+  csd_plant_desc d;
+  d.num_links = t.num_links;
+  d.num_joints = J;
+  d.num_configuration_variables = t.num_configuration_variables;
+  d.joint_type = jt.data();
+  d.joint_parent_link = jp.data();
+  d.joint_child_link = jc.data();
+  d.joint_config_idx = jq.data();
+  d.traversal = tr.data();
+  d.joint_X_PL_J = jx.data();
+  d.joint_axis = ja.data();
+  d.num_geometries = G;
+  d.geom_type = gt.data();
+  d.geom_link = gl.data();
+  d.geom_group = gg.data();
+  d.geom_dims = gd.data();
+  d.geom_X_L_B = gx.data();
+  d.num_collision_pairs = P;
+  d.collision_pairs = collision_pairs_flat.data();
+  csd_plant *p = nullptr;
+  check_status(csd_plant_create(&d, &p), "plant upload");
+  dev_.reset(p, [](csd_plant *x) { csd_plant_destroy(x); });
+  dev_hash_ = h;
+  return p;
+}
+
+MinimalPlant Plant::getMinimalPlant() const {
+  MinimalPlant mp;
+  mp.kin_tree = kin_tree_.getMyMinimalKinematicTreeStruct();
+  mp.scene_collision_geometries = geoms_;
+  mp.collision_pairs_flat = inspector_.getCollisionPairMatrix();
+  mp.num_scene_geometries = (int)geoms_.size();
+  mp.num_collision_pairs = (int)inspector_.robot_to_world_collision_pairs.size();
+  return mp;
+}
+
+std::vector<std::string> Plant::getSceneCollisionGeometryNames() const {
+  std::vector<std::string> keys(inspector_.scene_collision_geometry_name_to_geometry_index.size());
+  for (auto &kv : inspector_.scene_collision_geometry_name_to_geometry_index)
+    if (kv.second >= 0 && kv.second < (int)keys.size()) keys[kv.second] = kv.first;
+  return keys;
+}
+
+int Plant::getSceneCollisionGeometryIndexByName(const std::string &name) const {
+  auto it = inspector_.scene_collision_geometry_name_to_geometry_index.find(name);
+  if (it == inspector_.scene_collision_geometry_name_to_geometry_index.end())
+    throw std::out_of_range("Geometry name not found: " + name);
+  return it->second;
+}
+
+// ===========================================================================
+// HPolyhedron (hpolyhedron.cpp)
+// ===========================================================================
+HPolyhedron::HPolyhedron(const MinimalHPolyhedron &m) : A_(m.num_faces, m.dim), dim_(m.dim) {
+  for (int r = 0; r < m.num_faces; ++r)
+    for (int c = 0; c < m.dim; ++c) A_(r, c) = m.A[(size_t)r * m.dim + c];
+  b_.assign(m.b.begin(), m.b.begin() + m.num_faces);
+}
+
+HPolyhedron::HPolyhedron(const MatrixXf &A, const std::vector<float> &b)
+    : A_(A), b_(b), dim_(A.cols) {
+  if ((int)b.size() != A.rows) throw std::invalid_argument("A and b dimensions do not match");
+}
+
+void HPolyhedron::MakeBox(const std::vector<float> &lo, const std::vector<float> &hi) {
+  if (lo.size() != hi.size())
+    throw std::invalid_argument("Lower and upper limits must have the same dimension");
+  int d = (int)lo.size();
+  dim_ = d;
+  A_ = MatrixXf(2 * d, d);
+  b_.assign(2 * d, 0.f);
+  for (int i = 0; i < d; ++i) {  // rows interleaved (-e_i, +e_i), hpolyhedron.cpp:104-122
+    A_(2 * i, i) = -1.f;
+    A_(2 * i + 1, i) = 1.f;
+    b_[2 * i] = -lo[i];
+    b_[2 * i + 1] = hi[i];
+  }
+}
+
+MinimalHPolyhedron HPolyhedron::GetMyMinimalHPolyhedron() const {
+  MinimalHPolyhedron m;
+  m.num_faces = A_.rows;
+  m.dim = A_.cols;
+  m.A.resize((size_t)m.num_faces * m.dim);
+  for (int r = 0; r < m.num_faces; ++r)
+    for (int c = 0; c < m.dim; ++c) m.A[(size_t)r * m.dim + c] = A_(r, c);
+  m.b = b_;
+  return m;
+}
+
+bool HPolyhedron::PointInSet(const std::vector<float> &x, float tol) const {
+  if ((int)x.size() != dim_)
+    throw std::invalid_argument("Point dimension does not match polyhedron dimension");
+  for (int r = 0; r < A_.rows; ++r) {
+    float s = 0.f;
+    for (int c = 0; c < dim_; ++c) s += A_(r, c) * x[c];
+    if (!(s <= b_[r] + tol)) return false;
+  }
+  return true;
+}
+
+namespace {
+// Dense two-phase simplex (Chvatal's auxiliary-variable start, Bland's rule):
+// maximise c.z s.t. M z <= h, z >= 0. Returns false if infeasible/unbounded.
+bool simplex_max(const std::vector<std::vector<double>> &M, const std::vector<double> &h,
+                 const std::vector<double> &c, std::vector<double> &z) {
+  const int m = (int)M.size(), n = (int)c.size();
+  // tableau columns: z (n), w (1), slack (m), rhs
+  const int W = n, S = n + 1, R = n + 1 + m, C = R + 1;
+  std::vector<std::vector<double>> T(m + 1, std::vector<double>(C, 0.0));
+  std::vector<int> basis(m);
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < n; ++j) T[i][j] = M[i][j];
+    T[i][W] = -1.0;
+    T[i][S + i] = 1.0;
+    T[i][R] = h[i];
+    basis[i] = S + i;
+  }
+  const double eps = 1e-10;
+  auto pivot = [&](int r, int col) {
+    double p = T[r][col];
+    for (double &v : T[r]) v /= p;
+    for (int i = 0; i <= m; ++i)
+      if (i != r && T[i][col] != 0.0) {
+        double f = T[i][col];
+        for (int k = 0; k < C; ++k) T[i][k] -= f * T[r][k];
+      }
+    basis[r] = col;
+  };
+  auto run = [&](int forbid) {
+    for (int it = 0; it < 50000; ++it) {
+      int col = -1;
+      for (int k = 0; k < R; ++k)
+        if (k != forbid && T[m][k] < -eps) {
+          col = k;
+          break;
+        }
+      if (col < 0) return true;
+      int r = -1;
+      double best = 0;
+      for (int i = 0; i < m; ++i)
+        if (T[i][col] > eps) {
+          double ratio = T[i][R] / T[i][col];
+          if (r < 0 || ratio < best - 1e-12 || (std::fabs(ratio - best) <= 1e-12 && basis[i] < basis[r])) {
+            r = i;
+            best = ratio;
+          }
+        }
+      if (r < 0) return false;  // unbounded
+      pivot(r, col);
+    }
+    return false;
+  };
+  // phase 1: minimise w (objective row stores -(max -w) coefficients)
+  int worst = 0;
+  for (int i = 1; i < m; ++i)
+    if (T[i][R] < T[worst][R]) worst = i;
+  if (m > 0 && T[worst][R] < 0) {
+    std::fill(T[m].begin(), T[m].end(), 0.0);
+    T[m][W] = 1.0;
+    pivot(worst, W);
+    if (!run(-1)) return false;
+    if (T[m][R] < -1e-7) return false;  // infeasible
+    for (int i = 0; i < m; ++i)
+      if (basis[i] == W) {
+        for (int k = 0; k < R; ++k)
+          if (k != W && std::fabs(T[i][k]) > eps) {
+            pivot(i, k);
+            break;
+          }
+      }
+  }
+  // phase 2
+  std::fill(T[m].begin(), T[m].end(), 0.0);
+  for (int j = 0; j < n; ++j) T[m][j] = -c[j];
+  for (int i = 0; i < m; ++i) {
+    int bcol = basis[i];
+    if (bcol < n && T[m][bcol] != 0.0) {
+      double f = T[m][bcol];
+      for (int k = 0; k < C; ++k) T[m][k] -= f * T[i][k];
+    }
+  }
+  for (int i = 0; i <= m; ++i) T[i][W] = 0.0;  // w stays at zero
+  if (!run(W)) return false;
+  z.assign(n, 0.0);
+  for (int i = 0; i < m; ++i)
+    if (basis[i] < n) z[basis[i]] = T[i][R];
+  return true;
+}
+}  // namespace
+
+std::vector<float> HPolyhedron::ChebyshevCenter() const {
+  // max r s.t. a_i x + |a_i| r <= b_i, r >= 0, x = xp - xn
+  const int m = A_.rows, d = dim_;
+  std::vector<std::vector<double>> M(m, std::vector<double>(2 * d + 1, 0.0));
+  std::vector<double> h(m), c(2 * d + 1, 0.0), z;
+  for (int i = 0; i < m; ++i) {
+    double nrm = 0;
+    for (int j = 0; j < d; ++j) {
+      M[i][j] = A_(i, j);
+      M[i][d + j] = -A_(i, j);
+      nrm += (double)A_(i, j) * A_(i, j);
+    }
+    M[i][2 * d] = std::sqrt(nrm);
+    h[i] = b_[i];
+  }
+  c[2 * d] = 1.0;
+  // bound r so the LP stays bounded for unbounded polyhedra
+  M.push_back(std::vector<double>(2 * d + 1, 0.0));
+  M.back()[2 * d] = 1.0;
+  h.push_back(1e6);
+  if (!simplex_max(M, h, c, z)) throw std::runtime_error("Failed to solve LP for Chebyshev center");
+  std::vector<float> x(d);
+  for (int j = 0; j < d; ++j) x[j] = (float)(z[j] - z[d + j]);
+  return x;
+}
+
+std::vector<float> HPolyhedron::GetFeasiblePoint() const {
+  std::vector<float> x = ChebyshevCenter();
+  if (!PointInSet(x, 1e-5f)) throw std::runtime_error("The polyhedron is empty.");
+  return x;
+}
+
+// ===========================================================================
+// GPU entry points
+// ===========================================================================
+namespace {
+const float *upload_configs(const MatrixXf &q, int dof) {
+  require(q.rows == dof, "configurations have " + std::to_string(q.rows) +
+                             " rows but the plant has " + std::to_string(dof) +
+                             " configuration variables");
+  size_t bytes = q.data.size() * sizeof(float);
+  void *d = scratch().q.get(std::max<size_t>(bytes, 4));
+  check_status(csd_memcpy_h2d(d, q.data.data(), bytes, nullptr), "configuration upload");
+  return (const float *)d;
+}
+}  // namespace
+
+std::vector<uint8_t> checkCollisionFreeCuda(const MatrixXf &configurations,
+                                            const MinimalPlant &plant) {
+  csd_plant *p = plant.device();
+  const int dof = plant.kin_tree.num_configuration_variables;
+  const int64_t n = configurations.cols;
+  std::vector<uint8_t> out(n);
+  if (n == 0) return out;
+  const float *dq = upload_configs(configurations, dof);
+  uint8_t *dout = (uint8_t *)scratch().out.get(n);
+  // (dim, N) column-major: config n at q[n*dim], joint stride 1
+  check_status(csd_check_collision_free(p, dq, n, dof, 1, dout, nullptr), "collision check");
+  check_status(csd_memcpy_d2h(out.data(), dout, n, nullptr), "result download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  return out;
+}
+
+std::vector<uint8_t> checkCollisionFreeVoxelsCuda(const MatrixXf &configurations,
+                                                  const MatrixXf &voxels, float voxel_radius,
+                                                  const MinimalPlant &plant,
+                                                  const std::vector<GeometryIndex> &robot_geoms) {
+  csd_plant *p = plant.device();
+  const int dof = plant.kin_tree.num_configuration_variables;
+  const int64_t n = configurations.cols;
+  std::vector<uint8_t> out(n);
+  if (n == 0) return out;
+  require(voxels.cols == 0 || voxels.rows == 3, "voxels must be a 3 x V matrix");
+  for (auto g : robot_geoms)
+    require(g >= 0 && g < plant.num_scene_geometries, "invalid robot geometry id");
+  const float *dq = upload_configs(configurations, dof);
+  float *dv = nullptr;
+  if (voxels.cols > 0) {
+    dv = (float *)scratch().aux.get(voxels.data.size() * 4);
+    check_status(csd_memcpy_h2d(dv, voxels.data.data(), voxels.data.size() * 4, nullptr),
+                 "voxel upload");
+  }
+  uint8_t *dout = (uint8_t *)scratch().out.get(n);
+  check_status(csd_check_collision_free_voxels(p, dq, n, dof, 1, dv, voxels.cols, voxel_radius,
+                                               robot_geoms.data(), (int)robot_geoms.size(), dout,
+                                               nullptr),
+               "voxel collision check");
+  check_status(csd_memcpy_d2h(out.data(), dout, n, nullptr), "result download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  return out;
+}
+
+std::vector<uint8_t> checkCollisionFreeVoxelsWithoutSelfCollisionCuda(
+    const MatrixXf &configurations, const MatrixXf &voxels, float voxel_radius,
+    const MinimalPlant &plant, const std::vector<GeometryIndex> &robot_geoms) {
+  csd_plant *p = plant.device();
+  const int dof = plant.kin_tree.num_configuration_variables;
+  const int64_t n = configurations.cols, V = voxels.cols;
+  require(V > 0, "at least one voxel is required");
+  require(voxels.rows == 3, "voxels must be a 3 x V matrix");
+  std::vector<uint8_t> out((size_t)n * V);
+  if (n == 0) return out;
+  const float *dq = upload_configs(configurations, dof);
+  float *dv = (float *)scratch().aux.get(voxels.data.size() * 4);
+  check_status(csd_memcpy_h2d(dv, voxels.data.data(), voxels.data.size() * 4, nullptr),
+               "voxel upload");
+  uint8_t *dout = (uint8_t *)scratch().out.get(out.size());
+  check_status(csd_voxel_flags(p, dq, n, dof, 1, dv, (int)V, voxel_radius, robot_geoms.data(),
+                               (int)robot_geoms.size(), dout, nullptr),
+               "voxel flags");
+  check_status(csd_memcpy_d2h(out.data(), dout, out.size(), nullptr), "result download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  return out;
+}
+
+namespace {
+MinimalPlant plant_of_tree(const MinimalKinematicTree &tree) {
+  MinimalPlant mp;
+  mp.kin_tree = tree;
+  return mp;
+}
+}  // namespace
+
+MatrixXf computeForwardKinematicsCuda(const MatrixXf &configurations,
+                                      const MinimalKinematicTree &tree) {
+  static thread_local MinimalPlant cache;
+  cache.kin_tree = tree;
+  csd_plant *p = cache.device();
+  const int L = tree.num_links;
+  const int64_t n = configurations.cols;
+  MatrixXf out(4, (int)(4 * L * n));
+  if (n == 0) return out;
+  const float *dq = upload_configs(configurations, tree.num_configuration_variables);
+  size_t bytes = (size_t)n * L * 16 * sizeof(float);
+  float *dt = (float *)scratch().out.get(bytes);
+  check_status(csd_forward_kinematics(p, dq, n, tree.num_configuration_variables, 1, dt, nullptr),
+               "forward kinematics");
+  check_status(csd_memcpy_d2h(out.data.data(), dt, bytes, nullptr), "result download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  return out;
+}
+
+std::vector<Matrix4f> computeLinkFrameToWorldTransformsMinimal(const std::vector<float> &q,
+                                                                const MinimalKinematicTree &tree) {
+  MatrixXf c(tree.num_configuration_variables, 1);
+  require((int)q.size() == tree.num_configuration_variables,
+          "configuration has the wrong number of variables");
+  c.data = q;
+  MatrixXf tf = computeForwardKinematicsCuda(c, tree);
+  std::vector<Matrix4f> out(tree.num_links);
+  for (int l = 0; l < tree.num_links; ++l) std::memcpy(out[l].data(), &tf.data[16 * l], 64);
+  return out;
+}
+
+bool checkCollisionFree(const std::vector<float> &q, const MinimalPlant &plant) {
+  MatrixXf c((int)q.size(), 1);
+  c.data = q;
+  return checkCollisionFreeCuda(c, plant)[0] != 0;
+}
+
+bool checkCollisionFreeVoxels(const std::vector<float> &q, const MatrixXf &voxels,
+                              float voxel_radius, const MinimalPlant &plant,
+                              const std::vector<GeometryIndex> &robot_geoms) {
+  MatrixXf c((int)q.size(), 1);
+  c.data = q;
+  return checkCollisionFreeVoxelsCuda(c, voxels, voxel_radius, plant, robot_geoms)[0] != 0;
+}
+
+bool checkEdgeCollisionFree(const std::vector<float> &q1, const std::vector<float> &q2,
+                            const MinimalPlant &plant, float step_size) {
+  csd_plant *p = plant.device();
+  const int dof = plant.kin_tree.num_configuration_variables;
+  require((int)q1.size() == dof && (int)q2.size() == dof, "edge endpoints have the wrong size");
+  std::vector<float> nodes(q1);
+  nodes.insert(nodes.end(), q2.begin(), q2.end());
+  float *dn = (float *)scratch().aux.get(nodes.size() * 4);
+  int32_t e[2] = {0, 1};
+  int32_t *de = (int32_t *)scratch().aux2.get(8);
+  uint8_t *dout = (uint8_t *)scratch().out.get(1);
+  check_status(csd_memcpy_h2d(dn, nodes.data(), nodes.size() * 4, nullptr), "upload");
+  check_status(csd_memcpy_h2d(de, e, 8, nullptr), "upload");
+  check_status(csd_validate_edges(p, dn, 2, de, 1, step_size, 0, dout, nullptr), "edge check");
+  uint8_t r = 0;
+  check_status(csd_memcpy_d2h(&r, dout, 1, nullptr), "download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  return r != 0;
+}
+
+bool pairCollisionFree(const CollisionGeometry &a, const Matrix4f &X_W_LA,
+                       const CollisionGeometry &b, const Matrix4f &X_W_LB) {
+  // two world-fixed links posed by X_W_LA / X_W_LB, one prismatic joint so
+  // the plant has a moving body: geometry A rides on it at q = 0
+  MinimalPlant mp;
+  MinimalKinematicTree &t = mp.kin_tree;
+  t.num_links = 3;
+  t.num_joints = 2;
+  t.num_configuration_variables = 1;
+  t.links.resize(3);
+  MinimalJoint j0, j1;
+  j0.type = PRISMATIC;
+  j0.X_PL_J = X_W_LA;
+  j0.parent_link = 0;
+  j0.child_link = 1;
+  j0.axis = {1.f, 0.f, 0.f};
+  j0.index = 0;
+  j1.type = FIXED;
+  j1.X_PL_J = X_W_LB;
+  j1.parent_link = 0;
+  j1.child_link = 2;
+  j1.index = 1;
+  t.joints = {j0, j1};
+  t.joint_traversal_sequence = {1, 0};
+  t.joint_idx_to_config_idx = {0, -1};
+  CollisionGeometry ga = a, gb = b;
+  ga.link_index = 1;
+  gb.link_index = 2;
+  mp.scene_collision_geometries = {ga, gb};
+  mp.num_scene_geometries = 2;
+  mp.collision_pairs_flat = {0, 1};
+  mp.num_collision_pairs = 1;
+  return checkCollisionFree({0.f}, mp);
+}
+
+std::vector<MatrixXf> UniformSampleInHPolyhedronCuda(const std::vector<HPolyhedron> &polys,
+                                                     const MatrixXf &starts, int n,
+                                                     int mixing_steps, uint64_t seed) {
+  ensure_device();
+  std::vector<MatrixXf> out;
+  if (polys.empty()) return out;
+  const int dim = polys[0].ambient_dimension();
+  require(n >= 0 && mixing_steps >= 0, "negative sample or mixing count");
+  require(starts.rows == dim && starts.cols == (int)polys.size(),
+          "starting_points must be dim x num_polyhedra");
+  int fmax = 0;
+  for (auto &p : polys) {
+    require(p.ambient_dimension() == dim, "polyhedra of different dimensions");
+    fmax = std::max(fmax, p.A().rows);
+  }
+  for (size_t k = 0; k < polys.size(); ++k) {
+    std::vector<float> x(dim);
+    for (int d = 0; d < dim; ++d) x[d] = starts(d, (int)k);
+    if (!polys[k].PointInSet(x, 1e-4f))
+      throw std::runtime_error("[UniformSampleInHPolyhedronCuda] starting point " +
+                               std::to_string(k) + " is not contained in its polyhedron");
+  }
+  const int P = (int)polys.size();
+  const int F = std::max(fmax, 1);
+  std::vector<float> A((size_t)P * F * dim, 0.f), b((size_t)P * F, 0.f);
+  std::vector<int32_t> nf(P);
+  for (int k = 0; k < P; ++k) {
+    MinimalHPolyhedron m = polys[k].GetMyMinimalHPolyhedron();
+    nf[k] = m.num_faces;
+    std::copy(m.A.begin(), m.A.end(), A.begin() + (size_t)k * F * dim);
+    std::copy(m.b.begin(), m.b.end(), b.begin() + (size_t)k * F);
+  }
+  float *dA = (float *)scratch().aux.get(A.size() * 4);
+  float *db = (float *)scratch().aux2.get(b.size() * 4 + P * 4 + (size_t)P * dim * 4);
+  int32_t *dnf = (int32_t *)(db + b.size());
+  float *dst = (float *)(dnf + P);
+  check_status(csd_memcpy_h2d(dA, A.data(), A.size() * 4, nullptr), "upload");
+  check_status(csd_memcpy_h2d(db, b.data(), b.size() * 4, nullptr), "upload");
+  check_status(csd_memcpy_h2d(dnf, nf.data(), P * 4, nullptr), "upload");
+  check_status(csd_memcpy_h2d(dst, starts.data.data(), (size_t)P * dim * 4, nullptr), "upload");
+  size_t obytes = (size_t)P * n * dim * 4;
+  float *dout = (float *)scratch().out.get(std::max<size_t>(obytes, 4));
+  check_status(csd_hit_and_run(dA, db, dnf, F, dim, P, dst, n, mixing_steps, seed, 0, dout,
+                               nullptr),
+               "hit-and-run");
+  std::vector<float> host(obytes / 4);
+  check_status(csd_memcpy_d2h(host.data(), dout, obytes, nullptr), "download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  for (int k = 0; k < P; ++k) {
+    MatrixXf m(dim, n);
+    std::copy(host.begin() + (size_t)k * n * dim, host.begin() + (size_t)(k + 1) * n * dim,
+              m.data.begin());
+    out.push_back(std::move(m));
+  }
+  return out;
+}
+
+std::vector<float> HPolyhedron::UniformSample(const std::vector<float> &prev, int mixing) const {
+  MatrixXf s(dim_, 1);
+  require((int)prev.size() == dim_, "wrong dimension of previous sample!");
+  s.data = prev;
+  static thread_local uint64_t calls = 0;
+  auto r = UniformSampleInHPolyhedronCuda({*this}, s, 1, mixing, 1337 + calls++);
+  return r[0].data;
+}
+
+std::vector<std::pair<int, int>> VisibilityGraph(const MatrixXf &configurations,
+                                                 const MinimalPlant &plant, float step_size,
+                                                 int64_t max_configs_per_batch) {
+  csd_plant *p = plant.device();
+  const int dof = plant.kin_tree.num_configuration_variables;
+  const int n = configurations.cols;
+  require(configurations.rows == dof, "configurations have the wrong number of rows");
+  require(n >= 2, "at least two configurations are required");
+  require(step_size >= 1e-12f, "step size must be positive");
+  require(max_configs_per_batch > 1, "max_configs_per_batch must exceed 1");
+  const int64_t E = (int64_t)n * (n - 1) / 2;
+  float *dn = (float *)scratch().aux.get(configurations.data.size() * 4);
+  check_status(csd_memcpy_h2d(dn, configurations.data.data(), configurations.data.size() * 4,
+                              nullptr),
+               "upload");
+  // edges are checked in fixed-size batches of the upper triangle
+  const int64_t batch = std::min<int64_t>(E, 1 << 20);
+  std::vector<int32_t> edges(2 * batch);
+  std::vector<uint8_t> flags(batch);
+  int32_t *de = (int32_t *)scratch().aux2.get(edges.size() * 4);
+  uint8_t *dout = (uint8_t *)scratch().out.get(batch);
+  std::vector<std::pair<int, int>> result;
+  int i = 0, j = 1;
+  for (int64_t done = 0; done < E; done += batch) {
+    int64_t cnt = std::min(batch, E - done);
+    for (int64_t k = 0; k < cnt; ++k) {
+      edges[2 * k] = i;
+      edges[2 * k + 1] = j;
+      if (++j == n) {
+        ++i;
+        j = i + 1;
+      }
+    }
+    check_status(csd_memcpy_h2d(de, edges.data(), cnt * 8, nullptr), "upload");
+    check_status(csd_validate_edges(p, dn, n, de, cnt, step_size, 1, dout, nullptr),
+                 "edge validation");
+    check_status(csd_memcpy_d2h(flags.data(), dout, cnt, nullptr), "download");
+    check_status(csd_stream_synchronize(nullptr), "synchronize");
+    for (int64_t k = 0; k < cnt; ++k)
+      if (flags[k]) result.emplace_back(edges[2 * k], edges[2 * k + 1]);
+  }
+  return result;
+}
+
+// ===========================================================================
+// EI-ZO
+// ===========================================================================
+uint32_t numSamplesUnadaptive(float epsilon, float delta, float tau) {
+  // cuda_set_builder_utils.h:198-203, float arithmetic
+  return (uint32_t)std::ceil(2 * std::log(1 / delta) / (epsilon * tau * tau));
+}
+
+namespace {
+csd_eizo_options to_c(const EizoOptions &o) {
+  csd_eizo_options c;
+  c.configuration_margin = o.configuration_margin;
+  c.tau = o.tau;
+  c.delta = o.delta;
+  c.epsilon = o.epsilon;
+  c.bisection_steps = o.bisection_steps;
+  c.num_particles = o.num_particles;
+  c.max_hyperplanes_per_iteration = o.max_hyperplanes_per_iteration;
+  c.max_iterations = o.max_iterations;
+  c.mixing_steps = o.mixing_steps;
+  c.seed = o.seed;
+  return c;
+}
+}  // namespace
+
+CudaEdgeInflator::CudaEdgeInflator(const MinimalPlant &plant,
+                                   const std::vector<GeometryIndex> &robot_geometry_ids,
+                                   const EizoOptions &options, const HPolyhedron &domain,
+                                   int max_regions, int max_voxels)
+    : plant_(plant),
+      robot_ids_(robot_geometry_ids),
+      options_(options),
+      domain_(domain),
+      max_regions_(max_regions),
+      max_voxels_(max_voxels) {
+  const int dim = domain.ambient_dimension();
+  require(dim == plant.kin_tree.num_configuration_variables,
+          "domain dimension differs from the plant's configuration variables");
+  for (auto g : robot_geometry_ids)
+    require(g >= 0 && g < plant.num_scene_geometries, "invalid robot geometry id");
+  require(max_regions >= 1, "max_regions must be positive");
+  // face capacity: domain + every face the iterations could add, rounded up
+  int64_t cap = (int64_t)domain.A().rows +
+                (int64_t)options.max_iterations * options.max_hyperplanes_per_iteration;
+  cap = std::min<int64_t>(cap, 4096);
+  face_capacity_ = (int)((cap + 15) / 16 * 16);
+  require(domain.A().rows < face_capacity_, "domain uses the whole face capacity");
+  csd_eizo_options o = to_c(options);
+  check_status(csd_eizo_create(plant_.device(), robot_ids_.data(), (int)robot_ids_.size(), &o,
+                               max_regions_, face_capacity_, (int)options.num_particles,
+                               max_voxels_, &ws_),
+               "EI-ZO workspace");
+}
+
+CudaEdgeInflator::~CudaEdgeInflator() {
+  if (ws_) csd_eizo_destroy(ws_);
+}
+
+std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, const MatrixXf &le,
+                                                        const MatrixXf &voxels, float voxel_radius,
+                                                        const std::vector<uint32_t> &budgets,
+                                                        std::vector<EizoInfo> *infos) {
+  const int dim = domain_.ambient_dimension();
+  const int R = ls.cols;
+  require(ls.rows == dim && le.rows == dim && le.cols == R, "line segments must be dim x R");
+  require(R <= max_regions_, "more edges than the inflator was created for");
+  require(voxels.cols <= max_voxels_, "more voxels than the inflator was created for");
+  require(voxels.cols == 0 || voxels.rows == 3, "voxels must be a 3 x V matrix");
+  require(budgets.empty() || (int)budgets.size() == R, "one particle budget per edge");
+  for (int r = 0; r < R; ++r) {
+    std::vector<float> a(dim), b(dim);
+    for (int d = 0; d < dim; ++d) {
+      a[d] = ls(d, r);
+      b[d] = le(d, r);
+    }
+    require(domain_.PointInSet(a, 1e-5f) && domain_.PointInSet(b, 1e-5f),
+            "line segment endpoints must lie in the domain");
+  }
+  MinimalHPolyhedron dom = domain_.GetMyMinimalHPolyhedron();
+  std::vector<float> A((size_t)R * face_capacity_ * dim), b((size_t)R * face_capacity_);
+  std::vector<csd_eizo_region_info> inf(R);
+  check_status(csd_eizo_run(ws_, dom.A.data(), dom.b.data(), dom.num_faces, ls.data.data(),
+                            le.data.data(), R, budgets.empty() ? nullptr : budgets.data(),
+                            voxels.data.data(), voxels.cols, voxel_radius, A.data(), b.data(),
+                            inf.data()),
+               "EI-ZO");
+  std::vector<HPolyhedron> out;
+  if (infos) infos->assign(R, EizoInfo());
+  for (int r = 0; r < R; ++r) {
+    MinimalHPolyhedron m;
+    m.dim = dim;
+    m.num_faces = inf[r].num_faces;
+    m.A.assign(A.begin() + (size_t)r * face_capacity_ * dim,
+               A.begin() + ((size_t)r * face_capacity_ + m.num_faces) * dim);
+    m.b.assign(b.begin() + (size_t)r * face_capacity_,
+               b.begin() + (size_t)r * face_capacity_ + m.num_faces);
+    out.emplace_back(m);
+    if (infos) {
+      EizoInfo &e = (*infos)[r];
+      e.iterations = inf[r].iterations;
+      e.converged = inf[r].converged != 0;
+      e.samples_drawn = inf[r].samples_drawn;
+      e.faces_added = inf[r].faces_added;
+    }
+  }
+  return out;
+}
+
+HPolyhedron CudaEdgeInflator::inflateEdge(const std::vector<float> &line_start,
+                                          const std::vector<float> &line_end,
+                                          const MatrixXf &voxels, float voxel_radius,
+                                          bool verbose) {
+  const int dim = domain_.ambient_dimension();
+  require((int)line_start.size() == dim && (int)line_end.size() == dim,
+          "line segment endpoints have the wrong dimension");
+  MatrixXf ls(dim, 1), le(dim, 1);
+  ls.data = line_start;
+  le.data = line_end;
+  std::vector<EizoInfo> infos;
+  std::vector<HPolyhedron> r = inflateEdges(ls, le, voxels, voxel_radius, {}, &infos);
+  last_info_ = infos[0];
+  if (verbose)
+    printf("EI-ZO: %d iterations, %s, %lld samples, %d faces added\n", last_info_.iterations,
+           last_info_.converged ? "certified" : "iteration limit reached",
+           (long long)last_info_.samples_drawn, last_info_.faces_added);
+  return r[0];
+}
+
+MinimalHPolyhedron EizoCuda(const std::vector<float> &line_start, const std::vector<float> &line_end,
+                            const MatrixXf &voxels, float voxel_radius, const HPolyhedron &domain,
+                            const MinimalPlant &plant,
+                            const std::vector<GeometryIndex> &robot_geometry_ids,
+                            const EizoOptions &options, EizoInfo &eizo_info) {
+  // argument checks of cuda_edge_inflation_zero_order.cu:355-386 (without the
+  // reference's dim < 14 restriction)
+  if (domain.ambient_dimension() > 32)
+    throw std::runtime_error("EI-ZO supports at most 32 configuration variables");
+  if (domain.A().rows > 0.9 * 4000)
+    throw std::runtime_error(
+        "ERROR: The domain polyhedron is already reserving almost all of the faces.");
+  float delta_min = (float)(6 * options.delta /
+                            (M_PI * M_PI * options.max_iterations * options.max_iterations));
+  uint32_t max_cfg = numSamplesUnadaptive(options.epsilon, delta_min, options.tau);
+  if (options.num_particles > max_cfg)
+    throw std::runtime_error(
+        "Current required particle count exceeds buffer size. There were " +
+        std::to_string(options.num_particles) + " num_particles requested, and " +
+        std::to_string(max_cfg) + " max_num_configurations space provided by the unadaptive test.");
+  CudaEdgeInflator inf(plant, robot_geometry_ids, options, domain, 1,
+                       std::max(voxels.cols, 1));
+  HPolyhedron P = inf.inflateEdge(line_start, line_end, voxels, voxel_radius, options.verbose);
+  eizo_info = inf.lastInfo();
+  return P.GetMyMinimalHPolyhedron();
+}
+
+}  // namespace csdecomp

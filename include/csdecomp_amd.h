@@ -1,0 +1,270 @@
+/*
+ * csdecomp_amd.h -- C-ABI of the MI355X (gfx950) implementation of
+ * csdecomp's data-parallel hot path: batched forward kinematics, collision
+ * checking of sphere/box/capsule geometry, hit-and-run sampling and EI-ZO
+ * edge inflation.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (CSD_OK == 0); csd_status_string
+ *    names it. Nothing in this library calls exit().
+ *  - Pointers documented "device" are HBM pointers the caller owns; "host"
+ *    pointers are read synchronously before the call returns.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Kernels are launched asynchronously on it unless documented otherwise.
+ *  - Configurations are addressed with two strides: element (config n, joint
+ *    j) lives at q[n * stride_n + j * stride_j]. Structure-of-arrays
+ *    (stride_n = 1, stride_j = N) gives coalesced reads; the reference's
+ *    (dim, N) column-major Eigen layout is stride_n = dim, stride_j = 1.
+ *  - 4x4 transforms are column-major (Eigen default), as in the reference.
+ */
+#ifndef CSDECOMP_AMD_H
+#define CSDECOMP_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CSD_OK = 0,
+  CSD_ERR_INVALID_ARGUMENT = 1,
+  CSD_ERR_CAPACITY = 2,      /* a fixed capacity was exceeded */
+  CSD_ERR_HIP = 3,           /* a HIP runtime call failed */
+  CSD_ERR_NO_DEVICE = 4,     /* no gfx950 device visible */
+  CSD_ERR_UNSUPPORTED = 5,   /* geometry / option combination unsupported */
+  CSD_ERR_OUT_OF_MEMORY = 6, /* hipMalloc failed */
+};
+
+enum { CSD_JOINT_FIXED = 0, CSD_JOINT_REVOLUTE = 1, CSD_JOINT_PRISMATIC = 2 };
+enum { CSD_BOX = 0, CSD_SPHERE = 1, CSD_CYLINDER = 2, CSD_CAPSULE = 3 };
+enum { CSD_GROUP_ROBOT = 0, CSD_GROUP_WORLD = 1, CSD_GROUP_VOXELS = 2 };
+
+/* Host-side description of a plant. Field-for-field the content of the
+ * reference's MinimalPlant (plant/minimal_plant.h:16-39, MinimalKinematicTree
+ * plant/minimal_kinematic_tree.h:42-53, CollisionGeometry
+ * plant/collision_geometry.h:27-35) without the compile-time capacities. */
+typedef struct csd_plant_desc {
+  int32_t num_links;
+  int32_t num_joints;
+  int32_t num_configuration_variables;
+  const int32_t *joint_type;        /* [J] CSD_JOINT_* */
+  const int32_t *joint_parent_link; /* [J] */
+  const int32_t *joint_child_link;  /* [J] */
+  const int32_t *joint_config_idx;  /* [J], -1 for fixed joints */
+  const int32_t *traversal;         /* [J] joint_traversal_sequence */
+  const float *joint_X_PL_J;        /* [J][16] column-major */
+  const float *joint_axis;          /* [J][3] unit axis */
+  int32_t num_geometries;
+  const int32_t *geom_type;  /* [G] CSD_BOX / CSD_SPHERE / CSD_CAPSULE */
+  const int32_t *geom_link;  /* [G] link index */
+  const int32_t *geom_group; /* [G] CSD_GROUP_* */
+  const float *geom_dims;    /* [G][3] box: full edge lengths; sphere: (r,-1,-1);
+                                capsule: (radius, length, radius), axis = z */
+  const float *geom_X_L_B;   /* [G][16] column-major */
+  int32_t num_collision_pairs;
+  const int32_t *collision_pairs; /* [P][2] geometry indices (A, B) */
+} csd_plant_desc;
+
+typedef struct csd_plant csd_plant; /* opaque, device resident */
+
+/* ---- library ---- */
+const char *csd_status_string(int status);
+/* Selects the device and checks it is gfx950. Replaces the implicit CUDA
+ * context setup of utils/cuda_utilities.h:28-64 (CudaPtr). */
+int csd_init(int device);
+/* Last error message recorded on this thread (for exceptions). */
+const char *csd_last_error(void);
+
+/* Device memory and stream helpers, so host code above this boundary never
+ * calls HIP directly. Replace CudaPtr (utils/cuda_utilities.h:28-64); errors
+ * are returned, never exit()ed (the reference's CUDACHECKERROR,
+ * utils/cuda_utilities.h:11-20, terminates the process). */
+int csd_malloc(void **ptr, size_t bytes);
+int csd_free(void *ptr);
+int csd_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int csd_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int csd_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int csd_memset(void *dst, int value, size_t bytes, void *stream);
+int csd_stream_synchronize(void *stream);
+int csd_stream_create(void **stream);
+int csd_stream_destroy(void *stream);
+
+/* ---- plant ----
+ * Replaces the per-call `CudaPtr<const MinimalPlant>` upload
+ * (plant/cuda_collision_checker.cu:455-462): the plant is compiled once into
+ * a compact device image (fixed joints folded, world-fixed geometry
+ * pre-transformed, pairs grouped by type). */
+int csd_plant_create(const csd_plant_desc *desc, csd_plant **out);
+int csd_plant_destroy(csd_plant *plant);
+int csd_plant_info(const csd_plant *plant, int32_t *num_links,
+                   int32_t *num_configuration_variables,
+                   int32_t *num_collision_pairs, int32_t *fast_path);
+
+/* ---- forward kinematics ----
+ * Replaces executeForwardKinematicsKernel (plant/cuda_forward_kinematics.h:27-30)
+ * and computeForwardKinematicsCuda (:53-55). transforms (device):
+ * [N][num_links][16] column-major, as the reference's transforms_flat. */
+int csd_forward_kinematics(const csd_plant *plant, const float *q, int64_t n,
+                           int64_t stride_n, int64_t stride_j,
+                           float *transforms, void *stream);
+
+/* ---- collision checking ----
+ * Replaces executeCollisionFreeKernel + poolPairCollisionFreeResultsKernel
+ * (plant/cuda_collision_checker.cu:16-85,182-209) and checkCollisionFreeCuda
+ * (:437-487): FK and all pair tests fused, no per-pair intermediates.
+ * out (device): [N] uint8, 1 = collision free. */
+int csd_check_collision_free(const csd_plant *plant, const float *q, int64_t n,
+                             int64_t stride_n, int64_t stride_j, uint8_t *out,
+                             void *stream);
+
+/* Replaces executeCollisionFreeVoxelsKernel (cuda_collision_checker.cu:255-307)
+ * / checkCollisionFreeVoxelsCuda (:377-435). voxels (device) [V][3] centres,
+ * robot_geoms (HOST array, at most 128 entries) [R] geometry indices checked
+ * against the voxels; it is passed to the kernel by value. */
+int csd_check_collision_free_voxels(const csd_plant *plant, const float *q,
+                                    int64_t n, int64_t stride_n,
+                                    int64_t stride_j, const float *voxels,
+                                    int32_t num_voxels, float voxel_radius,
+                                    const int32_t *robot_geoms,
+                                    int32_t num_robot_geoms, uint8_t *out,
+                                    void *stream);
+
+/* Replaces checkCollisionFreeVoxelsWithoutSelfCollisionCuda
+ * (cuda_collision_checker.cu:310-375): out (device) [N][V] uint8, 1 iff no
+ * robot geometry touches voxel v at config n. */
+int csd_voxel_flags(const csd_plant *plant, const float *q, int64_t n,
+                    int64_t stride_n, int64_t stride_j, const float *voxels,
+                    int32_t num_voxels, float voxel_radius,
+                    const int32_t *robot_geoms, int32_t num_robot_geoms,
+                    uint8_t *out, void *stream);
+
+/* ---- hit-and-run sampling ----
+ * Replaces executeUniformSampleInHPolyhedronKernel
+ * (geometry/cuda_hit_and_run_sampling.h:25-29). Polytopes (device): A
+ * [num_polytopes][face_stride][dim] row-major, b [num_polytopes][face_stride],
+ * num_faces [num_polytopes] (device int32). starts (device)
+ * [num_polytopes][dim]. out (device) [num_polytopes][num_samples][dim].
+ * Randomness: Philox4x32-10 keyed by seed; chain i of polytope p uses
+ * counters (i, step, p, round*16 + k). */
+int csd_hit_and_run(const float *A, const float *b, const int32_t *num_faces,
+                    int32_t face_stride, int32_t dim, int32_t num_polytopes,
+                    const float *starts, int64_t num_samples,
+                    int32_t mixing_steps, uint64_t seed, uint32_t round,
+                    float *out, void *stream);
+
+/* ---- EI-ZO ----
+ * Options: field-for-field EizoOptions
+ * (planning/cuda_edge_inflation_zero_order.h:18-32). */
+typedef struct csd_eizo_options {
+  float configuration_margin;
+  float tau;
+  float delta;
+  float epsilon;
+  uint32_t bisection_steps;
+  uint32_t num_particles;
+  uint32_t max_hyperplanes_per_iteration;
+  uint32_t max_iterations;
+  uint32_t mixing_steps;
+  uint64_t seed;
+} csd_eizo_options;
+
+/* Per-region result summary (EizoInfo, cuda_edge_inflation_zero_order.h:37-54
+ * aggregated per region). */
+typedef struct csd_eizo_region_info {
+  int32_t iterations;
+  int32_t converged;
+  int32_t num_faces;
+  int32_t faces_added;
+  int64_t samples_drawn;
+  int64_t collisions_found;
+} csd_eizo_region_info;
+
+typedef struct csd_eizo csd_eizo; /* persistent device workspace */
+
+/* Replaces EizoGpuMemory / CudaEdgeInflator's constructor
+ * (cuda_edge_inflation_zero_order.cu:448-494, cuda_polytope_builder.cu:69-108):
+ * allocates once for up to max_regions concurrent regions of at most
+ * face_capacity faces. robot_geoms is a host array. */
+int csd_eizo_create(const csd_plant *plant, const int32_t *robot_geoms,
+                    int32_t num_robot_geoms, const csd_eizo_options *options,
+                    int32_t max_regions, int32_t face_capacity,
+                    int32_t max_particles, int32_t max_voxels,
+                    csd_eizo **out);
+int csd_eizo_destroy(csd_eizo *ws);
+
+/* Inflates num_regions edges at once (executeEizo,
+ * cuda_edge_inflation_zero_order.cu:94-339, batched). All arrays are host
+ * arrays; the call is synchronous.
+ *  domain_A [domain_faces][dim], domain_b [domain_faces]
+ *  line_start / line_end [num_regions][dim]
+ *  particle_budgets [num_regions] (NULL = options.num_particles each)
+ *  voxels [num_voxels][3]
+ * Outputs: out_A [num_regions][face_capacity][dim], out_b
+ * [num_regions][face_capacity], infos [num_regions]. */
+int csd_eizo_run(csd_eizo *ws, const float *domain_A, const float *domain_b,
+                 int32_t domain_faces, const float *line_start,
+                 const float *line_end, int32_t num_regions,
+                 const uint32_t *particle_budgets, const float *voxels,
+                 int32_t num_voxels, float voxel_radius, float *out_A,
+                 float *out_b, csd_eizo_region_info *infos);
+
+/* EI-ZO stage kernels exposed for stage-wise parity tests (device arrays).
+ * Replaces executeProjectSamplesOntoLineSegmentsKernel and the bisection
+ * kernels (planning/cuda_set_builder_utils.h:25-131), fused: per particle,
+ * project onto its segment then run `steps` bisection steps with the full
+ * FK + collision check inline. */
+int csd_bisect(const csd_plant *plant, const float *particles,
+               const float *line_start, const float *line_end,
+               const uint32_t *segment_idx, int32_t num_particles,
+               int32_t steps, const float *voxels, int32_t num_voxels,
+               float voxel_radius, const int32_t *robot_geoms,
+               int32_t num_robot_geoms, float *scol, float *proj,
+               void *stream);
+
+/* Greedy FP64 face placement (maybePlaceFace +
+ * checkParticlesMadeRedundantByLastFace, cuda_set_builder_utils.cu:359-455,
+ * loop of cuda_edge_inflation_zero_order.cu:268-306) for one region, one
+ * workgroup. A/b/num_faces device; returns faces added in *added (device). */
+int csd_place_faces(float *A, float *b, int32_t *num_faces, int32_t capacity,
+                    int32_t dim, const float *scol, const float *proj,
+                    int32_t num_particles, const float *line_start,
+                    const float *line_end, float margin, int32_t max_faces,
+                    int32_t *added, void *stream);
+
+/* ---- edge validation and occupancy ----
+ * Configurations along each edge are generated in-kernel, never stored.
+ * rule 0 -- checkEdgeCollisionFree (plant/collision_checker.cpp:41-53):
+ *   n = ceil(|q2-q1| / step), configurations q1 + (k/n)(q2-q1), k < n
+ *   (q2 itself is not checked).
+ * rule 1 -- VisibilityGraph (planning/cuda_visibility_graph.cu:58-172):
+ *   n = (uint)(|q2-q1| / step + 2) points: q1, q2 and t = (k+1)/(n-1),
+ *   k < n-2.
+ * nodes (device) [num_nodes][dim], edges (device) [E][2] node indices,
+ * out (device) [E] uint8 (1 = collision free). */
+int csd_validate_edges(const csd_plant *plant, const float *nodes,
+                       int32_t num_nodes, const int32_t *edges,
+                       int64_t num_edges, float step, int32_t rule,
+                       uint8_t *out, void *stream);
+
+/* Swept occupancy on a regular voxel grid (voxel centre / id convention of
+ * planning/roadmap_builder.cpp:681-741, voxel radius sqrt(3)*resolution):
+ * grid (device) [nz][ny][nx] uint8 is OR-ed with 1 for every voxel touched
+ * by a robot geometry at any of the configurations. */
+int csd_occupancy(const csd_plant *plant, const float *q, int64_t n,
+                  int64_t stride_n, int64_t stride_j, const int32_t *robot_geoms,
+                  int32_t num_robot_geoms, const float *map_center,
+                  const float *map_size, float resolution, uint8_t *grid,
+                  void *stream);
+
+/* Brute-force k nearest neighbours (squared L2 in joint space); replaces
+ * ComputeJointDistancesKernel + host sort (planning/drm_cuda_utils.cu:6-39,
+ * planning/roadmap_builder.cpp:160-200). nodes (device) [N][dim];
+ * out_idx (device) [N][k] sorted by distance then index. */
+int csd_knn(const float *nodes, int32_t num_nodes, int32_t dim, int32_t k,
+            int32_t *out_idx, float *out_d2, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

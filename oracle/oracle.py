@@ -1,0 +1,301 @@
+"""ctypes wrapper of the CPU restatement (csd_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; nothing in csdecomp_amd/ imports
+it. Build with ``make -C oracle``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "libcsd_oracle.so")
+
+RULE_CPU, RULE_GPU = 0, 1
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class OrcPlant(C.Structure):
+    _fields_ = [
+        ("num_links", C.c_int32), ("num_joints", C.c_int32), ("num_config_vars", C.c_int32),
+        ("joint_type", C.c_void_p), ("joint_parent_link", C.c_void_p), ("joint_child_link", C.c_void_p),
+        ("joint_config_idx", C.c_void_p), ("traversal", C.c_void_p),
+        ("joint_X_PL_J", C.c_void_p), ("joint_axis", C.c_void_p),
+        ("num_geoms", C.c_int32),
+        ("geom_type", C.c_void_p), ("geom_link", C.c_void_p), ("geom_group", C.c_void_p),
+        ("geom_dims", C.c_void_p), ("geom_X_L_B", C.c_void_p),
+        ("num_pairs", C.c_int32), ("pairs", C.c_void_p),
+    ]
+
+
+class EizoOptions(C.Structure):
+    _fields_ = [
+        ("configuration_margin", C.c_float), ("tau", C.c_float), ("delta", C.c_float),
+        ("epsilon", C.c_float), ("bisection_steps", C.c_uint32), ("num_particles", C.c_uint32),
+        ("max_hyperplanes_per_iteration", C.c_uint32), ("max_iterations", C.c_uint32),
+        ("mixing_steps", C.c_uint32), ("seed", C.c_uint64),
+    ]
+
+
+class EizoInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
+                ("samples_drawn", C.c_int64), ("faces_added", C.c_int32)]
+
+
+def _load():
+    if not os.path.exists(LIB):
+        raise RuntimeError(f"oracle not built: {LIB} (run `make -C oracle`)")
+    lib = C.CDLL(LIB)
+    P = C.POINTER(OrcPlant)
+    sig = {
+        "orc_fk_f32": (None, [P, _f32p, _f32p]),
+        "orc_fk_f64": (None, [P, _f64p, _f64p]),
+        "orc_check_batch": (None, [P, _f32p, C.c_int64, C.c_int, _u8p, C.c_int]),
+        "orc_check_voxels_batch": (None, [P, _f32p, C.c_int64, _f32p, C.c_int32, C.c_float,
+                                          _i32p, C.c_int32, C.c_int, _u8p, C.c_int]),
+        "orc_voxel_flags_batch": (None, [P, _f32p, C.c_int64, _f32p, C.c_int32, C.c_float,
+                                         _i32p, C.c_int32, C.c_int, _u8p, C.c_int]),
+        "orc_min_sd_batch": (None, [P, _f32p, C.c_int64, _f32p, C.c_int32, C.c_float, _i32p,
+                                    C.c_int32, _f64p, C.c_int]),
+        "orc_fk_error_batch": (None, [P, _f32p, C.c_int64, _f64p, C.c_int]),
+        "orc_check_edge": (C.c_int, [P, _f32p, _f32p, C.c_float, C.c_int]),
+        "orc_edge_num_steps": (C.c_int32, [_f32p, _f32p, C.c_int32, C.c_float]),
+        "orc_philox4x32_10": (None, [np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")]),
+        "orc_hit_and_run": (None, [_f32p, _f32p, C.c_int32, C.c_int32, _f32p, C.c_int64,
+                                   C.c_int64, C.c_int32, C.c_uint64, C.c_uint32, C.c_uint32, _f32p]),
+        "orc_num_samples_unadaptive": (C.c_uint32, [C.c_float, C.c_float, C.c_float]),
+        "orc_delta_k": (C.c_float, [C.c_float, C.c_int32]),
+        "orc_project_onto_segment": (None, [_f32p, _f32p, _f32p, C.c_int32, _f32p, C.c_void_p]),
+        "orc_bisect": (None, [P, _f32p, _f32p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                              C.c_float, C.c_void_p, C.c_int32, C.c_int, _f32p]),
+        "orc_place_faces": (C.c_int32, [_f32p, _f32p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                        _f32p, _f32p, C.c_int32, _f32p, _f32p, C.c_float, C.c_int32]),
+        "orc_eizo": (C.c_int32, [P, _i32p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, _f32p,
+                                 _f32p, C.POINTER(C.c_int32), C.c_int32, _f32p, _f32p,
+                                 C.POINTER(EizoOptions), C.c_uint32, C.c_int, C.POINTER(EizoInfo)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+class Plant:
+    """Flat arrays of a plant (the content of csdecomp's MinimalPlant)."""
+
+    def __init__(self, minimal_plant):
+        mp = minimal_plant
+        t = mp.kin_tree
+        J = t.num_joints
+        self.num_links = t.num_links
+        self.dof = t.num_configuration_variables
+        joints = t.joints
+        self.joint_type = np.array([int(joints[j].type) for j in range(J)], np.int32)
+        self.joint_parent = np.array([joints[j].parent_link for j in range(J)], np.int32)
+        self.joint_child = np.array([joints[j].child_link for j in range(J)], np.int32)
+        self.joint_cfg = np.array(list(t.joint_idx_to_config_idx)[:J], np.int32)
+        self.traversal = np.array(list(t.joint_traversal_sequence)[:J], np.int32)
+        self.joint_X = np.ascontiguousarray(
+            np.stack([np.asarray(joints[j].X_PL_J, np.float32).T.reshape(16) for j in range(J)])
+            if J else np.zeros((0, 16), np.float32))
+        self.joint_axis = np.ascontiguousarray(
+            np.stack([np.asarray(joints[j].axis, np.float32) for j in range(J)])
+            if J else np.zeros((0, 3), np.float32))
+        geoms = mp.scene_collision_geometries
+        G = len(geoms)
+        self.geom_type = np.array([int(g.type) for g in geoms], np.int32)
+        self.geom_link = np.array([g.link_index for g in geoms], np.int32)
+        self.geom_group = np.array([int(g.group) for g in geoms], np.int32)
+        self.geom_dims = np.ascontiguousarray(
+            np.stack([np.asarray(g.dimensions, np.float32) for g in geoms])
+            if G else np.zeros((0, 3), np.float32))
+        self.geom_X = np.ascontiguousarray(
+            np.stack([np.asarray(g.X_L_B, np.float32).T.reshape(16) for g in geoms])
+            if G else np.zeros((0, 16), np.float32))
+        self.pairs = np.ascontiguousarray(np.asarray(mp.collision_pairs_flat, np.int32))
+        self.num_pairs = mp.num_collision_pairs
+        self._s = OrcPlant(
+            self.num_links, J, self.dof,
+            self.joint_type.ctypes.data, self.joint_parent.ctypes.data, self.joint_child.ctypes.data,
+            self.joint_cfg.ctypes.data, self.traversal.ctypes.data,
+            self.joint_X.ctypes.data, self.joint_axis.ctypes.data, G,
+            self.geom_type.ctypes.data, self.geom_link.ctypes.data, self.geom_group.ctypes.data,
+            self.geom_dims.ctypes.data, self.geom_X.ctypes.data, self.num_pairs,
+            self.pairs.ctypes.data)
+
+    @property
+    def ref(self):
+        return C.byref(self._s)
+
+
+def _aos(q, dof):
+    """(dof, N) configurations -> contiguous [N][dof] float32."""
+    q = np.asarray(q, np.float32)
+    if q.ndim == 1:
+        q = q[:, None]
+    assert q.shape[0] == dof
+    return np.ascontiguousarray(q.T)
+
+
+def fk(plant, q, double=False):
+    """Link transforms of one configuration: [L][4][4] (row-major view)."""
+    L = plant.num_links
+    if double:
+        out = np.zeros(L * 16, np.float64)
+        lib().orc_fk_f64(plant.ref, np.ascontiguousarray(q, np.float64), out)
+    else:
+        out = np.zeros(L * 16, np.float32)
+        lib().orc_fk_f32(plant.ref, np.ascontiguousarray(q, np.float32), out)
+    return out.reshape(L, 4, 4).transpose(0, 2, 1)
+
+
+def check(plant, q, rule=RULE_GPU, threads=0):
+    qa = _aos(q, plant.dof)
+    out = np.zeros(qa.shape[0], np.uint8)
+    lib().orc_check_batch(plant.ref, qa, qa.shape[0], rule, out, threads)
+    return out
+
+
+def _vox(voxels):
+    if voxels is None:
+        return np.zeros((0, 3), np.float32), 0
+    v = np.asarray(voxels, np.float32)
+    v = np.ascontiguousarray(v.T) if v.shape[0] == 3 else np.ascontiguousarray(v)
+    return v, v.shape[0]
+
+
+def check_voxels(plant, q, voxels, radius, robot_geoms, rule=RULE_GPU, threads=0):
+    qa = _aos(q, plant.dof)
+    v, nv = _vox(voxels)
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    out = np.zeros(qa.shape[0], np.uint8)
+    lib().orc_check_voxels_batch(plant.ref, qa, qa.shape[0], v, nv, radius, rg, len(rg), rule,
+                                 out, threads)
+    return out
+
+
+def voxel_flags(plant, q, voxels, radius, robot_geoms, rule=RULE_GPU, threads=0):
+    qa = _aos(q, plant.dof)
+    v, nv = _vox(voxels)
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    out = np.zeros((qa.shape[0], nv), np.uint8)
+    lib().orc_voxel_flags_batch(plant.ref, qa, qa.shape[0], v, nv, radius, rg, len(rg), rule,
+                                out.reshape(-1), threads)
+    return out
+
+
+def min_sd(plant, q, voxels=None, radius=0.0, robot_geoms=(), threads=0):
+    qa = _aos(q, plant.dof)
+    v, nv = _vox(voxels)
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    out = np.zeros(qa.shape[0], np.float64)
+    lib().orc_min_sd_batch(plant.ref, qa, qa.shape[0], v, nv, radius, rg, len(rg), out, threads)
+    return out
+
+
+def fk_error(plant, q, threads=0):
+    qa = _aos(q, plant.dof)
+    out = np.zeros((qa.shape[0], plant.num_links), np.float64)
+    lib().orc_fk_error_batch(plant.ref, qa, qa.shape[0], out.reshape(-1), threads)
+    return out
+
+
+def check_edge(plant, q1, q2, step=0.01, rule=RULE_GPU):
+    return bool(lib().orc_check_edge(plant.ref, np.ascontiguousarray(q1, np.float32),
+                                     np.ascontiguousarray(q2, np.float32), step, rule))
+
+
+def edge_num_steps(q1, q2, step):
+    q1 = np.ascontiguousarray(q1, np.float32)
+    return lib().orc_edge_num_steps(q1, np.ascontiguousarray(q2, np.float32), len(q1), step)
+
+
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.ascontiguousarray(ctr, np.uint32), np.ascontiguousarray(key, np.uint32), out)
+    return out
+
+
+def hit_and_run(A, b, start, n, mixing, seed=1337, stream=0, round_=0, chain_offset=0):
+    A = np.ascontiguousarray(A, np.float32)
+    F, dim = A.shape
+    out = np.zeros((n, dim), np.float32)
+    lib().orc_hit_and_run(A, np.ascontiguousarray(b, np.float32), F, dim,
+                          np.ascontiguousarray(start, np.float32), n, chain_offset, mixing, seed,
+                          stream, round_, out)
+    return out
+
+
+def num_samples_unadaptive(eps, delta, tau):
+    return lib().orc_num_samples_unadaptive(eps, delta, tau)
+
+
+def delta_k(delta, it):
+    return lib().orc_delta_k(delta, it)
+
+
+def project(p, q1, q2):
+    p = np.ascontiguousarray(p, np.float32)
+    proj = np.zeros_like(p)
+    d = C.c_float(0)
+    lib().orc_project_onto_segment(p, np.ascontiguousarray(q1, np.float32),
+                                   np.ascontiguousarray(q2, np.float32), len(p), proj,
+                                   C.cast(C.pointer(d), C.c_void_p))
+    return proj, d.value
+
+
+def bisect(plant, particle, proj, steps, rule=RULE_GPU):
+    sc = np.zeros(plant.dof, np.float32)
+    lib().orc_bisect(plant.ref, np.ascontiguousarray(particle, np.float32),
+                     np.ascontiguousarray(proj, np.float32), plant.dof, steps, None, 0, 0.0,
+                     None, 0, rule, sc)
+    return sc
+
+
+def place_faces(A, b, nf, cap, scol, proj, ls, le, margin, max_faces):
+    """A [cap][dim], b [cap] float32 arrays modified in place; returns added."""
+    n = C.c_int32(nf)
+    dim = A.shape[1]
+    added = lib().orc_place_faces(A.reshape(-1), b, C.byref(n), cap, dim,
+                                  np.ascontiguousarray(scol, np.float32).reshape(-1),
+                                  np.ascontiguousarray(proj, np.float32).reshape(-1),
+                                  len(scol), np.ascontiguousarray(ls, np.float32),
+                                  np.ascontiguousarray(le, np.float32), margin, max_faces)
+    return added, n.value
+
+
+def eizo(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, stream=0, rule=RULE_GPU):
+    dim = plant.dof
+    A = np.zeros((cap, dim), np.float32)
+    b = np.zeros(cap, np.float32)
+    F = A_dom.shape[0]
+    A[:F] = A_dom
+    b[:F] = b_dom
+    nf = C.c_int32(F)
+    o = EizoOptions(*opts)
+    info = EizoInfo()
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    rc = lib().orc_eizo(plant.ref, rg, len(rg), None, 0, 0.0, A.reshape(-1), b, C.byref(nf), cap,
+                        np.ascontiguousarray(ls, np.float32), np.ascontiguousarray(le, np.float32),
+                        C.byref(o), stream, rule, C.byref(info))
+    if rc != 0:
+        raise RuntimeError("oracle EI-ZO face capacity exceeded")
+    return A[:nf.value], b[:nf.value], info

@@ -1,0 +1,241 @@
+"""GPU parity of FK and collision checking against the CPU restatement.
+
+Bar: flags identical except configurations whose FP64 signed distance lies
+within 1e-5 m of contact; FK within 1e-5 (FP32 vs FP64) per link."""
+import numpy as np
+import pytest
+
+from conftest import assert_flags_match, parse
+from test_oracle import KNOWN, VOX_COL, VOX_FREE, voxel_grid_6x6x6
+
+pytestmark = pytest.mark.gpu
+
+
+def _synthetic(name):
+    from csdecomp_amd import scenes
+    return {"cfg0": lambda: scenes.arm_scene(1000, 16, 0),
+            "cfg1": lambda: scenes.arm_scene(1001, 48, 16),
+            "cfg2": lambda: scenes.dual_arm_scene(1002, 72, 24)}[name]()
+
+
+@pytest.mark.parametrize("fname,free,col", KNOWN, ids=[k[0] for k in KNOWN])
+def test_known_answer_configs_api(gpu, csd, fname, free, col):
+    mp = parse(fname).get_minimal_plant()
+    free = np.asarray(free, np.float32).T
+    col = np.asarray(col, np.float32).T
+    assert list(csd.CheckCollisionFreeCuda(free, mp)) == [1] * free.shape[1]
+    assert list(csd.CheckCollisionFreeCuda(col, mp)) == [0] * col.shape[1]
+    for k in range(free.shape[1]):
+        assert csd.CheckCollisionFree(free[:, k], mp)
+    for k in range(col.shape[1]):
+        assert not csd.CheckCollisionFree(col[:, k], mp)
+
+
+def test_known_answer_voxels_api(gpu, csd):
+    p = parse("movable_block.urdf")
+    mp = p.get_minimal_plant()
+    rg = list(p.get_scene_inspector().robot_geometry_ids)
+    vox = csd.Voxels(voxel_grid_6x6x6())
+    assert list(csd.CheckCollisionFreeVoxelsCuda(VOX_FREE, vox, 0.02, mp, rg)) == [1, 1, 1]
+    assert list(csd.CheckCollisionFreeVoxelsCuda(VOX_COL, vox, 0.02, mp, rg)) == [0] * 5
+
+
+PLANTS = ["2_branch_5dof.urdf", "3dofrobot.urdf", "3dofrobot_prismatic.urdf",
+          "3dofrobot_spheres.urdf", "movable_block.urdf", "cfg0", "cfg1", "cfg2"]
+
+
+def _plant(name):
+    if name.endswith(".urdf"):
+        p = parse(name)
+        t = p.get_kinematic_tree()
+        lo = np.asarray(t.get_position_lower_limits())
+        hi = np.asarray(t.get_position_upper_limits())
+        return p.get_minimal_plant(), np.clip(lo, -4, 4), np.clip(hi, -4, 4)
+    pl = _synthetic(name)
+    return pl.getMinimalPlant(), pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+
+
+@pytest.mark.parametrize("name", PLANTS)
+def test_random_flags_vs_oracle(gpu, csd, orc, name):
+    import torch
+    from csdecomp_amd import capi, scenes
+    mp, lo, hi = _plant(name)
+    n = 20000 if name != "cfg2" else 8000
+    q = scenes.uniform_configs(2000 + len(name), n, lo, hi)  # (dof, n)
+    op = orc.Plant(mp)
+    sd = orc.min_sd(op, q)
+    ref = orc.check(op, q, orc.RULE_GPU)
+    # reference API path: (dim, N) column-major host array
+    api = np.asarray(csd.CheckCollisionFreeCuda(q, mp), np.uint8)
+    assert_flags_match(api, ref, sd, f"{name} api")
+    # C-ABI path, structure-of-arrays on the device
+    dp = capi.DevicePlant(mp)
+    qs = torch.from_numpy(np.ascontiguousarray(q)).to(gpu)  # [dof][n]
+    out = torch.empty(n, dtype=torch.uint8, device=gpu)
+    capi.check_collision_free(dp, qs, n, 1, n, out)
+    torch.cuda.synchronize()
+    assert_flags_match(out.cpu().numpy(), ref, sd, f"{name} soa")
+    assert 0 < ref.sum() < n or name in ("3dofrobot.urdf",)
+
+
+def test_fast_path_selected(gpu):
+    from csdecomp_amd import capi
+    for name in ("cfg0", "cfg1", "cfg2"):
+        mp, _, _ = _plant(name)
+        assert capi.DevicePlant(mp).info()["fast_path"], name
+    mp, _, _ = _plant("2_branch_5dof.urdf")
+    assert not capi.DevicePlant(mp).info()["fast_path"]
+
+
+def test_fk_golden_gpu(gpu, csd):
+    from test_oracle import is_approx
+    p = parse("3dofrobot_prismatic.urdf")
+    tree = p.get_minimal_kinematic_tree()
+    tf = csd.computeForwardKinematicsCuda(np.ones((3, 1), np.float32), tree)
+    i = p.get_kinematic_tree().get_link_index("link3")
+    exp = [[0.60952558, 0.17919833, 0.77224771, 1.75935346],
+           [0.59500984, 0.54030231, -0.59500984, 1.19001968],
+           [-0.52387199, 0.8221687, 0.22270331, -1.88921496], [0, 0, 0, 1]]
+    assert is_approx(tf[:, 4 * i:4 * i + 4], exp)
+    m = csd.computeLinkFrameToWorldTransformsMinimal(np.ones(3, np.float32), tree)
+    assert is_approx(m[4 * i:4 * i + 4], exp)
+    p2 = parse("2_branch_5dof.urdf")
+    t2 = p2.get_kinematic_tree()
+    tfs = t2.compute_link_frame_to_world_transforms(np.ones(5, np.float32))
+    assert is_approx(tfs[t2.get_link_index("link6")],
+                     [[-0.9899925, -0.14112001, 0.0, -1.0596646], [0.14112001, -0.9899925, 0.0, 0.58067513],
+                      [0, 0, 1, 0], [0, 0, 0, 1]])
+
+
+@pytest.mark.parametrize("name", ["2_branch_5dof.urdf", "3dofrobot_prismatic.urdf", "cfg0", "cfg2"])
+def test_fk_error_per_link(gpu, csd, orc, name):
+    """FP32 GPU FK against the FP64 oracle, per link (max abs over the 3x4)."""
+    from csdecomp_amd import scenes
+    mp, lo, hi = _plant(name)
+    q = scenes.uniform_configs(5, 500, lo, hi)
+    tf = np.asarray(csd.computeForwardKinematicsCuda(q, mp.kin_tree))  # (4, 4*L*N)
+    op = orc.Plant(mp)
+    L = mp.kin_tree.num_links
+    worst = np.zeros(L)
+    for k in range(q.shape[1]):
+        Xd = orc.fk(op, q[:, k].astype(np.float64), double=True)
+        for l in range(L):
+            g = tf[:, 4 * (k * L + l):4 * (k * L + l) + 4]
+            worst[l] = max(worst[l], np.abs(g[:3, :] - Xd[l][:3, :]).max())
+    assert worst.max() < 1e-5, worst
+
+
+def test_voxels_vs_oracle(gpu, csd, orc):
+    from csdecomp_amd import scenes
+    p = parse("movable_block.urdf")
+    mp = p.get_minimal_plant()
+    rg = list(p.get_scene_inspector().robot_geometry_ids)
+    t = p.get_kinematic_tree()
+    q = scenes.uniform_configs(9, 3000, t.get_position_lower_limits(), t.get_position_upper_limits())
+    vox = voxel_grid_6x6x6()
+    op = orc.Plant(mp)
+    ref = orc.check_voxels(op, q, vox, 0.02, rg, orc.RULE_GPU)
+    sd = orc.min_sd(op, q, vox, 0.02, rg)
+    got = np.asarray(csd.CheckCollisionFreeVoxelsCuda(q, csd.Voxels(vox), 0.02, mp, rg), np.uint8)
+    assert_flags_match(got, ref, sd, "voxels")
+    flags = csd.CheckCollisionFreeVoxelsWithoutSelfCollisionCuda(q[:, :200], csd.Voxels(vox), 0.02, mp, rg)
+    ref2 = orc.voxel_flags(op, q[:, :200], vox, 0.02, rg, orc.RULE_GPU)
+    mism = np.argwhere(np.asarray(flags) != ref2)
+    assert len(mism) <= 2, mism[:5]  # only contact-band voxel/geometry pairs may differ
+
+
+def test_fast_path_voxels(gpu, csd, orc):
+    """Robot spheres x voxels through the fast kernel."""
+    from csdecomp_amd import scenes
+    pl = scenes.arm_scene(1000, 16, 0)
+    mp = pl.getMinimalPlant()
+    rg = list(pl.getRobotGeometryIds())
+    rng = np.random.default_rng(4)
+    vox = rng.uniform(-1, 1, size=(3, 50)).astype(np.float32)
+    q = scenes.uniform_configs(12, 5000, pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
+    op = orc.Plant(mp)
+    ref = orc.check_voxels(op, q, vox, 0.03, rg, orc.RULE_GPU)
+    sd = orc.min_sd(op, q, vox, 0.03, rg)
+    got = np.asarray(csd.CheckCollisionFreeVoxelsCuda(q, csd.Voxels(vox), 0.03, mp, rg), np.uint8)
+    assert_flags_match(got, ref, sd, "fast voxels")
+
+
+def test_capsules_vs_oracle(gpu, csd, orc):
+    """Capsule obstacles (fast path) and a capsule link (generic path) against
+    the FP64 segment distances of the oracle."""
+    from csdecomp_amd import scenes
+    from csdecomp_amd import _pycsdecomp as c
+    pl = scenes.arm_scene(1000, 8, 0)
+    mp = pl.getMinimalPlant()
+    geoms = list(mp.scene_collision_geometries)
+    obs_link = geoms[-1].link_index
+    rng = np.random.default_rng(5)
+    start = len(geoms)
+    for _ in range(6):
+        g = c.CollisionGeometry()
+        g.type = c.ShapePrimitive.CAPSULE
+        r, ln = rng.uniform(0.05, 0.15), rng.uniform(0.2, 0.6)
+        g.dimensions = np.array([r, ln, r], np.float32)
+        g.link_index = obs_link
+        g.group = c.CollisionGroup.WORLD
+        X = np.eye(4, dtype=np.float32)
+        X[:3, :3] = scenes.random_rotation(rng)
+        X[:3, 3] = rng.uniform(-0.8, 0.8, 3)
+        g.X_L_B = X
+        geoms.append(g)
+    robot = list(pl.getRobotGeometryIds())
+    pairs = list(np.asarray(mp.collision_pairs_flat).reshape(-1, 2))
+    for gi in range(start, len(geoms)):
+        pairs += [(r, gi) for r in robot]
+    mp.set_scene_collision_geometries(geoms)
+    mp.set_collision_pairs_from_matrix(np.asarray(pairs, np.int32).T)
+    q = scenes.uniform_configs(13, 6000, pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
+    op = orc.Plant(mp)
+    sd = orc.min_sd(op, q)
+    got = np.asarray(csd.CheckCollisionFreeCuda(q, mp), np.uint8)
+    assert_flags_match(got, orc.check(op, q, orc.RULE_GPU), sd, "capsule obstacles")
+    # a capsule riding on link 3 -> generic kernel (capsule-box, capsule-sphere)
+    g = c.CollisionGeometry()
+    g.type = c.ShapePrimitive.CAPSULE
+    g.dimensions = np.array([0.04, 0.3, 0.04], np.float32)
+    g.link_index = geoms[8].link_index
+    g.group = c.CollisionGroup.ROBOT
+    g.X_L_B = np.eye(4, dtype=np.float32)
+    geoms.append(g)
+    ci = len(geoms) - 1
+    pairs += [(ci, gi) for gi in range(start - 8, len(geoms) - 1) if geoms[gi].group == c.CollisionGroup.WORLD]
+    mp.set_scene_collision_geometries(geoms)
+    mp.set_collision_pairs_from_matrix(np.asarray(pairs, np.int32).T)
+    op = orc.Plant(mp)
+    sd = orc.min_sd(op, q)
+    got = np.asarray(csd.CheckCollisionFreeCuda(q, mp), np.uint8)
+    assert_flags_match(got, orc.check(op, q, orc.RULE_GPU), sd, "capsule link")
+
+
+def test_pair_collision_free(gpu, csd):
+    c = csd
+    a, b = c.CollisionGeometry(), c.CollisionGeometry()
+    a.type = b.type = c.ShapePrimitive.SPHERE
+    a.dimensions = np.array([0.5, -1, -1], np.float32)
+    b.dimensions = np.array([0.25, -1, -1], np.float32)
+    Xa, Xb = np.eye(4, dtype=np.float32), np.eye(4, dtype=np.float32)
+    Xb[0, 3] = 0.76
+    assert c.PairCollisionFree(a, Xa, b, Xb)
+    Xb[0, 3] = 0.74
+    assert not c.PairCollisionFree(a, Xa, b, Xb)
+
+
+def test_empty_and_errors(gpu, csd):
+    import torch
+    from csdecomp_amd import capi
+    mp, _, _ = _plant("cfg0")
+    assert list(csd.CheckCollisionFreeCuda(np.zeros((7, 0), np.float32), mp)) == []
+    with pytest.raises(ValueError):
+        csd.CheckCollisionFreeCuda(np.zeros((6, 3), np.float32), mp)
+    dp = capi.DevicePlant(mp)
+    out = torch.empty(4, dtype=torch.uint8, device=gpu)
+    with pytest.raises(capi.CsdError) as e:
+        capi.check_collision_free(dp, None, 4, 1, 4, out)
+    assert e.value.status == 1
+    with pytest.raises(capi.CsdError):
+        capi.check_collision_free(dp, out, -1, 1, 4, out)

@@ -1,0 +1,292 @@
+"""GPU parity of hit-and-run, the EI-ZO stages and end-to-end EI-ZO against
+the CPU restatement; edge validation, occupancy and k-NN against brute
+force."""
+import numpy as np
+import pytest
+
+from conftest import parse
+
+pytestmark = pytest.mark.gpu
+
+
+def _box_domain(lo, hi):
+    d = len(lo)
+    A = np.zeros((2 * d, d), np.float32)
+    b = np.zeros(2 * d, np.float32)
+    for i in range(d):
+        A[2 * i, i], A[2 * i + 1, i] = -1, 1
+        b[2 * i], b[2 * i + 1] = -lo[i], hi[i]
+    return A, b
+
+
+def test_hit_and_run_matches_oracle(gpu, orc):
+    import torch
+    from csdecomp_amd import capi
+    A = np.array([[-1, 0], [1, 0], [0, -1], [0, 1], [1, 1]], np.float32)
+    b = np.array([0, 6, 0, 6, 11.25], np.float32)
+    n = 4096
+    dA = torch.from_numpy(A.reshape(-1)).to(gpu)
+    db = torch.from_numpy(b).to(gpu)
+    nf = torch.tensor([5], dtype=torch.int32, device=gpu)
+    st = torch.tensor([1.0, 1.0], device=gpu)
+    out = torch.empty(n * 2, device=gpu)
+    capi.hit_and_run(dA, db, nf, 5, 2, 1, st, n, 50, 1337, 3, out)
+    torch.cuda.synchronize()
+    g = out.cpu().numpy().reshape(n, 2)
+    ref = orc.hit_and_run(A, b, [1, 1], n, 50, seed=1337, stream=0, round_=3)
+    # same Philox stream: only libm-vs-device-math rounding separates them
+    assert np.abs(g - ref).max() < 1e-3
+    assert np.mean(np.abs(g - ref).max(1) < 1e-4) > 0.99
+    assert np.all(g @ A.T <= b + 1e-4)
+
+
+def test_uniform_sample_api(gpu, csd):
+    P1, P2 = csd.HPolyhedron(), csd.HPolyhedron()
+    P1.MakeBox(np.array([-1, -1, -1], np.float32), np.array([1, 1, 1], np.float32))
+    P2.MakeBox(np.array([0, 0, 0], np.float32), np.array([2, 0.5, 4], np.float32))
+    starts = np.array([[0, 0, 0], [1, 0.25, 2]], np.float32).T
+    out = csd.UniformSampleInHPolyhedraCuda([P1, P2], starts, 3000, 40)
+    assert len(out) == 2 and out[0].shape == (3, 3000)
+    for P, s in zip((P1, P2), out):
+        assert all(P.PointInSet(s[:, k], 1e-5) for k in range(0, 3000, 7))
+    np.testing.assert_allclose(out[1].mean(1), [1, 0.25, 2], atol=0.08)
+    assert not np.allclose(out[0][:, 0], out[1][:, 0])
+    with pytest.raises(RuntimeError):
+        csd.UniformSampleInHPolyhedraCuda([P1], np.array([[5.0, 0, 0]], np.float32).T, 10, 5)
+
+
+def test_bisection_stage_parity(gpu, orc):
+    """Identical particle sets -> identical optimised collisions (exact unless
+    a midpoint falls in the contact band)."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    op = orc.Plant(mp)
+    q = scenes.uniform_configs(77, 4000, pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
+    col = q[:, orc.check(op, q, orc.RULE_GPU) == 0][:, :1000]
+    ls = np.zeros(7, np.float32)
+    le = np.full(7, 0.3, np.float32)
+    n = col.shape[1]
+    dp = capi.DevicePlant(mp)
+    dpart = torch.from_numpy(np.ascontiguousarray(col.T)).to(gpu)
+    dls, dle = torch.from_numpy(ls).to(gpu), torch.from_numpy(le).to(gpu)
+    seg = torch.zeros(n, dtype=torch.int32, device=gpu)
+    scol = torch.empty(n * 7, device=gpu)
+    proj = torch.empty(n * 7, device=gpu)
+    capi.bisect(dp, dpart, dls, dle, seg, n, 10, scol, proj)
+    torch.cuda.synchronize()
+    scol, proj = scol.cpu().numpy().reshape(n, 7), proj.cpu().numpy().reshape(n, 7)
+    exact = 0
+    for k in range(n):
+        pr, _ = orc.project(col[:, k], ls, le)
+        np.testing.assert_allclose(proj[k], pr, atol=1e-6)
+        ref = orc.bisect(op, col[:, k], pr, 10)
+        exact += np.array_equal(ref, scol[k])
+    assert exact >= n - 3
+
+
+def test_face_stage_parity(gpu, orc):
+    import torch
+    from csdecomp_amd import capi
+    rng = np.random.default_rng(8)
+    dim, n, cap = 7, 3000, 64
+    ls, le = rng.uniform(-0.2, 0.2, dim).astype(np.float32), rng.uniform(-0.2, 0.2, dim).astype(np.float32)
+    proj = (ls + rng.uniform(0, 1, (n, 1)) * (le - ls)).astype(np.float32)
+    scol = (proj + rng.normal(0, 0.5, (n, dim))).astype(np.float32)
+    A0, b0 = _box_domain(np.full(dim, -3.0), np.full(dim, 3.0))
+    A = np.zeros((cap, dim), np.float32)
+    b = np.zeros(cap, np.float32)
+    A[:14], b[:14] = A0, b0
+    added_ref, nf_ref = orc.place_faces(A, b, 14, cap, scol, proj, ls, le, 0.01, 10)
+    dA = torch.zeros(cap * dim, device=gpu)
+    dA[:14 * dim] = torch.from_numpy(A0.reshape(-1))
+    db = torch.zeros(cap, device=gpu)
+    db[:14] = torch.from_numpy(b0)
+    dnf = torch.tensor([14], dtype=torch.int32, device=gpu)
+    added = torch.zeros(1, dtype=torch.int32, device=gpu)
+    capi.place_faces(dA, db, dnf, cap, dim, torch.from_numpy(scol).to(gpu), torch.from_numpy(proj).to(gpu),
+                     n, torch.from_numpy(ls).to(gpu), torch.from_numpy(le).to(gpu), 0.01, 10, added)
+    torch.cuda.synchronize()
+    assert int(added.item()) == added_ref == 10 and int(dnf.item()) == nf_ref
+    gA = dA.cpu().numpy().reshape(cap, dim)[:nf_ref]
+    gb = db.cpu().numpy()[:nf_ref]
+    np.testing.assert_allclose(gA, A[:nf_ref], rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(gb, b[:nf_ref], rtol=1e-4, atol=1e-6)
+
+
+def _boxes():
+    from csdecomp_amd import scenes
+    p = scenes.boxes_in_corners()
+    t = p.get_kinematic_tree()
+    dom = None
+    import csdecomp_amd as csd
+    dom = csd.HPolyhedron()
+    dom.MakeBox(t.get_position_lower_limits(), t.get_position_upper_limits())
+    return p, p.get_minimal_plant(), dom
+
+
+@pytest.mark.parametrize("case", [
+    # edge_inflation_zero_order_test.cpp FCItest1, FCItest2, FCItest3
+    ([0, 0], [1, 0], [[1.8, 0.2]], [[1.8, 0.4]]),
+    ([-0.5, 0], [0, 0.5], [[0.0, 0.0]], [[0.0, 0.8]]),
+    ([-0.5, 1.75], [0.5, 1.75], [[-1.75, 1.75], [1.75, 1.75]], []),
+])
+def test_eizo_boxes_in_corners(gpu, csd, orc, case):
+    ls, le, inside_pts, outside_pts = case
+    p, mp, dom = _boxes()
+    opt = csd.EizoOptions()
+    opt.num_particles = 1000
+    rg = list(p.get_scene_inspector().robot_geometry_ids)
+    mpoly, info = csd.EizoCuda(np.array(ls, np.float32), np.array(le, np.float32), None, 0.01, dom,
+                               mp, rg, opt)
+    P = csd.HPolyhedron(mpoly)
+    assert info.converged
+    assert P.PointInSet(np.array(ls, np.float32)) and P.PointInSet(np.array(le, np.float32))
+    for x in inside_pts:
+        assert P.PointInSet(np.array(x, np.float32))
+    for x in outside_pts:
+        assert not P.PointInSet(np.array(x, np.float32))
+    # certified fraction in collision < 1.1 eps over 1000 uniform samples
+    c = P.ChebyshevCenter()
+    s = csd.UniformSampleInHPolyhedraCuda([P], c[:, None], 1000, 50)[0]
+    frac = 1 - np.mean(csd.CheckCollisionFreeCuda(s, mp))
+    assert frac < opt.epsilon * 1.1
+
+
+def test_eizo_voxels(gpu, csd):
+    # FCI2dVoxelTest1 (edge_inflation_zero_order_test.cpp:498-586)
+    p, mp, dom = _boxes()
+    opt = csd.EizoOptions()
+    opt.num_particles = 1000
+    vox = csd.Voxels(np.array([[-0.55, -1.50, 1.90], [2.05, 1.80, 1.90], [0, 0, 0]], np.float32))
+    rg = list(p.get_scene_inspector().robot_geometry_ids)
+    inf = csd.CudaEdgeInflator(mp, rg, opt, dom)
+    P = inf.inflateEdge(np.array([-0.5, 1.75], np.float32), np.array([0.5, 1.75], np.float32), vox,
+                        0.05, False)
+    assert P.PointInSet(np.array([-0.5, 1.75], np.float32))
+    assert not P.PointInSet(np.array([-1.75, 1.75], np.float32))
+    assert P.PointInSet(np.array([1.75, 1.8], np.float32))
+
+
+def test_eizo_batched_7dof(gpu, csd, orc):
+    """Batched inflation of several 7-DOF edges: containment, determinism and
+    certification against the exact checker."""
+    from csdecomp_amd import scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    op = orc.Plant(mp)
+    rng = np.random.default_rng(21)
+    q = scenes.uniform_configs(31, 20000, lo, hi)
+    free = q[:, orc.check(op, q, orc.RULE_GPU) == 1]
+    R = 6
+    ls = free[:, :R].copy()
+    d = rng.normal(size=(7, R))
+    le = np.clip(ls + 0.3 * d / np.linalg.norm(d, axis=0), lo[:, None], hi[:, None]).astype(np.float32)
+    ok = [orc.check_edge(op, ls[:, r], le[:, r], 0.01) for r in range(R)]
+    ls, le = np.asfortranarray(ls[:, ok]), np.asfortranarray(le[:, ok])
+    R = ls.shape[1]
+    dom = csd.HPolyhedron()
+    dom.MakeBox(lo, hi)
+    opt = csd.EizoOptions()
+    opt.epsilon = 0.01
+    opt.num_particles = 2000
+    opt.bisection_steps = 9
+    opt.max_iterations = 60
+    rg = list(pl.getRobotGeometryIds())
+    inf = csd.CudaEdgeInflator(mp, rg, opt, dom, R, 10)
+    regions, infos = inf.inflateEdges(ls, le, None, 0.0, [])
+    regions2, _ = inf.inflateEdges(ls, le, None, 0.0, [])
+    for r in range(R):
+        P = regions[r]
+        assert infos[r].converged
+        assert P.PointInSet(ls[:, r], 1e-5) and P.PointInSet(le[:, r], 1e-5)
+        np.testing.assert_array_equal(P.A(), regions2[r].A())  # deterministic
+        c = ((ls[:, r] + le[:, r]) / 2).astype(np.float32)
+        s = csd.UniformSampleInHPolyhedraCuda([P], c[:, None], 3000, 50, seed=99 + r)[0]
+        frac = 1 - np.mean(csd.CheckCollisionFreeCuda(s, mp))
+        assert frac < 2.5 * opt.epsilon, (r, frac)
+
+
+def test_edges_vs_oracle(gpu, csd, orc):
+    import torch
+    from csdecomp_amd import capi, scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    op = orc.Plant(mp)
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    q = scenes.uniform_configs(41, 300, lo, hi)
+    free_nodes = q[:, orc.check(op, q, orc.RULE_GPU) == 1]
+    rng = np.random.default_rng(3)
+    nodes = np.concatenate([free_nodes, q[:, :40]], axis=1)
+    N = nodes.shape[1]
+    edges = np.array([(i, j) for i in range(N) for j in rng.choice(N, 4, replace=False) if i != j], np.int32)
+    dp = capi.DevicePlant(mp)
+    dn = torch.from_numpy(np.ascontiguousarray(nodes.T)).to(gpu)
+    de = torch.from_numpy(edges).to(gpu)
+    out = torch.empty(len(edges), dtype=torch.uint8, device=gpu)
+    capi.validate_edges(dp, dn, N, de, len(edges), 0.05, 0, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    ref = np.array([orc.check_edge(op, nodes[:, i], nodes[:, j], 0.05) for i, j in edges], np.uint8)
+    assert np.mean(got == ref) > 0.995 and ref.sum() > 0 and ref.sum() < len(ref)
+    assert csd.CheckEdgeCollisionFree(nodes[:, 0], nodes[:, 0], mp)
+
+
+def test_visibility_graph_reference_pattern(gpu, csd):
+    # cuda_visibility_graph_test.cpp:132-155
+    p, mp, _ = _boxes()
+    nodes = np.array([[0, 1.5, 0, -1.5, 0], [0, 0, 1.5, 0, -1.5]], np.float32)
+    A = csd.VisibilityGraph(nodes, mp, 0.1, 1000000)
+    exp = {(0, 1): 1, (0, 2): 1, (0, 3): 1, (0, 4): 1, (1, 2): 0, (1, 3): 1, (1, 4): 0, (2, 3): 0,
+           (2, 4): 1, (3, 4): 0}
+    for (i, j), v in exp.items():
+        assert A[i, j] == v, (i, j)
+    B = csd.VisibilityGraph(nodes, mp, 0.1, 100)
+    assert (A != B).nnz == 0
+
+
+def test_occupancy_vs_brute_force(gpu, orc):
+    import torch
+    from csdecomp_amd import capi, scenes
+    pl = scenes.arm_scene(1000, 16, 0)
+    mp = pl.getMinimalPlant()
+    op = orc.Plant(mp)
+    rg = list(pl.getRobotGeometryIds())
+    q = scenes.uniform_configs(51, 20, pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
+    res, n = 1.0 / 16, 16  # 16^3 grid over [-1, 1]^3
+    dp = capi.DevicePlant(mp)
+    grid = torch.zeros(n ** 3, dtype=torch.uint8, device=gpu)
+    capi.occupancy(dp, torch.from_numpy(np.ascontiguousarray(q)).to(gpu), q.shape[1], 1, q.shape[1], rg,
+                   [0, 0, 0], [2, 2, 2], res, grid)
+    torch.cuda.synchronize()
+    got = grid.cpu().numpy()
+    idx = np.arange(n)
+    cen = (np.float64(0) - 2 / 2.0 + (idx * 2 * np.float32(res)).astype(np.float32) + res).astype(np.float32)
+    vx = np.array([[cen[ix], cen[iy], cen[iz]] for iz in idx for iy in idx for ix in idx], np.float32)
+    vr = np.float32(np.sqrt(3.0) * res)
+    flags = orc.voxel_flags(op, q, vx.T, vr, rg, orc.RULE_GPU)  # [N][V], 1 = free
+    ref = (flags == 0).any(0).astype(np.uint8)
+    assert np.mean(got == ref) > 0.999 and ref.sum() > 50
+
+
+def test_knn_vs_brute_force(gpu):
+    import torch
+    from csdecomp_amd import capi
+    rng = np.random.default_rng(2)
+    nodes = rng.uniform(-np.pi, np.pi, (3000, 7)).astype(np.float32)
+    nodes[17] = nodes[5]  # exact duplicate is skipped like the node itself
+    k = 10
+    dn = torch.from_numpy(nodes).to(gpu)
+    idx = torch.empty(3000 * k, dtype=torch.int32, device=gpu)
+    d2 = torch.empty(3000 * k, device=gpu)
+    capi.knn(dn, 3000, 7, k, idx, d2)
+    torch.cuda.synchronize()
+    idx = idx.cpu().numpy().reshape(3000, k)
+    for i in range(0, 3000, 37):
+        dd = ((nodes - nodes[i]) ** 2).sum(1)
+        dd[dd <= np.finfo(np.float32).eps] = np.inf
+        order = np.lexsort((np.arange(3000), dd))[:k]
+        assert list(order) == list(idx[i]), i
+    assert 17 not in idx[5] and 5 not in idx[17]
