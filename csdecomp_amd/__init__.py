@@ -8,6 +8,15 @@ is no CPU fallback. Importing fails loudly if the native module is missing
 """
 import os as _os
 
+# PyTorch-ROCm wheels ship their own libamdhip64 (SONAME libamdhip64.so.7,
+# requested by torch as "libamdhip64.so"). If this package's library loaded
+# first, torch would later map a second HIP runtime and see no GPU. Loading
+# torch first (when it is installed) makes both share one runtime.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:
+    pass
+
 try:
     from . import _pycsdecomp
 except ImportError as _e:  # pragma: no cover - exercised only on broken installs

@@ -26,12 +26,9 @@ HOST_SOURCES = ["csdecomp.cpp", "bindings.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 in generated code.
-# promote-alloca limit: lets the per-lane register arrays (body frames,
-# sphere centres) stay in VGPRs under wave-uniform dynamic indexing.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-mllvm", "-amdgpu-promote-alloca-to-vector-limit=4096",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
 

@@ -39,32 +39,60 @@ template <int NB, int KM, bool ANY = false, class Q>
 __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
                                               const VoxelArgs &V, bool col) {
   constexpr int NS = NB * KM;
-  CSD_BODY_FRAMES(B, NB);
-  fk_bodies<NB>(P, qa, B);
-
-  CSD_SLOT_ARRAY(cx, NS);
-  CSD_SLOT_ARRAY(cy, NS);
-  CSD_SLOT_ARRAY(cz, NS);
+  // Chain FK fused with the sphere centres: only the current body frame is
+  // live, and every register index below is a compile-time constant.
+  float cx[NS], cy[NS], cz[NS];
+  {
+    float X[12];
+    aff_identity(X);
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    cx.at(s) = cy.at(s) = cz.at(s) = 0.f;
-    if (s / KM < P.num_bodies) {
-      const float4 sp = P.slot_sphere[s];
-      if (sp.w >= 0.f) {
-        float X[12];
-        B.get(s / KM, X);
-        aff_point(X, sp.x, sp.y, sp.z, cx.at(s), cy.at(s), cz.at(s));
+    for (int b = 0; b < NB; ++b) {
+      if (b < P.num_bodies) {
+        const DevBody &bd = P.bodies[b];
+        float M[12];
+        joint_local(bd, qa(bd.q_idx), M);
+        if (bd.parent < 0) {
+#pragma unroll
+          for (int k = 0; k < 12; ++k) X[k] = M[k];
+        } else {
+          float Y[12];
+          aff_mul(X, M, Y);
+#pragma unroll
+          for (int k = 0; k < 12; ++k) X[k] = Y[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const float4 sp = P.slot_sphere[b * KM + k];
+          aff_point(X, sp.x, sp.y, sp.z, cx[b * KM + k], cy[b * KM + k], cz[b * KM + k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < KM; ++k) cx[b * KM + k] = cy[b * KM + k] = cz[b * KM + k] = 0.f;
       }
     }
   }
   col = col || P.always_colliding;
 
-  // moving sphere x moving sphere (self collision): cheap, test first
-  for (int p = 0; p < P.num_self_pairs; ++p) {
-    if ((p & 31) == 0 && wave_done<ANY>(col)) break;
-    const int4 sp = P.self_pairs[p];
-    col |= hit_sphere_sphere(cx.at(sp.x), cy.at(sp.x), cz.at(sp.x), cx.at(sp.y), cy.at(sp.y),
-                             cz.at(sp.y), __int_as_float(sp.z));
+  // moving sphere x moving sphere (self collision) as body-pair blocks;
+  // absent sphere pairs carry r2 = 0 and can never report a hit
+#pragma unroll
+  for (int ba = 0; ba < NB; ++ba) {
+    const uint64_t act = P.self_active[ba];
+    if (act == 0) continue;
+    if (wave_done<ANY>(col)) break;
+#pragma unroll
+    for (int bb = ba + 1; bb < NB; ++bb) {
+      if ((act >> bb) & 1) {
+        const float *r2 = P.self_r2 + ((size_t)ba * NB + bb) * (KM * KM);
+#pragma unroll
+        for (int ka = 0; ka < KM; ++ka)
+#pragma unroll
+          for (int kb = 0; kb < KM; ++kb)
+            col |= hit_sphere_sphere(cx[ba * KM + ka], cy[ba * KM + ka], cz[ba * KM + ka],
+                                     cx[bb * KM + kb], cy[bb * KM + kb], cz[bb * KM + kb],
+                                     r2[ka * KM + kb]);
+      }
+    }
   }
 
   // moving sphere x world-fixed obstacle
@@ -79,7 +107,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
         const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
         if (on) {
           const float r = P.slot_sphere[s].w;
-          col |= hit_sphere_obox(cx.at(s), cy.at(s), cz.at(s), r * r, ov);
+          col |= hit_sphere_obox(cx[s], cy[s], cz[s], r * r, ov);
         }
       }
     } else if (t == CSD_SPHERE) {
@@ -88,7 +116,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
         const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
         if (on) {
           const float rs = P.slot_sphere[s].w + ov[3];
-          col |= hit_sphere_sphere(cx.at(s), cy.at(s), cz.at(s), ov[0], ov[1], ov[2], rs * rs);
+          col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], rs * rs);
         }
       }
     } else {  // capsule
@@ -97,7 +125,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
         const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
         if (on) {
           const float rs = P.slot_sphere[s].w + ov[7];
-          col |= seg_point_d2(cx.at(s), cy.at(s), cz.at(s), ov[0], ov[1], ov[2], ov[3], ov[4],
+          col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4],
                               ov[5], ov[6]) < rs * rs;
         }
       }
@@ -113,7 +141,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
       const bool on = s < 64 ? ((V.mask0 >> s) & 1) : ((V.mask1 >> (s - 64)) & 1);
       if (on) {
         const float rs = V.radius + P.slot_sphere[s].w;
-        col |= hit_sphere_sphere(vx, vy, vz, cx.at(s), cy.at(s), cz.at(s), rs * rs);
+        col |= hit_sphere_sphere(vx, vy, vz, cx[s], cy[s], cz[s], rs * rs);
       }
     }
   }

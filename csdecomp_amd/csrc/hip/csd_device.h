@@ -65,33 +65,36 @@ __device__ __forceinline__ void joint_local(const DevBody &bd, float v, float *M
   }
 }
 
-// Body frames of one configuration held in VGPRs as twelve per-component
-// arrays (each at most 16 floats, so each becomes one VGPR tuple). A
-// wave-uniform body index then lowers to s_set_gpr_idx moves, not scratch.
-// The arrays are separate locals (CSD_BODY_FRAMES declares them) bound by
-// reference: a struct holding the arrays themselves is kept in scratch.
+// Wave-uniform dynamic index into a register array, as a select chain: every
+// array access has a compile-time index, so the arrays stay scalar VGPRs
+// (no scratch, no whole-vector copies).
+template <int N>
+__device__ __forceinline__ float pick(const float *v, int j) {
+  float r = v[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) r = (j == k) ? v[k] : r;
+  return r;
+}
+
+// Body frames of one configuration held in VGPRs; the parent frame of a
+// body is fetched with a select chain on the (wave-uniform) body index.
 template <int NB>
 struct BodyFrames {
-  using A = float[NB];
-  A &m0, &m1, &m2, &m3, &m4, &m5, &m6, &m7, &m8, &m9, &m10, &m11;
+  float m[NB][12];
   __device__ __forceinline__ void get(int b, float *X) const {
-    X[0] = m0[b]; X[1] = m1[b]; X[2] = m2[b]; X[3] = m3[b];
-    X[4] = m4[b]; X[5] = m5[b]; X[6] = m6[b]; X[7] = m7[b];
-    X[8] = m8[b]; X[9] = m9[b]; X[10] = m10[b]; X[11] = m11[b];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) X[k] = m[0][k];
+#pragma unroll
+    for (int c = 1; c < NB; ++c)
+#pragma unroll
+      for (int k = 0; k < 12; ++k) X[k] = (b == c) ? m[c][k] : X[k];
   }
   __device__ __forceinline__ void set(int b, const float *X) {
-    m0[b] = X[0]; m1[b] = X[1]; m2[b] = X[2]; m3[b] = X[3];
-    m4[b] = X[4]; m5[b] = X[5]; m6[b] = X[6]; m7[b] = X[7];
-    m8[b] = X[8]; m9[b] = X[9]; m10[b] = X[10]; m11[b] = X[11];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[b][k] = X[k];  // b is a compile-time index here
   }
 };
-#define CSD_BODY_FRAMES(NAME, NB)                                                   \
-  float NAME##_0[NB], NAME##_1[NB], NAME##_2[NB], NAME##_3[NB], NAME##_4[NB],       \
-      NAME##_5[NB], NAME##_6[NB], NAME##_7[NB], NAME##_8[NB], NAME##_9[NB],         \
-      NAME##_10[NB], NAME##_11[NB];                                                 \
-  ::csd::BodyFrames<NB> NAME{NAME##_0, NAME##_1, NAME##_2, NAME##_3, NAME##_4,      \
-                             NAME##_5, NAME##_6, NAME##_7, NAME##_8, NAME##_9,      \
-                             NAME##_10, NAME##_11}
+#define CSD_BODY_FRAMES(NAME, NB) ::csd::BodyFrames<NB> NAME
 
 // Configuration accessors: element j of the lane's configuration, either
 // from HBM with strides or from a register array (EI-ZO kernels generate
@@ -101,9 +104,10 @@ struct QMem {
   int64_t base, sj;
   __device__ __forceinline__ float operator()(int j) const { return q[base + (int64_t)j * sj]; }
 };
+template <int N>
 struct QReg {
-  const float *v;  // register array, indexed with a wave-uniform j
-  __device__ __forceinline__ float operator()(int j) const { return v[j]; }
+  const float *v;  // register array of N floats, indexed with a wave-uniform j
+  __device__ __forceinline__ float operator()(int j) const { return pick<N>(v, j); }
 };
 
 // Forward kinematics of all bodies of one configuration.
@@ -129,33 +133,6 @@ __device__ __forceinline__ void fk_bodies(const PlantImage &P, const Q &qa, Body
   }
 }
 
-// Per-slot scalars (sphere centre coordinates) in 32-float chunks: a
-// wave-uniform slot index selects the chunk with a scalar branch and the
-// element with s_set_gpr_idx.
-template <int NS>
-struct SlotArray {
-  using A0 = float[32];
-  using A1 = float[NS > 32 ? 32 : 1];
-  using A2 = float[NS > 64 ? 32 : 1];
-  using A3 = float[NS > 96 ? 32 : 1];
-  A0 &a0;
-  A1 &a1;
-  A2 &a2;
-  A3 &a3;
-  __device__ __forceinline__ float &at(int s) {
-    if (NS <= 32) return a0[s];
-    switch (s >> 5) {
-      case 0: return a0[s & 31];
-      case 1: return a1[(s & 31) % (NS > 32 ? 32 : 1)];
-      case 2: return a2[(s & 31) % (NS > 64 ? 32 : 1)];
-      default: return a3[(s & 31) % (NS > 96 ? 32 : 1)];
-    }
-  }
-};
-#define CSD_SLOT_ARRAY(NAME, NS)                                                  \
-  float NAME##_0[32], NAME##_1[NS > 32 ? 32 : 1], NAME##_2[NS > 64 ? 32 : 1],     \
-      NAME##_3[NS > 96 ? 32 : 1];                                                 \
-  ::csd::SlotArray<NS> NAME{NAME##_0, NAME##_1, NAME##_2, NAME##_3}
 
 // ---------------------------------------------------------------------------
 // pair primitives (hit == true means colliding)

@@ -10,6 +10,7 @@ namespace csd {
 
 constexpr int kEdgeBlock = 256;
 constexpr int kEdgeDim = 16;
+constexpr float kMaxEdgePoints = 1.0e8f;  // longer edges are rejected, not checked
 
 // sequential float sum of squares without contraction, the order the CPU
 // restatement uses (so step counts agree bit for bit)
@@ -31,7 +32,8 @@ __device__ __forceinline__ float norm_diff_rn(const float *a, const float *b, in
 //         t = (k+1)/(n-1) for k < n-2 (planning/cuda_visibility_graph.cu:58-172)
 template <int VAR>
 __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
-    PlantImage P, const float *__restrict__ nodes, int dim, const int32_t *__restrict__ edges,
+    PlantImage P, const float *__restrict__ nodes, int num_nodes, int dim,
+    const int32_t *__restrict__ edges,
     int64_t E, float step, int rule, uint8_t *__restrict__ out) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -41,6 +43,10 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
   R.n = 0;
   for (int64_t e = wave; e < E; e += nwaves) {
     const int a = edges[2 * e], b = edges[2 * e + 1];
+    if (a < 0 || a >= num_nodes || b < 0 || b >= num_nodes) {
+      if (lane == 0) out[e] = 0;  // invalid node index: reported as not free
+      continue;
+    }
     float q1[kEdgeDim], q2[kEdgeDim];
 #pragma unroll
     for (int d = 0; d < kEdgeDim; ++d) {
@@ -48,6 +54,10 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
       q2[d] = d < dim ? nodes[(size_t)b * dim + d] : 0.f;
     }
     const float dist = norm_diff_rn(q1, q2, dim);
+    if (!(dist / step < kMaxEdgePoints)) {  // non-finite or absurdly long edge
+      if (lane == 0) out[e] = 0;
+      continue;
+    }
     int64_t npts;
     float tstep = 0.f;
     if (rule == 0) {
@@ -70,7 +80,7 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
 #pragma unroll
       for (int d = 0; d < kEdgeDim; ++d)
         q[d] = __fadd_rn(__fmul_rn(t, q2[d]), __fmul_rn(__fsub_rn(1.f, t), q1[d]));
-      const bool col = collides_variant<VAR, true>(P, QReg{q}, V, R, !valid) && valid;
+      const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, !valid) && valid;
       any = __any(col);
     }
     if (lane == 0) out[e] = any ? 0 : 1;
@@ -228,7 +238,7 @@ int csd_validate_edges(const csd_plant *plant, const float *nodes, int32_t num_n
   const int64_t waves = std::min<int64_t>(num_edges, 256 * 64);
   dim3 g((unsigned)((waves * 64 + kEdgeBlock - 1) / kEdgeBlock));
   CSD_DISPATCH_VARIANT(var, k_validate_edges, g, dim3(kEdgeBlock), 0, (hipStream_t)stream,
-                       plant->img, nodes, dim, edges, num_edges, step, rule, out);
+                       plant->img, nodes, num_nodes, dim, edges, num_edges, step, rule, out);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

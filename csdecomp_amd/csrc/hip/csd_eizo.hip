@@ -73,6 +73,8 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
         u[4 * c + 1] = u01(o.y);
         u[4 * c + 2] = u01(o.z);
         u[4 * c + 3] = u01(o.w);
+      } else {
+        u[4 * c] = u[4 * c + 1] = u[4 * c + 2] = u[4 * c + 3] = 0.5f;
       }
     }
     float dir[kDimMax];
@@ -103,7 +105,7 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
     }
     if (isinf(tmax) || isinf(tmin) || tmax < tmin) continue;
     // theta = u (0.999999 tmax - tmin) + tmin, in double as the reference
-    const float theta = (float)((double)u[2 * npairs] *
+    const float theta = (float)((double)pick<kDimMax + 4>(u, 2 * npairs) *
                                     (0.999999 * (double)tmax - (double)tmin) +
                                 (double)tmin);
 #pragma unroll
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample_check(
   if (blockIdx.x * kSampleBlock < N)
     hit_and_run(D.A + (size_t)r * cap * dim, D.b + (size_t)r * cap, D.nf[r], dim, x, i,
                 (uint32_t)r, round, mixing, k0, k1);
-  const bool col = collides_variant<VAR>(P, QReg{x}, V, R, !valid) && valid;
+  const bool col = collides_variant<VAR>(P, QReg<kDimMax>{x}, V, R, !valid) && valid;
 
   __shared__ uint32_t wcnt[kSampleBlock / 64], wfirst[kSampleBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
     for (int d = 0; d < kDimMax; ++d) mid[d] = 0.5f * (hi[d] + lo[d]);
-    const bool col = collides_variant<VAR>(P, QReg{mid}, V, R, !valid);
+    const bool col = collides_variant<VAR>(P, QReg<kDimMax>{mid}, V, R, !valid);
 #pragma unroll
     for (int d = 0; d < kDimMax; ++d) {
       hi[d] = col ? mid[d] : hi[d];
@@ -678,7 +680,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_bisect_standalone(
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
     for (int d = 0; d < kDimMax; ++d) mid[d] = 0.5f * (hi[d] + lo[d]);
-    const bool col = collides_variant<VAR>(P, QReg{mid}, V, R, !valid);
+    const bool col = collides_variant<VAR>(P, QReg<kDimMax>{mid}, V, R, !valid);
 #pragma unroll
     for (int d = 0; d < kDimMax; ++d) {
       hi[d] = col ? mid[d] : hi[d];

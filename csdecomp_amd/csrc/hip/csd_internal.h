@@ -86,7 +86,11 @@ struct PlantImage {
   const DevObstacle *obstacles;
   const int32_t *obstacle_type;   // [num_obstacles]
   const uint64_t *obstacle_mask;  // [num_obstacles][2] slot masks
-  const int4 *self_pairs;         // [num_self_pairs] (slotA, slotB, (rA+rB)^2 bits, 0)
+  // self collision of the fast path as body-pair blocks: bit bb of
+  // self_active[ba] says bodies ba < bb have sphere pairs; self_r2 holds
+  // (rA+rB)^2 per (ba, bb, ka, kb), 0 for absent pairs (never a hit)
+  const uint64_t *self_active;    // [nbt]
+  const float *self_r2;           // [nbt][nbt][kmax][kmax]
   const int32_t *geom_slot;       // [G] slot of moving sphere geometry, -1 else
 };
 

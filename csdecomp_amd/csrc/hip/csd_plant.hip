@@ -387,7 +387,11 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   const int NP = (int)pair_types.size();
 
   // ---- fast-path tables: every moving geometry in a pair is a sphere ----
+  // the fast kernel also needs chain-ordered bodies (parent is the previous
+  // body or the world), so FK keeps only the current frame live
   bool fast = NB <= 16;
+  for (int b = 0; b < NB && fast; ++b)
+    if (bodies[b].parent != -1 && bodies[b].parent != b - 1) fast = false;
   std::vector<int> per_body(NB, 0);
   for (int p = 0; p < NP && fast; ++p)
     for (int s = 0; s < 2; ++s) {
@@ -408,7 +412,10 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   std::vector<DevObstacle> obstacles;
   std::vector<int32_t> obstacle_type;
   std::vector<uint64_t> obstacle_mask;
-  std::vector<int32_t> self_pairs;  // (slotA, slotB, bits(rsum^2), 0)
+  const int nbt_alloc = fast ? nbt : 1, km_alloc = fast ? kmax : 1;
+  std::vector<uint64_t> self_active(nbt_alloc, 0);
+  std::vector<float> self_r2((size_t)nbt_alloc * nbt_alloc * km_alloc * km_alloc, 0.f);
+  int num_self_pairs = 0;
   if (fast) {
     std::vector<int> used(NB, 0);
     for (int g = 0; g < G; ++g) {
@@ -424,14 +431,12 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
       int ma = gbody[a] >= 0, mb = gbody[b] >= 0;
       if (ma && mb) {
         int sa = geom_slot[a], sb = geom_slot[b];
-        float rs = geoms[a].dims[0] + geoms[b].dims[0];
-        float r2 = rs * rs;
-        int32_t bits;
-        memcpy(&bits, &r2, 4);
-        self_pairs.push_back(std::min(sa, sb));
-        self_pairs.push_back(std::max(sa, sb));
-        self_pairs.push_back(bits);
-        self_pairs.push_back(0);
+        if (sa > sb) std::swap(sa, sb);
+        const int ba = sa / kmax, bb = sb / kmax, ka = sa % kmax, kb = sb % kmax;
+        const float rs = geoms[a].dims[0] + geoms[b].dims[0];
+        self_active[ba] |= (uint64_t)1 << bb;
+        self_r2[(((size_t)ba * nbt + bb) * kmax + ka) * kmax + kb] = rs * rs;
+        ++num_self_pairs;
         continue;
       }
       int sg = ma ? a : b;  // moving sphere
@@ -502,7 +507,8 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   size_t o_obs = bld.add(obstacles.data(), obstacles.size());
   size_t o_otype = bld.add(obstacle_type.data(), obstacle_type.size());
   size_t o_omask = bld.add(obstacle_mask.data(), obstacle_mask.size());
-  size_t o_self = bld.add(self_pairs.data(), self_pairs.size());
+  size_t o_sact = bld.add(self_active.data(), self_active.size());
+  size_t o_sr2 = bld.add(self_r2.data(), self_r2.size());
   size_t o_gslot = bld.add(geom_slot.data(), geom_slot.size());
 
   csd_plant *pl = new csd_plant();
@@ -535,7 +541,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   img.kmax = fast ? kmax : 0;
   img.nslots = nslots;
   img.num_obstacles = (int)obstacle_type.size();
-  img.num_self_pairs = (int)(self_pairs.size() / 4);
+  img.num_self_pairs = num_self_pairs;
   img.bodies = (const DevBody *)(base + o_bodies);
   img.geoms = (const DevGeom *)(base + o_geoms);
   img.pairs = (const int32_t *)(base + o_pairs);
@@ -545,7 +551,8 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   img.obstacles = (const DevObstacle *)(base + o_obs);
   img.obstacle_type = (const int32_t *)(base + o_otype);
   img.obstacle_mask = (const uint64_t *)(base + o_omask);
-  img.self_pairs = (const int4 *)(base + o_self);
+  img.self_active = (const uint64_t *)(base + o_sact);
+  img.self_r2 = (const float *)(base + o_sr2);
   img.geom_slot = (const int32_t *)(base + o_gslot);
   pl->geom_type.assign(d->geom_type, d->geom_type + G);
   pl->geom_link.assign(d->geom_link, d->geom_link + G);

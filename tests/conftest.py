@@ -46,8 +46,9 @@ def gpu():
     from csdecomp_amd import capi
     capi.init(0)
     import torch
-    assert torch.cuda.is_available()
-    return torch.device("cuda:0")
+    dev = torch.device("cuda:0")
+    torch.zeros(1, device=dev)  # device memory must be usable
+    return dev
 
 
 BAND = 1e-5  # flags may differ only within 1e-5 m of contact (FP64 distance)
