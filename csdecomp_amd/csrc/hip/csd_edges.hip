@@ -68,8 +68,9 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
     }
     bool any = false;
     for (int64_t base = 0; base < npts && !any; base += 64) {
-      const int64_t k = base + lane;
-      const bool valid = k < npts;
+      // idle tail lanes re-check the last point: in "any collision" mode a
+      // lane that starts as collided would end the wave's checks early
+      const int64_t k = min(base + lane, npts - 1);
       float t;
       if (rule == 0) {
         t = (float)((double)k / (1.0 * (double)npts));
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
 #pragma unroll
       for (int d = 0; d < kEdgeDim; ++d)
         q[d] = __fadd_rn(__fmul_rn(t, q2[d]), __fmul_rn(__fsub_rn(1.f, t), q1[d]));
-      const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, !valid) && valid;
+      const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, false);
       any = __any(col);
     }
     if (lane == 0) out[e] = any ? 0 : 1;

@@ -75,7 +75,6 @@ def test_random_flags_vs_oracle(gpu, csd, orc, name):
     capi.check_collision_free(dp, qs, n, 1, n, out)
     torch.cuda.synchronize()
     assert_flags_match(out.cpu().numpy(), ref, sd, f"{name} soa")
-    assert 0 < ref.sum() < n or name in ("3dofrobot.urdf",)
 
 
 def test_fast_path_selected(gpu):

@@ -9,6 +9,26 @@ from conftest import parse
 pytestmark = pytest.mark.gpu
 
 
+def free_edges(orc, op, lo, hi, count, length, seed, step=0.01):
+    """`count` straight edges with collision-free endpoints whose
+    checkEdgeCollisionFree discretisation (step 0.01) is free."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.asarray(lo, np.float32), np.asarray(hi, np.float32)
+    starts, ends = [], []
+    while len(starts) < count:
+        q = rng.uniform(lo, hi, size=(4096, len(lo))).astype(np.float32)
+        free = q[orc.check(op, q.T, orc.RULE_GPU) == 1]
+        for q1 in free:
+            d = rng.normal(size=len(lo))
+            q2 = np.clip(q1 + rng.uniform(*length) * d / np.linalg.norm(d), lo, hi).astype(np.float32)
+            if orc.check_edge(op, q1, q2, step) and orc.check(op, q2[:, None], orc.RULE_GPU)[0]:
+                starts.append(q1)
+                ends.append(q2)
+                if len(starts) == count:
+                    break
+    return np.asfortranarray(np.array(starts).T), np.asfortranarray(np.array(ends).T)
+
+
 def _box_domain(lo, hi):
     d = len(lo)
     A = np.zeros((2 * d, d), np.float32)
@@ -55,9 +75,27 @@ def test_uniform_sample_api(gpu, csd):
         csd.UniformSampleInHPolyhedraCuda([P1], np.array([[5.0, 0, 0]], np.float32).T, 10, 5)
 
 
+def bisect_outcomes(orc, op, p, lo, steps, band=1e-5):
+    """Every result the bisection of cuda_set_builder_utils.cu:239-319 can
+    reach when decisions at midpoints within `band` of contact (FP64 signed
+    distance) may go either way; float32 arithmetic as on the device."""
+    half = np.float32(0.5)
+    states = [(p.astype(np.float32), lo.astype(np.float32), p.astype(np.float32))]
+    for _ in range(steps):
+        nxt = []
+        for hi, lo_, sc in states:
+            mid = ((hi + lo_) * half).astype(np.float32)
+            col = orc.check(op, mid[:, None], orc.RULE_GPU)[0] == 0
+            sd = orc.min_sd(op, mid[:, None])[0]
+            for c in ({True, False} if abs(sd) < band else {col}):
+                nxt.append((mid, lo_, mid) if c else (hi, mid, sc))
+        states = nxt
+    return [s[2] for s in states]
+
+
 def test_bisection_stage_parity(gpu, orc):
-    """Identical particle sets -> identical optimised collisions (exact unless
-    a midpoint falls in the contact band)."""
+    """Identical particle sets -> identical optimised collisions, except where
+    a midpoint falls in the contact band (then any reachable outcome)."""
     import torch
     from csdecomp_amd import capi, scenes
     pl = scenes.arm_scene(1001, 48, 16)
@@ -81,9 +119,13 @@ def test_bisection_stage_parity(gpu, orc):
     for k in range(n):
         pr, _ = orc.project(col[:, k], ls, le)
         np.testing.assert_allclose(proj[k], pr, atol=1e-6)
-        ref = orc.bisect(op, col[:, k], pr, 10)
-        exact += np.array_equal(ref, scol[k])
-    assert exact >= n - 3
+        ref = orc.bisect(op, col[:, k], proj[k], 10)
+        if np.array_equal(ref, scol[k]):
+            exact += 1
+            continue
+        outs = bisect_outcomes(orc, op, col[:, k], proj[k], 10)
+        assert any(np.array_equal(o, scol[k]) for o in outs), (k, scol[k], ref)
+    assert exact >= 0.95 * n
 
 
 def test_face_stage_parity(gpu, orc):
@@ -177,16 +219,9 @@ def test_eizo_batched_7dof(gpu, csd, orc):
     mp = pl.getMinimalPlant()
     lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
     op = orc.Plant(mp)
-    rng = np.random.default_rng(21)
-    q = scenes.uniform_configs(31, 20000, lo, hi)
-    free = q[:, orc.check(op, q, orc.RULE_GPU) == 1]
-    R = 6
-    ls = free[:, :R].copy()
-    d = rng.normal(size=(7, R))
-    le = np.clip(ls + 0.3 * d / np.linalg.norm(d, axis=0), lo[:, None], hi[:, None]).astype(np.float32)
-    ok = [orc.check_edge(op, ls[:, r], le[:, r], 0.01) for r in range(R)]
-    ls, le = np.asfortranarray(ls[:, ok]), np.asfortranarray(le[:, ok])
+    ls, le = free_edges(orc, op, lo, hi, 6, (0.1, 0.4), seed=21)
     R = ls.shape[1]
+    assert R == 6
     dom = csd.HPolyhedron()
     dom.MakeBox(lo, hi)
     opt = csd.EizoOptions()
@@ -216,12 +251,14 @@ def test_edges_vs_oracle(gpu, csd, orc):
     mp = pl.getMinimalPlant()
     op = orc.Plant(mp)
     lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
-    q = scenes.uniform_configs(41, 300, lo, hi)
-    free_nodes = q[:, orc.check(op, q, orc.RULE_GPU) == 1]
+    ls, le = free_edges(orc, op, lo, hi, 60, (0.05, 0.5), seed=41, step=0.05)
+    q = scenes.uniform_configs(41, 80, lo, hi)
     rng = np.random.default_rng(3)
-    nodes = np.concatenate([free_nodes, q[:, :40]], axis=1)
+    nodes = np.concatenate([ls, le, q], axis=1)
     N = nodes.shape[1]
-    edges = np.array([(i, j) for i in range(N) for j in rng.choice(N, 4, replace=False) if i != j], np.int32)
+    edges = [(i, 60 + i) for i in range(60)]  # free by construction
+    edges += [(i, j) for i in range(N) for j in rng.choice(N, 3, replace=False) if i != j]
+    edges = np.array(edges, np.int32)
     dp = capi.DevicePlant(mp)
     dn = torch.from_numpy(np.ascontiguousarray(nodes.T)).to(gpu)
     de = torch.from_numpy(edges).to(gpu)
@@ -230,7 +267,14 @@ def test_edges_vs_oracle(gpu, csd, orc):
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     ref = np.array([orc.check_edge(op, nodes[:, i], nodes[:, j], 0.05) for i, j in edges], np.uint8)
-    assert np.mean(got == ref) > 0.995 and ref.sum() > 0 and ref.sum() < len(ref)
+    assert ref[:60].all() and not ref.all()
+    for e in np.nonzero(got != ref)[0]:
+        # a mismatch is only allowed if a checked point lies in the contact band
+        q1, q2 = nodes[:, edges[e][0]], nodes[:, edges[e][1]]
+        n = orc.edge_num_steps(q1, q2, 0.05)
+        t = np.array([np.float32(k / (1.0 * n)) for k in range(n)], np.float32)
+        pts = (t[None, :] * q2[:, None] + (np.float32(1) - t)[None, :] * q1[:, None]).astype(np.float32)
+        assert np.abs(orc.min_sd(op, pts)).min() < 1e-5, e
     assert csd.CheckEdgeCollisionFree(nodes[:, 0], nodes[:, 0], mp)
 
 
