@@ -138,7 +138,7 @@ def main():
     f_cfg = algorithmic_flops_per_config(mp, plant)
     achieved_tflops = n * f_cfg / (kern_ms * 1e-3) / 1e12
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "check_fast_cfg0_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "k_check_arm7_16box_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")

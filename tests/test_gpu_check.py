@@ -13,9 +13,9 @@ pytestmark = pytest.mark.gpu
 
 def _synthetic(name):
     from csdecomp_amd import scenes
-    return {"cfg0": lambda: scenes.arm_scene(1000, 16, 0),
-            "cfg1": lambda: scenes.arm_scene(1001, 48, 16),
-            "cfg2": lambda: scenes.dual_arm_scene(1002, 72, 24)}[name]()
+    return {"arm7_16box": lambda: scenes.arm_scene(1000, 16, 0),
+            "arm7_48box16sph": lambda: scenes.arm_scene(1001, 48, 16),
+            "dualarm14_72box24sph": lambda: scenes.dual_arm_scene(1002, 72, 24)}[name]()
 
 
 @pytest.mark.parametrize("fname,free,col", KNOWN, ids=[k[0] for k in KNOWN])
@@ -41,7 +41,7 @@ def test_known_answer_voxels_api(gpu, csd):
 
 
 PLANTS = ["2_branch_5dof.urdf", "3dofrobot.urdf", "3dofrobot_prismatic.urdf",
-          "3dofrobot_spheres.urdf", "movable_block.urdf", "cfg0", "cfg1", "cfg2"]
+          "3dofrobot_spheres.urdf", "movable_block.urdf", "arm7_16box", "arm7_48box16sph", "dualarm14_72box24sph"]
 
 
 def _plant(name):
@@ -60,7 +60,7 @@ def test_random_flags_vs_oracle(gpu, csd, orc, name):
     import torch
     from csdecomp_amd import capi, scenes
     mp, lo, hi = _plant(name)
-    n = 20000 if name != "cfg2" else 8000
+    n = 20000 if name != "dualarm14_72box24sph" else 8000
     q = scenes.uniform_configs(2000 + len(name), n, lo, hi)  # (dof, n)
     op = orc.Plant(mp)
     sd = orc.min_sd(op, q)
@@ -79,7 +79,7 @@ def test_random_flags_vs_oracle(gpu, csd, orc, name):
 
 def test_fast_path_selected(gpu):
     from csdecomp_amd import capi
-    for name in ("cfg0", "cfg1", "cfg2"):
+    for name in ("arm7_16box", "arm7_48box16sph", "dualarm14_72box24sph"):
         mp, _, _ = _plant(name)
         assert capi.DevicePlant(mp).info()["fast_path"], name
     mp, _, _ = _plant("2_branch_5dof.urdf")
@@ -106,7 +106,7 @@ def test_fk_golden_gpu(gpu, csd):
                       [0, 0, 1, 0], [0, 0, 0, 1]])
 
 
-@pytest.mark.parametrize("name", ["2_branch_5dof.urdf", "3dofrobot_prismatic.urdf", "cfg0", "cfg2"])
+@pytest.mark.parametrize("name", ["2_branch_5dof.urdf", "3dofrobot_prismatic.urdf", "arm7_16box", "dualarm14_72box24sph"])
 def test_fk_error_per_link(gpu, csd, orc, name):
     """FP32 GPU FK against the FP64 oracle, per link (max abs over the 3x4)."""
     from csdecomp_amd import scenes
@@ -227,7 +227,7 @@ def test_pair_collision_free(gpu, csd):
 def test_empty_and_errors(gpu, csd):
     import torch
     from csdecomp_amd import capi
-    mp, _, _ = _plant("cfg0")
+    mp, _, _ = _plant("arm7_16box")
     assert list(csd.CheckCollisionFreeCuda(np.zeros((7, 0), np.float32), mp)) == []
     with pytest.raises(ValueError):
         csd.CheckCollisionFreeCuda(np.zeros((6, 3), np.float32), mp)

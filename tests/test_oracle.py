@@ -250,7 +250,7 @@ def test_hit_and_run_uniform_on_box(orc):
 
 # ---------------------------------------------------------------- EI-ZO stages
 def test_num_samples_unadaptive(orc):
-    # SURVEY M_0 = 27,948 at eps 1e-3, delta 0.05, tau 0.5 (closed form)
+    # M_0 = 27,948 at eps 1e-3, delta 0.05, tau 0.5 (closed form)
     d0 = orc.delta_k(0.05, 0)
     assert d0 == np.float32(np.float32(6 * np.float32(0.05)) / (math.pi ** 2))
     assert orc.num_samples_unadaptive(1e-3, d0, 0.5) == 27948
