@@ -95,54 +95,68 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
     }
   }
 
-  // moving sphere x world-fixed obstacle
+  // moving sphere x world-fixed obstacle. The record and its row of contact
+  // distances are read once per obstacle (wave-uniform scalar loads); slots
+  // not paired with the obstacle carry r2 = -1 and never report a hit, so the
+  // slot loop has no per-slot branch. Slots of absent bodies are skipped.
   for (int o = 0; o < P.num_obstacles; ++o) {
     if (wave_done<ANY>(col)) break;
     const int t = P.obstacle_type[o];
-    const float *ov = P.obstacles[o].v;
-    const uint64_t m0 = P.obstacle_mask[2 * o], m1 = P.obstacle_mask[2 * o + 1];
+    float ov[16], r2[NS];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) r2[s] = P.obstacle_r2[(size_t)o * NS + s];
     if (t == CSD_BOX) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
-        if (on) {
-          const float r = P.slot_sphere[s].w;
-          col |= hit_sphere_obox(cx[s], cy[s], cz[s], r * r, ov);
+      for (int b = 0; b < NB; ++b) {
+        if (b >= P.num_bodies) break;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const int s = b * KM + k;
+          col |= hit_sphere_obox(cx[s], cy[s], cz[s], r2[s], ov);
         }
       }
     } else if (t == CSD_SPHERE) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
-        if (on) {
-          const float rs = P.slot_sphere[s].w + ov[3];
-          col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], rs * rs);
+      for (int b = 0; b < NB; ++b) {
+        if (b >= P.num_bodies) break;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const int s = b * KM + k;
+          col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], r2[s]);
         }
       }
     } else {  // capsule
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
-        if (on) {
-          const float rs = P.slot_sphere[s].w + ov[7];
-          col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4],
-                              ov[5], ov[6]) < rs * rs;
+      for (int b = 0; b < NB; ++b) {
+        if (b >= P.num_bodies) break;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const int s = b * KM + k;
+          col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                              ov[6]) < r2[s];
         }
       }
     }
   }
 
-  // robot spheres x voxel spheres (executeCollisionFreeVoxelsKernel)
-  for (int v = 0; v < V.num; ++v) {
-    if (wave_done<ANY>(col)) break;
-    const float vx = V.vox[3 * v], vy = V.vox[3 * v + 1], vz = V.vox[3 * v + 2];
+  // robot spheres x voxel spheres (executeCollisionFreeVoxelsKernel); the
+  // per-slot contact distances are formed once, -1 for slots not checked
+  if (V.num > 0) {
+    float vr2[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const bool on = s < 64 ? ((V.mask0 >> s) & 1) : ((V.mask1 >> (s - 64)) & 1);
-      if (on) {
-        const float rs = V.radius + P.slot_sphere[s].w;
-        col |= hit_sphere_sphere(vx, vy, vz, cx[s], cy[s], cz[s], rs * rs);
-      }
+      const float rs = V.radius + P.slot_sphere[s].w;
+      vr2[s] = on ? rs * rs : -1.f;
+    }
+    for (int v = 0; v < V.num; ++v) {
+      if (wave_done<ANY>(col)) break;
+      const float vx = V.vox[3 * v], vy = V.vox[3 * v + 1], vz = V.vox[3 * v + 2];
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        col |= hit_sphere_sphere(vx, vy, vz, cx[s], cy[s], cz[s], vr2[s]);
     }
   }
   return col;

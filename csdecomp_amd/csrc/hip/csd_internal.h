@@ -49,7 +49,7 @@ struct DevGeom {
 // Static obstacle record of the fast kernel (16 floats).
 //  box:     [0..8] R^T row-major, [9..11] -R^T p, [12..14] half extents
 //  sphere:  [0..2] centre, [3] radius
-//  capsule: [0..2] p0, [3..5] p1, [6] radius
+//  capsule: [0..2] p0, [3..5] d = p1 - p0, [6] 1/|d|^2, [7] radius
 struct DevObstacle {
   float v[16];
 };
@@ -85,7 +85,10 @@ struct PlantImage {
   const float4 *slot_sphere;   // [nslots] local centre + radius (r<0: empty)
   const DevObstacle *obstacles;
   const int32_t *obstacle_type;   // [num_obstacles]
-  const uint64_t *obstacle_mask;  // [num_obstacles][2] slot masks
+  // squared contact distance of (obstacle, sphere slot): box r^2, sphere
+  // and capsule (r + r_o)^2, all rounded in FP32 as the device would;
+  // -1 where the slot is not paired with the obstacle (never a hit)
+  const float *obstacle_r2;       // [num_obstacles][nslots]
   // self collision of the fast path as body-pair blocks: bit bb of
   // self_active[ba] says bodies ba < bb have sphere pairs; self_r2 holds
   // (rA+rB)^2 per (ba, bb, ka, kb), 0 for absent pairs (never a hit)

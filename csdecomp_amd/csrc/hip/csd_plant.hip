@@ -411,7 +411,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   std::vector<int32_t> geom_slot(std::max(G, 1), -1);
   std::vector<DevObstacle> obstacles;
   std::vector<int32_t> obstacle_type;
-  std::vector<uint64_t> obstacle_mask;
+  std::vector<std::pair<int, int>> obstacle_slot;  // (obstacle, slot) pairs
   const int nbt_alloc = fast ? nbt : 1, km_alloc = fast ? kmax : 1;
   std::vector<uint64_t> self_active(nbt_alloc, 0);
   std::vector<float> self_r2((size_t)nbt_alloc * nbt_alloc * km_alloc * km_alloc, 0.f);
@@ -480,12 +480,24 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
         }
         obstacles.push_back(rec);
         obstacle_type.push_back(t);
-        obstacle_mask.push_back(0);
-        obstacle_mask.push_back(0);
       }
-      int o = obs_of_geom[og], s = geom_slot[sg];
-      obstacle_mask[2 * o + (s >> 6)] |= (uint64_t)1 << (s & 63);
+      obstacle_slot.emplace_back(obs_of_geom[og], geom_slot[sg]);
     }
+  }
+  // contact distances per (obstacle, slot), computed in FP32 exactly as the
+  // device expressions r * r and (r + r_o) * (r + r_o)
+  std::vector<float> obstacle_r2(std::max<size_t>(obstacles.size() * nslots, 1), -1.f);
+  for (const auto &os : obstacle_slot) {
+    const int o = os.first, sl = os.second;
+    const float r = slot_sphere[sl].w;
+    float r2;
+    if (obstacle_type[o] == CSD_BOX) {
+      r2 = r * r;
+    } else {
+      const float rs = r + obstacles[o].v[obstacle_type[o] == CSD_SPHERE ? 3 : 7];
+      r2 = rs * rs;
+    }
+    obstacle_r2[(size_t)o * nslots + sl] = r2;
   }
 
   // ---- links for standalone FK ----
@@ -506,7 +518,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   size_t o_slots = bld.add(slot_sphere.data(), slot_sphere.size());
   size_t o_obs = bld.add(obstacles.data(), obstacles.size());
   size_t o_otype = bld.add(obstacle_type.data(), obstacle_type.size());
-  size_t o_omask = bld.add(obstacle_mask.data(), obstacle_mask.size());
+  size_t o_or2 = bld.add(obstacle_r2.data(), obstacle_r2.size());
   size_t o_sact = bld.add(self_active.data(), self_active.size());
   size_t o_sr2 = bld.add(self_r2.data(), self_r2.size());
   size_t o_gslot = bld.add(geom_slot.data(), geom_slot.size());
@@ -550,7 +562,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   img.slot_sphere = (const float4 *)(base + o_slots);
   img.obstacles = (const DevObstacle *)(base + o_obs);
   img.obstacle_type = (const int32_t *)(base + o_otype);
-  img.obstacle_mask = (const uint64_t *)(base + o_omask);
+  img.obstacle_r2 = (const float *)(base + o_or2);
   img.self_active = (const uint64_t *)(base + o_sact);
   img.self_r2 = (const float *)(base + o_sr2);
   img.geom_slot = (const int32_t *)(base + o_gslot);
