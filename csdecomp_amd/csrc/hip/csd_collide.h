@@ -22,6 +22,14 @@ struct VoxelArgs {
   uint64_t mask0, mask1;  // fast path: robot sphere slots
 };
 
+// Keeps a wave-uniform value in a vector register: the slot radii are read
+// by every pair test, and held in scalar registers they would spill.
+__device__ __forceinline__ float in_vgpr(float x) {
+  float v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+  return v;
+}
+
 // Wave-level early exit. ANY = false: every lane of the wave has found a
 // collision (independent configurations). ANY = true: some lane has (the
 // wave evaluates points of one edge and only "any collision" matters; the
@@ -39,9 +47,14 @@ template <int NB, int KM, bool ANY = false, class Q>
 __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
                                               const VoxelArgs &V, bool col) {
   constexpr int NS = NB * KM;
+  // Up to 32 sphere slots, contact distances come from per-pair tables held
+  // in scalar registers (no extra vector work per pair; the tables fit the
+  // scalar cache). Above that they are formed from the slot radii and pair
+  // masks, which keeps the per-obstacle data small.
+  constexpr bool kTab = NS <= 32;
   // Chain FK fused with the sphere centres: only the current body frame is
   // live, and every register index below is a compile-time constant.
-  float cx[NS], cy[NS], cz[NS];
+  float cx[NS], cy[NS], cz[NS], rad[NS];  // sphere centres (world) and radii
   {
     float X[12];
     aff_identity(X);
@@ -50,7 +63,9 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
       if (b < P.num_bodies) {
         const DevBody &bd = P.bodies[b];
         float M[12];
-        joint_local(bd, qa(bd.q_idx), M);
+        // with q_identity the joint value has a compile-time index (a plain
+        // register read for configurations held in registers)
+        joint_local(bd, P.q_identity ? qa(b) : qa(bd.q_idx), M);
         if (bd.parent < 0) {
 #pragma unroll
           for (int k = 0; k < 12; ++k) X[k] = M[k];
@@ -64,78 +79,166 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
         for (int k = 0; k < KM; ++k) {
           const float4 sp = P.slot_sphere[b * KM + k];
           aff_point(X, sp.x, sp.y, sp.z, cx[b * KM + k], cy[b * KM + k], cz[b * KM + k]);
+          if constexpr (!kTab) rad[b * KM + k] = in_vgpr(sp.w);
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < KM; ++k) cx[b * KM + k] = cy[b * KM + k] = cz[b * KM + k] = 0.f;
+        for (int k = 0; k < KM; ++k)
+          cx[b * KM + k] = cy[b * KM + k] = cz[b * KM + k] = rad[b * KM + k] = 0.f;
       }
     }
   }
   col = col || P.always_colliding;
 
-  // moving sphere x moving sphere (self collision) as body-pair blocks;
-  // absent sphere pairs carry r2 = 0 and can never report a hit
+  // moving sphere x moving sphere (self collision) as body-pair blocks. With
+  // tables, self_r2[ba][bb][ka][kb] holds (rA + rB)^2 (0 for absent pairs,
+  // never a hit); otherwise bit ka*KM+kb of self_bits[ba][bb] marks a pair,
+  // (rA + rB)^2 is formed in FP32 and absent pairs get -1.
+  // (Loops over bodies keep a constant trip count with uniform guards
+  // instead of `break`, so they unroll fully for 16 bodies too and every
+  // register index stays static.)
 #pragma unroll
   for (int ba = 0; ba < NB; ++ba) {
     const uint64_t act = P.self_active[ba];
-    if (act == 0) continue;
-    if (wave_done<ANY>(col)) break;
+    if (act == 0 || wave_done<ANY>(col)) continue;
 #pragma unroll
     for (int bb = ba + 1; bb < NB; ++bb) {
       if ((act >> bb) & 1) {
-        const float *r2 = P.self_r2 + ((size_t)ba * NB + bb) * (KM * KM);
+        if constexpr (kTab) {
+          const float *r2 = P.self_r2 + ((size_t)ba * NB + bb) * (KM * KM);
 #pragma unroll
-        for (int ka = 0; ka < KM; ++ka)
+          for (int ka = 0; ka < KM; ++ka)
 #pragma unroll
-          for (int kb = 0; kb < KM; ++kb)
-            col |= hit_sphere_sphere(cx[ba * KM + ka], cy[ba * KM + ka], cz[ba * KM + ka],
-                                     cx[bb * KM + kb], cy[bb * KM + kb], cz[bb * KM + kb],
-                                     r2[ka * KM + kb]);
+            for (int kb = 0; kb < KM; ++kb) {
+              const int sa = ba * KM + ka, sb = bb * KM + kb;
+              col |= hit_sphere_sphere(cx[sa], cy[sa], cz[sa], cx[sb], cy[sb], cz[sb],
+                                       r2[ka * KM + kb]);
+            }
+        } else {
+          const uint64_t bits = P.self_bits[ba * NB + bb];
+#pragma unroll
+          for (int ka = 0; ka < KM; ++ka)
+#pragma unroll
+            for (int kb = 0; kb < KM; ++kb) {
+              const int sa = ba * KM + ka, sb = bb * KM + kb;
+              const float rs = rad[sa] + rad[sb];
+              const float r2 = ((bits >> (ka * KM + kb)) & 1) ? rs * rs : -1.f;
+              col |= hit_sphere_sphere(cx[sa], cy[sa], cz[sa], cx[sb], cy[sb], cz[sb], r2);
+            }
+        }
       }
     }
   }
 
-  // moving sphere x world-fixed obstacle. The record and its row of contact
-  // distances are read once per obstacle (wave-uniform scalar loads); slots
-  // not paired with the obstacle carry r2 = -1 and never report a hit, so the
-  // slot loop has no per-slot branch. Slots of absent bodies are skipped.
-  for (int o = 0; o < P.num_obstacles; ++o) {
-    if (wave_done<ANY>(col)) break;
-    const int t = P.obstacle_type[o];
-    float ov[16], r2[NS];
+  // moving sphere x world-fixed obstacle, tables: the record and its row of
+  // contact distances obstacle_r2[o][s] (box r^2, sphere/capsule
+  // (r + r_o)^2, -1 if the slot is not paired with o) are read once per
+  // obstacle with a few wide scalar loads; the slot loop has no branch.
+  if constexpr (kTab) {
+    for (int o = 0; o < P.num_obstacles; ++o) {
+      if (wave_done<ANY>(col)) break;
+      const int t = P.obstacle_type[o];
+      float ov[16], r2[NS];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+      for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) r2[s] = P.obstacle_r2[(size_t)o * NS + s];
-    if (t == CSD_BOX) {
+      for (int s = 0; s < NS; ++s) r2[s] = P.obstacle_r2[(size_t)o * NS + s];
+      if (t == CSD_BOX) {
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (b >= P.num_bodies) break;
+        for (int b = 0; b < NB; ++b) {
+          if (b >= P.num_bodies) continue;
 #pragma unroll
-        for (int k = 0; k < KM; ++k) {
-          const int s = b * KM + k;
-          col |= hit_sphere_obox(cx[s], cy[s], cz[s], r2[s], ov);
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= hit_sphere_obox(cx[s], cy[s], cz[s], r2[s], ov);
+          }
+        }
+      } else if (t == CSD_SPHERE) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (b >= P.num_bodies) continue;
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], r2[s]);
+          }
+        }
+      } else {  // capsule
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (b >= P.num_bodies) continue;
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                                ov[6]) < r2[s];
+          }
         }
       }
-    } else if (t == CSD_SPHERE) {
+    }
+  }
+
+  // moving sphere x world-fixed obstacle, masks. Per obstacle only its 64-byte record and two mask words are read (scalar
+  // loads, issued one obstacle ahead so their latency hides behind the
+  // current obstacle's tests). Contact distances are formed in FP32 as
+  // r * r (box) and (r + r_o) * (r + r_o) (sphere, capsule); a slot not
+  // paired with the obstacle gets -1, which never reports a hit, so there is
+  // no per-slot branch. Slots of absent bodies are skipped per body.
+  if (!kTab && P.num_obstacles > 0) {
+    int tn = P.obstacle_type[0];
+    float ovn[16];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (b >= P.num_bodies) break;
+    for (int k = 0; k < 16; ++k) ovn[k] = P.obstacles[0].v[k];
+    uint64_t m0n = P.obstacle_mask[0], m1n = P.obstacle_mask[1];
+    for (int o = 0; o < P.num_obstacles; ++o) {
+      if (wave_done<ANY>(col)) break;
+      const int t = tn;
+      float ov[16];
 #pragma unroll
-        for (int k = 0; k < KM; ++k) {
-          const int s = b * KM + k;
-          col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], r2[s]);
+      for (int k = 0; k < 16; ++k) ov[k] = ovn[k];
+      const uint64_t m0 = m0n, m1 = m1n;
+      const int onext = o + 1 < P.num_obstacles ? o + 1 : o;
+      tn = P.obstacle_type[onext];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) ovn[k] = P.obstacles[onext].v[k];
+      m0n = P.obstacle_mask[2 * onext];
+      m1n = P.obstacle_mask[2 * onext + 1];
+      auto paired = [&](int s) { return s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1); };
+      if (t == CSD_BOX) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (b >= P.num_bodies) continue;
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            const float r2 = paired(s) ? rad[s] * rad[s] : -1.f;
+            col |= hit_sphere_obox(cx[s], cy[s], cz[s], r2, ov);
+          }
         }
-      }
-    } else {  // capsule
+      } else if (t == CSD_SPHERE) {
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (b >= P.num_bodies) break;
+        for (int b = 0; b < NB; ++b) {
+          if (b >= P.num_bodies) continue;
 #pragma unroll
-        for (int k = 0; k < KM; ++k) {
-          const int s = b * KM + k;
-          col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
-                              ov[6]) < r2[s];
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            const float rs = rad[s] + ov[3];
+            const float r2 = paired(s) ? rs * rs : -1.f;
+            col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], r2);
+          }
+        }
+      } else {  // capsule
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (b >= P.num_bodies) continue;
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            const float rs = rad[s] + ov[7];
+            const float r2 = paired(s) ? rs * rs : -1.f;
+            col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                                ov[6]) < r2;
+          }
         }
       }
     }
@@ -251,6 +354,38 @@ __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &q
         break;                                                                            \
       default:                                                                            \
         hipLaunchKernelGGL((KERNEL<::csd::kGeneric16>), grid, block, shmem, stream,      \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+    }                                                                                     \
+  } while (0)
+
+// host: launch KERNEL<VAR, X> for the runtime variant (X: a compile-time
+// integer of the caller, e.g. a padded dimension)
+#define CSD_DISPATCH_VARIANT_X(var, KERNEL, X, grid, block, shmem, stream, ...)             \
+  do {                                                                                    \
+    switch (var) {                                                                        \
+      case ::csd::kFast8x4:                                                               \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast8x4, X>), grid, block, shmem, stream,     \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kFast8x8:                                                               \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast8x8, X>), grid, block, shmem, stream,     \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kFast16x4:                                                              \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast16x4, X>), grid, block, shmem, stream,    \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kFast16x8:                                                              \
+        hipLaunchKernelGGL((KERNEL<::csd::kFast16x8, X>), grid, block, shmem, stream,    \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      case ::csd::kGeneric8:                                                              \
+        hipLaunchKernelGGL((KERNEL<::csd::kGeneric8, X>), grid, block, shmem, stream,    \
+                           __VA_ARGS__);                                                  \
+        break;                                                                            \
+      default:                                                                            \
+        hipLaunchKernelGGL((KERNEL<::csd::kGeneric16, X>), grid, block, shmem, stream,   \
                            __VA_ARGS__);                                                  \
         break;                                                                            \
     }                                                                                     \

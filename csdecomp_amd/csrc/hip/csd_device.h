@@ -104,10 +104,18 @@ struct QMem {
   int64_t base, sj;
   __device__ __forceinline__ float operator()(int j) const { return q[base + (int64_t)j * sj]; }
 };
+// Configuration held in registers, read with a wave-uniform index j: as a
+// vector value the read lowers to indexed register addressing
+// (s_set_gpr_idx_on + v_mov), never to a private-memory copy.
 template <int N>
 struct QReg {
-  const float *v;  // register array of N floats, indexed with a wave-uniform j
-  __device__ __forceinline__ float operator()(int j) const { return pick<N>(v, j); }
+  typedef float vec __attribute__((ext_vector_type(N)));
+  vec v;
+  __device__ __forceinline__ explicit QReg(const float (&a)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = a[k];
+  }
+  __device__ __forceinline__ float operator()(int j) const { return v[j]; }
 };
 
 // Forward kinematics of all bodies of one configuration.

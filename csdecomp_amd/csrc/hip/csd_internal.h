@@ -75,7 +75,7 @@ struct PlantImage {
   int32_t nslots;        // num_bodies * kmax
   int32_t num_obstacles; // static obstacles paired with moving spheres
   int32_t num_self_pairs; // moving sphere x moving sphere pairs
-  int32_t pad;
+  int32_t q_identity;     // 1 if body b is driven by configuration variable b
   // device arrays
   const DevBody *bodies;
   const DevGeom *geoms;
@@ -85,15 +85,19 @@ struct PlantImage {
   const float4 *slot_sphere;   // [nslots] local centre + radius (r<0: empty)
   const DevObstacle *obstacles;
   const int32_t *obstacle_type;   // [num_obstacles]
-  // squared contact distance of (obstacle, sphere slot): box r^2, sphere
-  // and capsule (r + r_o)^2, all rounded in FP32 as the device would;
-  // -1 where the slot is not paired with the obstacle (never a hit)
-  const float *obstacle_r2;       // [num_obstacles][nslots]
+  // bit s of obstacle_mask[2o + s/64]: sphere slot s is paired with
+  // obstacle o (all 64-byte records and masks together fit the scalar cache)
+  const uint64_t *obstacle_mask;  // [num_obstacles][2]
+  // plants of at most 32 sphere slots also get contact-distance tables:
+  // obstacle_r2[o][s] = r^2 (box) or (r + r_o)^2 (sphere, capsule) in FP32 as
+  // the device computes it, -1 if slot s is not paired with obstacle o
+  const float *obstacle_r2;       // [num_obstacles][nslots] (nslots <= 32)
   // self collision of the fast path as body-pair blocks: bit bb of
-  // self_active[ba] says bodies ba < bb have sphere pairs; self_r2 holds
-  // (rA+rB)^2 per (ba, bb, ka, kb), 0 for absent pairs (never a hit)
+  // self_active[ba] says bodies ba < bb have sphere pairs; bit ka*kmax+kb
+  // of self_bits[ba][bb] says slots (ba, ka) and (bb, kb) form a pair
   const uint64_t *self_active;    // [nbt]
-  const float *self_r2;           // [nbt][nbt][kmax][kmax]
+  const uint64_t *self_bits;      // [nbt][nbt]
+  const float *self_r2;           // [nbt][nbt][kmax][kmax] (nslots <= 32), 0 if absent
   const int32_t *geom_slot;       // [G] slot of moving sphere geometry, -1 else
 };
 

@@ -412,9 +412,11 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   std::vector<DevObstacle> obstacles;
   std::vector<int32_t> obstacle_type;
   std::vector<std::pair<int, int>> obstacle_slot;  // (obstacle, slot) pairs
-  const int nbt_alloc = fast ? nbt : 1, km_alloc = fast ? kmax : 1;
+  const int nbt_alloc = fast ? nbt : 1;
   std::vector<uint64_t> self_active(nbt_alloc, 0);
-  std::vector<float> self_r2((size_t)nbt_alloc * nbt_alloc * km_alloc * km_alloc, 0.f);
+  std::vector<uint64_t> self_bits((size_t)nbt_alloc * nbt_alloc, 0);
+  const bool tables = fast && nslots <= 32;
+  std::vector<float> self_r2(tables ? (size_t)nbt * nbt * kmax * kmax : 1, 0.f);
   int num_self_pairs = 0;
   if (fast) {
     std::vector<int> used(NB, 0);
@@ -433,9 +435,12 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
         int sa = geom_slot[a], sb = geom_slot[b];
         if (sa > sb) std::swap(sa, sb);
         const int ba = sa / kmax, bb = sb / kmax, ka = sa % kmax, kb = sb % kmax;
-        const float rs = geoms[a].dims[0] + geoms[b].dims[0];
         self_active[ba] |= (uint64_t)1 << bb;
-        self_r2[(((size_t)ba * nbt + bb) * kmax + ka) * kmax + kb] = rs * rs;
+        self_bits[(size_t)ba * nbt + bb] |= (uint64_t)1 << (ka * kmax + kb);
+        if (tables) {
+          const float rs = geoms[a].dims[0] + geoms[b].dims[0];
+          self_r2[(((size_t)ba * nbt + bb) * kmax + ka) * kmax + kb] = rs * rs;
+        }
         ++num_self_pairs;
         continue;
       }
@@ -484,21 +489,25 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
       obstacle_slot.emplace_back(obs_of_geom[og], geom_slot[sg]);
     }
   }
-  // contact distances per (obstacle, slot), computed in FP32 exactly as the
-  // device expressions r * r and (r + r_o) * (r + r_o)
-  std::vector<float> obstacle_r2(std::max<size_t>(obstacles.size() * nslots, 1), -1.f);
-  for (const auto &os : obstacle_slot) {
-    const int o = os.first, sl = os.second;
-    const float r = slot_sphere[sl].w;
-    float r2;
-    if (obstacle_type[o] == CSD_BOX) {
-      r2 = r * r;
-    } else {
-      const float rs = r + obstacles[o].v[obstacle_type[o] == CSD_SPHERE ? 3 : 7];
-      r2 = rs * rs;
+  std::vector<uint64_t> obstacle_mask(std::max<size_t>(2 * obstacles.size(), 2), 0);
+  for (const auto &os : obstacle_slot)
+    obstacle_mask[2 * os.first + (os.second >> 6)] |= (uint64_t)1 << (os.second & 63);
+  // contact distances in FP32, as the device expressions r * r and
+  // (r + r_o) * (r + r_o) round them
+  std::vector<float> obstacle_r2(tables ? std::max<size_t>(obstacles.size() * nslots, 1) : 1, -1.f);
+  if (tables)
+    for (const auto &os : obstacle_slot) {
+      const int o = os.first, sl = os.second;
+      const float r = slot_sphere[sl].w;
+      float r2;
+      if (obstacle_type[o] == CSD_BOX) {
+        r2 = r * r;
+      } else {
+        const float rs = r + obstacles[o].v[obstacle_type[o] == CSD_SPHERE ? 3 : 7];
+        r2 = rs * rs;
+      }
+      obstacle_r2[(size_t)o * nslots + sl] = r2;
     }
-    obstacle_r2[(size_t)o * nslots + sl] = r2;
-  }
 
   // ---- links for standalone FK ----
   std::vector<DevLink> links(L);
@@ -518,9 +527,11 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   size_t o_slots = bld.add(slot_sphere.data(), slot_sphere.size());
   size_t o_obs = bld.add(obstacles.data(), obstacles.size());
   size_t o_otype = bld.add(obstacle_type.data(), obstacle_type.size());
-  size_t o_or2 = bld.add(obstacle_r2.data(), obstacle_r2.size());
+  size_t o_omask = bld.add(obstacle_mask.data(), obstacle_mask.size());
   size_t o_sact = bld.add(self_active.data(), self_active.size());
+  size_t o_sbits = bld.add(self_bits.data(), self_bits.size());
   size_t o_sr2 = bld.add(self_r2.data(), self_r2.size());
+  size_t o_or2 = bld.add(obstacle_r2.data(), obstacle_r2.size());
   size_t o_gslot = bld.add(geom_slot.data(), geom_slot.size());
 
   csd_plant *pl = new csd_plant();
@@ -554,6 +565,9 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   img.nslots = nslots;
   img.num_obstacles = (int)obstacle_type.size();
   img.num_self_pairs = num_self_pairs;
+  img.q_identity = 1;
+  for (int bi = 0; bi < NB; ++bi)
+    if (bodies[bi].q_idx != bi) img.q_identity = 0;
   img.bodies = (const DevBody *)(base + o_bodies);
   img.geoms = (const DevGeom *)(base + o_geoms);
   img.pairs = (const int32_t *)(base + o_pairs);
@@ -562,9 +576,11 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   img.slot_sphere = (const float4 *)(base + o_slots);
   img.obstacles = (const DevObstacle *)(base + o_obs);
   img.obstacle_type = (const int32_t *)(base + o_otype);
-  img.obstacle_r2 = (const float *)(base + o_or2);
+  img.obstacle_mask = (const uint64_t *)(base + o_omask);
   img.self_active = (const uint64_t *)(base + o_sact);
+  img.self_bits = (const uint64_t *)(base + o_sbits);
   img.self_r2 = (const float *)(base + o_sr2);
+  img.obstacle_r2 = (const float *)(base + o_or2);
   img.geom_slot = (const int32_t *)(base + o_gslot);
   pl->geom_type.assign(d->geom_type, d->geom_type + G);
   pl->geom_link.assign(d->geom_link, d->geom_link + G);

@@ -420,14 +420,16 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def(py::init<>())
       .def_readonly("iterations", &EizoInfo::iterations)
       .def_readonly("converged", &EizoInfo::converged)
+      .def_readonly("face_capacity_reached", &EizoInfo::face_capacity_reached)
       .def_readonly("samples_drawn", &EizoInfo::samples_drawn)
       .def_readonly("faces_added", &EizoInfo::faces_added);
 
   py::class_<CudaEdgeInflator>(m, "CudaEdgeInflator")
       .def(py::init<const MinimalPlant &, const std::vector<GeometryIndex> &, const EizoOptions &,
-                    const HPolyhedron &, int, int>(),
+                    const HPolyhedron &, int, int, int>(),
            py::arg("plant"), py::arg("robot_geometry_ids"), py::arg("options"), py::arg("domain"),
-           py::arg("max_regions") = 1, py::arg("max_voxels") = 10000)
+           py::arg("max_regions") = 1, py::arg("max_voxels") = 10000,
+           py::arg("face_capacity") = 0)
       .def(
           "inflateEdge",
           [](CudaEdgeInflator &self, const py::array &ls, const py::array &le,

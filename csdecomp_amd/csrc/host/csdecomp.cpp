@@ -908,7 +908,7 @@ csd_eizo_options to_c(const EizoOptions &o) {
 CudaEdgeInflator::CudaEdgeInflator(const MinimalPlant &plant,
                                    const std::vector<GeometryIndex> &robot_geometry_ids,
                                    const EizoOptions &options, const HPolyhedron &domain,
-                                   int max_regions, int max_voxels)
+                                   int max_regions, int max_voxels, int face_capacity)
     : plant_(plant),
       robot_ids_(robot_geometry_ids),
       options_(options),
@@ -921,11 +921,17 @@ CudaEdgeInflator::CudaEdgeInflator(const MinimalPlant &plant,
   for (auto g : robot_geometry_ids)
     require(g >= 0 && g < plant.num_scene_geometries, "invalid robot geometry id");
   require(max_regions >= 1, "max_regions must be positive");
-  // face capacity: domain + every face the iterations could add, rounded up
-  int64_t cap = (int64_t)domain.A().rows +
-                (int64_t)options.max_iterations * options.max_hyperplanes_per_iteration;
-  cap = std::min<int64_t>(cap, 4096);
-  face_capacity_ = (int)((cap + 15) / 16 * 16);
+  // face capacity (domain faces included): as given, else the domain plus
+  // every face the iterations could add, rounded up, at most 4096. A region
+  // whose faces reach the capacity stops inflating (status "capacity").
+  if (face_capacity > 0) {
+    face_capacity_ = face_capacity;
+  } else {
+    int64_t cap = (int64_t)domain.A().rows +
+                  (int64_t)options.max_iterations * options.max_hyperplanes_per_iteration;
+    cap = std::min<int64_t>(cap, 4096);
+    face_capacity_ = (int)((cap + 15) / 16 * 16);
+  }
   require(domain.A().rows < face_capacity_, "domain uses the whole face capacity");
   csd_eizo_options o = to_c(options);
   check_status(csd_eizo_create(plant_.device(), robot_ids_.data(), (int)robot_ids_.size(), &o,
@@ -981,6 +987,7 @@ std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, cons
       EizoInfo &e = (*infos)[r];
       e.iterations = inf[r].iterations;
       e.converged = inf[r].converged != 0;
+      e.face_capacity_reached = inf[r].capacity_reached != 0;
       e.samples_drawn = inf[r].samples_drawn;
       e.faces_added = inf[r].faces_added;
     }
@@ -1001,6 +1008,12 @@ HPolyhedron CudaEdgeInflator::inflateEdge(const std::vector<float> &line_start,
   std::vector<EizoInfo> infos;
   std::vector<HPolyhedron> r = inflateEdges(ls, le, voxels, voxel_radius, {}, &infos);
   last_info_ = infos[0];
+  // the reference stops with this error when MinimalHPolyhedron's static
+  // face storage overflows (cuda_edge_inflation_zero_order.cu:299-304)
+  if (last_info_.face_capacity_reached)
+    throw std::runtime_error(
+        "Max face count exceeded: the region filled the inflator's face capacity (" +
+        std::to_string(face_capacity_) + " faces); create it with a larger face_capacity.");
   if (verbose)
     printf("EI-ZO: %d iterations, %s, %lld samples, %d faces added\n", last_info_.iterations,
            last_info_.converged ? "certified" : "iteration limit reached",

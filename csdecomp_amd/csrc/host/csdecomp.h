@@ -302,6 +302,7 @@ struct EizoInfo {
   // per-region summary of the batched driver
   int iterations = 0;
   bool converged = false;
+  bool face_capacity_reached = false;
   int64_t samples_drawn = 0;
   int faces_added = 0;
 };
@@ -312,7 +313,7 @@ class CudaEdgeInflator {
  public:
   CudaEdgeInflator(const MinimalPlant &plant, const std::vector<GeometryIndex> &robot_geometry_ids,
                    const EizoOptions &options, const HPolyhedron &domain,
-                   int max_regions = 1, int max_voxels = 10000);
+                   int max_regions = 1, int max_voxels = 10000, int face_capacity = 0);
   ~CudaEdgeInflator();
   CudaEdgeInflator(const CudaEdgeInflator &) = delete;
   CudaEdgeInflator &operator=(const CudaEdgeInflator &) = delete;

@@ -6,7 +6,8 @@ same plant API a user would call (KinematicTree + finalize_scene + Plant).
   included; radius U[0.03, 0.08] m, centres along the link), plus world-fixed
   oriented boxes (half extents U[0.05, 0.3] m, centres U[-1, 1]^3 m outside
   a 0.2 m ball, uniform random rotation) and spheres (radius U[0.05, 0.2] m).
-* ``dual_arm_scene``: two such arms with bases at (+-0.3, 0, 0).
+* ``dual_arm_scene``: two such arms with bases at (+-0.3, 0, 0) and a
+  sparser obstacle field (see its docstring).
 * ``boxes_in_corners``: the 2-DOF scene of the reference's EI-ZO tests
   (planning/tests/edge_inflation_zero_order_test.cpp:72-115).
 """
@@ -94,21 +95,27 @@ def _add_arm(tree, geoms, rng, prefix, base_xyz, parent=0):
         prev, offset = lk, (ln, 0.0, 0.0)
 
 
-def _add_obstacles(tree, geoms, rng, num_boxes, num_spheres):
+def _add_obstacles(tree, geoms, rng, num_boxes, num_spheres, half=(0.05, 0.3),
+                   radius=(0.05, 0.2), extent=1.0, keep_out=((0.0, 0.0, 0.0),), keep_r=0.2):
+    """World-fixed boxes (half extents U[half], uniform random rotation) and
+    spheres (radius U[radius]); centres uniform in [-extent, extent]^3 at
+    least keep_r away from every point of keep_out (the arm bases)."""
     obs = _link(tree, "obstacles")
     tree.add_joint(_joint("obstacle_weld", _c.JointType.FIXED, 0, obs))
+    ko = np.asarray(keep_out, np.float64)
+
+    def centre():
+        while True:
+            c = rng.uniform(-extent, extent, size=3)
+            if np.linalg.norm(ko - c, axis=1).min() >= keep_r:
+                return c
+
     for _ in range(num_boxes):
-        while True:
-            c = rng.uniform(-1, 1, size=3)
-            if np.linalg.norm(c) >= 0.2:
-                break
-        geoms.append(_box(obs, c, rng.uniform(0.05, 0.3, size=3), random_rotation(rng)))
+        c = centre()
+        geoms.append(_box(obs, c, rng.uniform(half[0], half[1], size=3), random_rotation(rng)))
     for _ in range(num_spheres):
-        while True:
-            c = rng.uniform(-1, 1, size=3)
-            if np.linalg.norm(c) >= 0.2:
-                break
-        geoms.append(_sphere(obs, c, rng.uniform(0.05, 0.2)))
+        c = centre()
+        geoms.append(_sphere(obs, c, rng.uniform(radius[0], radius[1])))
 
 
 def _finish(tree, geoms):
@@ -129,15 +136,24 @@ def arm_scene(seed, num_boxes=16, num_spheres=0):
     return _finish(tree, geoms)
 
 
-def dual_arm_scene(seed, num_boxes=72, num_spheres=24):
-    """Two 7-DOF arms with bases 0.6 m apart + obstacles; returns a Plant."""
+def dual_arm_scene(seed, num_boxes=72, num_spheres=24, half=(0.05, 0.2), radius=(0.05, 0.15),
+                   extent=1.5, keep_r=0.4):
+    """Two 7-DOF arms with bases 0.6 m apart + obstacles; returns a Plant.
+
+    With the single-arm obstacle distribution (half extents U[0.05, 0.3] m,
+    radius U[0.05, 0.2] m, centres in [-1, 1]^3), 96 obstacles leave no free
+    configuration for two arms (none in 100,000 uniform samples). The
+    defaults here (smaller obstacles over [-1.5, 1.5]^3, centres at least
+    0.4 m from either base) leave about 4% of configurations free, close to
+    the single-arm benchmark scene."""
     rng = np.random.default_rng(seed)
     tree = _c.KinematicTree()
     _link(tree, "world")
     geoms = []
     _add_arm(tree, geoms, rng, "a_", (0.3, 0, 0))
     _add_arm(tree, geoms, rng, "b_", (-0.3, 0, 0))
-    _add_obstacles(tree, geoms, rng, num_boxes, num_spheres)
+    _add_obstacles(tree, geoms, rng, num_boxes, num_spheres, half, radius, extent,
+                   ((0.3, 0.0, 0.0), (-0.3, 0.0, 0.0)), keep_r)
     return _finish(tree, geoms)
 
 
@@ -171,3 +187,54 @@ def uniform_configs(seed, n, lower, upper):
     rng = np.random.default_rng(seed)
     lo, hi = np.asarray(lower, np.float64), np.asarray(upper, np.float64)
     return np.asfortranarray(rng.uniform(lo[:, None], hi[:, None], size=(len(lo), n)).astype(np.float32))
+
+
+def free_segments(plant, count, seed, length=(0.2, 1.5), log_uniform=False, step=0.01,
+                  batch=1 << 18):
+    """`count` straight joint-space segments whose endpoints are collision
+    free and whose checkEdgeCollisionFree discretisation at `step` is free,
+    checked on the GPU. Starts are uniform in the joint limits; directions
+    are uniform on the sphere; lengths are U[length] (log-uniform when
+    `log_uniform`); ends are clipped to the limits. Returns two (dim, count)
+    float32 arrays (starts, ends)."""
+    import torch
+    from . import capi
+
+    mp = plant.getMinimalPlant()
+    lo = np.asarray(plant.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(plant.getPositionUpperLimits(), np.float32)
+    dim = len(lo)
+    rng = np.random.default_rng(seed)
+    dp = capi.DevicePlant(mp)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    starts, ends = [], []
+    while len(starts) < count:
+        q1 = rng.uniform(lo, hi, size=(batch, dim)).astype(np.float32)
+        d = rng.normal(size=(batch, dim))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        if log_uniform:
+            ln = np.exp(rng.uniform(np.log(length[0]), np.log(length[1]), size=(batch, 1)))
+        else:
+            ln = rng.uniform(length[0], length[1], size=(batch, 1))
+        q2 = np.clip(q1 + ln * d, lo, hi).astype(np.float32)
+        nodes = torch.from_numpy(np.concatenate([q1, q2])).to(dev)
+        free = torch.empty(2 * batch, dtype=torch.uint8, device=dev)
+        capi.check_collision_free(dp, nodes, 2 * batch, dim, 1, free)
+        ok = (free[:batch] & free[batch:]).nonzero().flatten().to(torch.int32)
+        if ok.numel() == 0:
+            continue
+        edges = torch.stack([ok, ok + batch], dim=1).contiguous()
+        valid = torch.empty(ok.numel(), dtype=torch.uint8, device=dev)
+        capi.validate_edges(dp, nodes, 2 * batch, edges, ok.numel(), step, 0, valid)
+        idx = ok[valid.bool()].cpu().numpy()
+        for i in idx[: count - len(starts)]:
+            starts.append(q1[i])
+            ends.append(q2[i])
+    return (np.asfortranarray(np.array(starts, np.float32).T),
+            np.asfortranarray(np.array(ends, np.float32).T))
+
+
+def log_uniform_budgets(seed, count, low=512, high=16384):
+    """Per-segment particle budgets, log-uniform integers in [low, high]."""
+    rng = np.random.default_rng(seed)
+    return [int(round(math.exp(x))) for x in rng.uniform(math.log(low), math.log(high), count)]

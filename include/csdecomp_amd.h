@@ -173,9 +173,11 @@ typedef struct csd_eizo_options {
  * aggregated per region). */
 typedef struct csd_eizo_region_info {
   int32_t iterations;
-  int32_t converged;
+  int32_t converged;        /* 1: the unadaptive test certified the region */
   int32_t num_faces;
   int32_t faces_added;
+  int32_t capacity_reached; /* 1: stopped because face_capacity was full */
+  int32_t pad;
   int64_t samples_drawn;
   int64_t collisions_found;
 } csd_eizo_region_info;
@@ -201,7 +203,9 @@ int csd_eizo_destroy(csd_eizo *ws);
  *  particle_budgets [num_regions] (NULL = options.num_particles each)
  *  voxels [num_voxels][3]
  * Outputs: out_A [num_regions][face_capacity][dim], out_b
- * [num_regions][face_capacity], infos [num_regions]. */
+ * [num_regions][face_capacity], infos [num_regions]. A region whose faces
+ * fill face_capacity stops there (infos[r].capacity_reached = 1); the other
+ * regions continue. */
 int csd_eizo_run(csd_eizo *ws, const float *domain_A, const float *domain_b,
                  int32_t domain_faces, const float *line_start,
                  const float *line_end, int32_t num_regions,
