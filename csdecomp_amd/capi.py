@@ -42,6 +42,15 @@ _SIGS = {
     "csd_occupancy": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _VP, _VP, _F, _VP, _VP]),
     "csd_knn": (C.c_int, [_VP, _I32, _I32, _I32, _VP, _VP, _VP]),
     "csd_stream_synchronize": (C.c_int, [_VP]),
+    "csd_mlp_create": (C.c_int, [_I32, _VP, _VP, _U64, C.POINTER(_VP)]),
+    "csd_mlp_destroy": (C.c_int, [_VP]),
+    "csd_mlp_num_params": (C.c_int, [_VP, C.POINTER(_I64)]),
+    "csd_mlp_get_params": (C.c_int, [_VP, _VP]),
+    "csd_mlp_set_params": (C.c_int, [_VP, _VP]),
+    "csd_mlp_predict": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _VP]),
+    "csd_mlp_gradients": (C.c_int, [_VP, _VP, _VP, _I32, _VP, _VP, _VP]),
+    "csd_mlp_train_step": (C.c_int, [_VP, _VP, _VP, _I32, _F, _VP, _VP]),
+    "csd_signed_distance": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _VP]),
 }
 
 _lib = None
@@ -216,3 +225,57 @@ def knn(nodes, num_nodes, dim, k, out_idx, out_d2=None, stream=None):
 
 def synchronize(stream=None):
     check(lib().csd_stream_synchronize(_stream(stream)), "csd_stream_synchronize")
+
+
+def signed_distance(plant, q, n, stride_n, stride_j, out, stream=None):
+    """Smallest FP32 signed distance over the collision pairs (labels)."""
+    check(lib().csd_signed_distance(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+                                    _stream(stream)), "csd_signed_distance")
+
+
+class Mlp:
+    """The signed-distance surrogate network (csd_mlp_* of the C-ABI)."""
+
+    def __init__(self, in_dim, lower, upper, seed=0):
+        lo = np.ascontiguousarray(lower, np.float32)
+        hi = np.ascontiguousarray(upper, np.float32)
+        h = _VP()
+        check(lib().csd_mlp_create(in_dim, _ptr(lo), _ptr(hi), seed, C.byref(h)), "csd_mlp_create")
+        self.handle = h
+        self.in_dim = in_dim
+        cnt = _I64()
+        check(lib().csd_mlp_num_params(self.handle, C.byref(cnt)), "csd_mlp_num_params")
+        self.num_params = int(cnt.value)
+
+    def get_params(self):
+        p = np.zeros(self.num_params, np.float32)
+        check(lib().csd_mlp_get_params(self.handle, _ptr(p)), "csd_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params
+        check(lib().csd_mlp_set_params(self.handle, _ptr(p)), "csd_mlp_set_params")
+
+    def predict(self, q, n, stride_n, stride_j, out, stream=None):
+        check(lib().csd_mlp_predict(self.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+                                    _stream(stream)), "csd_mlp_predict")
+
+    def gradients(self, q, target, batch, grads=None, loss=None, stream=None):
+        check(lib().csd_mlp_gradients(self.handle, _ptr(q), _ptr(target), batch,
+                                      _ptr(grads) if grads is not None else None,
+                                      _ptr(loss) if loss is not None else None,
+                                      _stream(stream)), "csd_mlp_gradients")
+
+    def train_step(self, q, target, batch, lr, loss=None, stream=None):
+        check(lib().csd_mlp_train_step(self.handle, _ptr(q), _ptr(target), batch, lr,
+                                       _ptr(loss) if loss is not None else None,
+                                       _stream(stream)), "csd_mlp_train_step")
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().csd_mlp_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

@@ -40,6 +40,46 @@ __device__ __forceinline__ bool wave_done(bool col) {
   else return __all(col);
 }
 
+// Chain FK of the fast path fused with the sphere centres: only the current
+// body frame is live, and every register index is a compile-time constant.
+// RAD: also return the slot radii (kept in vector registers).
+template <int NB, int KM, bool RAD, class Q>
+__device__ __forceinline__ void fast_spheres(const PlantImage &P, const Q &qa,
+                                             float (&cx)[NB * KM], float (&cy)[NB * KM],
+                                             float (&cz)[NB * KM], float (&rad)[NB * KM]) {
+  float X[12];
+  aff_identity(X);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b < P.num_bodies) {
+      const DevBody &bd = P.bodies[b];
+      float M[12];
+      // with q_identity the joint value has a compile-time index (a plain
+      // register read for configurations held in registers)
+      joint_local(bd, P.q_identity ? qa(b) : qa(bd.q_idx), M);
+      if (bd.parent < 0) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) X[k] = M[k];
+      } else {
+        float Y[12];
+        aff_mul(X, M, Y);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) X[k] = Y[k];
+      }
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        const float4 sp = P.slot_sphere[b * KM + k];
+        aff_point(X, sp.x, sp.y, sp.z, cx[b * KM + k], cy[b * KM + k], cz[b * KM + k]);
+        if constexpr (RAD) rad[b * KM + k] = in_vgpr(sp.w);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < KM; ++k)
+        cx[b * KM + k] = cy[b * KM + k] = cz[b * KM + k] = rad[b * KM + k] = 0.f;
+    }
+  }
+}
+
 // Fast path: every moving geometry is a sphere (PlantImage::fast).
 // `col` enters as the lane's initial state (true = nothing left to decide,
 // e.g. an idle tail lane) and is returned OR-ed with the result.
@@ -52,42 +92,8 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // scalar cache). Above that they are formed from the slot radii and pair
   // masks, which keeps the per-obstacle data small.
   constexpr bool kTab = NS <= 32;
-  // Chain FK fused with the sphere centres: only the current body frame is
-  // live, and every register index below is a compile-time constant.
   float cx[NS], cy[NS], cz[NS], rad[NS];  // sphere centres (world) and radii
-  {
-    float X[12];
-    aff_identity(X);
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      if (b < P.num_bodies) {
-        const DevBody &bd = P.bodies[b];
-        float M[12];
-        // with q_identity the joint value has a compile-time index (a plain
-        // register read for configurations held in registers)
-        joint_local(bd, P.q_identity ? qa(b) : qa(bd.q_idx), M);
-        if (bd.parent < 0) {
-#pragma unroll
-          for (int k = 0; k < 12; ++k) X[k] = M[k];
-        } else {
-          float Y[12];
-          aff_mul(X, M, Y);
-#pragma unroll
-          for (int k = 0; k < 12; ++k) X[k] = Y[k];
-        }
-#pragma unroll
-        for (int k = 0; k < KM; ++k) {
-          const float4 sp = P.slot_sphere[b * KM + k];
-          aff_point(X, sp.x, sp.y, sp.z, cx[b * KM + k], cy[b * KM + k], cz[b * KM + k]);
-          if constexpr (!kTab) rad[b * KM + k] = in_vgpr(sp.w);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < KM; ++k)
-          cx[b * KM + k] = cy[b * KM + k] = cz[b * KM + k] = rad[b * KM + k] = 0.f;
-      }
-    }
-  }
+  fast_spheres<NB, KM, !kTab>(P, qa, cx, cy, cz, rad);
   col = col || P.always_colliding;
 
   // moving sphere x moving sphere (self collision) as body-pair blocks. With
@@ -263,6 +269,75 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
     }
   }
   return col;
+}
+
+// Smallest signed distance over the plant's collision pairs in FP32 (fast
+// path plants): sphere-sphere |c_a - c_b| - r_a - r_b, sphere-box distance of
+// the centre to the box (negative inside) - r, sphere-capsule distance to the
+// segment - r_a - r_c. Voxels are not included. A plant with a colliding
+// world-fixed pair reports -inf. Used to label training data for the
+// signed-distance surrogate.
+template <int NB, int KM, class Q>
+__device__ __forceinline__ float min_signed_distance_fast(const PlantImage &P, const Q &qa) {
+  constexpr int NS = NB * KM;
+  float cx[NS], cy[NS], cz[NS], rad[NS];
+  fast_spheres<NB, KM, true>(P, qa, cx, cy, cz, rad);
+  float sd = P.always_colliding ? -INFINITY : INFINITY;
+#pragma unroll
+  for (int ba = 0; ba < NB; ++ba) {
+    const uint64_t act = P.self_active[ba];
+    if (act == 0) continue;
+#pragma unroll
+    for (int bb = ba + 1; bb < NB; ++bb) {
+      if ((act >> bb) & 1) {
+        const uint64_t bits = P.self_bits[ba * NB + bb];
+#pragma unroll
+        for (int ka = 0; ka < KM; ++ka)
+#pragma unroll
+          for (int kb = 0; kb < KM; ++kb) {
+            const int sa = ba * KM + ka, sb = bb * KM + kb;
+            const float dx = cx[sa] - cx[sb], dy = cy[sa] - cy[sb], dz = cz[sa] - cz[sb];
+            const float d = sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) - (rad[sa] + rad[sb]);
+            sd = ((bits >> (ka * KM + kb)) & 1) ? fminf(sd, d) : sd;
+          }
+      }
+    }
+  }
+  for (int o = 0; o < P.num_obstacles; ++o) {
+    const int t = P.obstacle_type[o];
+    float ov[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+    const uint64_t m0 = P.obstacle_mask[2 * o], m1 = P.obstacle_mask[2 * o + 1];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b >= P.num_bodies) continue;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        const int s = b * KM + k;
+        float d;
+        if (t == CSD_BOX) {
+          const float lx = fmaf(ov[0], cx[s], fmaf(ov[1], cy[s], fmaf(ov[2], cz[s], ov[9])));
+          const float ly = fmaf(ov[3], cx[s], fmaf(ov[4], cy[s], fmaf(ov[5], cz[s], ov[10])));
+          const float lz = fmaf(ov[6], cx[s], fmaf(ov[7], cy[s], fmaf(ov[8], cz[s], ov[11])));
+          const float qx = fabsf(lx) - ov[12], qy = fabsf(ly) - ov[13], qz = fabsf(lz) - ov[14];
+          const float ex = fmaxf(qx, 0.f), ey = fmaxf(qy, 0.f), ez = fmaxf(qz, 0.f);
+          d = sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez))) +
+              fminf(fmaxf(qx, fmaxf(qy, qz)), 0.f) - rad[s];
+        } else if (t == CSD_SPHERE) {
+          const float dx = cx[s] - ov[0], dy = cy[s] - ov[1], dz = cz[s] - ov[2];
+          d = sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) - (rad[s] + ov[3]);
+        } else {
+          d = sqrtf(seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                                 ov[6])) -
+              (rad[s] + ov[7]);
+        }
+        const bool on = s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1);
+        sd = on ? fminf(sd, d) : sd;
+      }
+    }
+  }
+  return sd;
 }
 
 // Generic path: any sphere / box / capsule pair, full 3x4 geometry poses.

@@ -268,6 +268,44 @@ int csd_occupancy(const csd_plant *plant, const float *q, int64_t n,
 int csd_knn(const float *nodes, int32_t num_nodes, int32_t dim, int32_t k,
             int32_t *out_idx, float *out_d2, void *stream);
 
+/* ---- signed-distance surrogate ----
+ * No counterpart in the reference (its checkers return booleans only). An MLP
+ *   x = 2 (q - lower) / (upper - lower) - 1   (in_dim <= 32, zero-padded)
+ *   a_l = bf16(SiLU(W_l a_{l-1} + b_l)), l = 1..4, width 256
+ *   y = w_5 . a_4 + b_5
+ * with BF16 weights/activations and FP32 accumulation on the MFMA units, FP32
+ * master weights and Adam (beta 0.9 / 0.999, eps 1e-8). Parameters are one
+ * FP32 vector: W1 [256][32], b1 [256], W2 [256][256], b2, W3, b3, W4, b4,
+ * w5 [256], b5. EI-ZO uses it to steer sampling; the exact checker still
+ * judges every sample. */
+typedef struct csd_mlp csd_mlp;
+int csd_mlp_create(int32_t in_dim, const float *lower, const float *upper,
+                   uint64_t seed, csd_mlp **out);  /* lower/upper: host [in_dim] */
+int csd_mlp_destroy(csd_mlp *model);
+int csd_mlp_num_params(const csd_mlp *model, int64_t *count);
+int csd_mlp_get_params(const csd_mlp *model, float *host_params);
+int csd_mlp_set_params(csd_mlp *model, const float *host_params);
+/* y for n configurations (device q addressed with the two strides). */
+int csd_mlp_predict(const csd_mlp *model, const float *q, int64_t n,
+                    int64_t stride_n, int64_t stride_j, float *out,
+                    void *stream);
+/* Mean squared error over one batch (batch a multiple of 128): q (device)
+ * [batch][in_dim], target (device) [batch]. _gradients writes dLoss/dparams
+ * (device, num_params floats, may be NULL) and the loss (device float, may be
+ * NULL) without updating; _train_step also applies one Adam step. */
+int csd_mlp_gradients(csd_mlp *model, const float *q, const float *target,
+                      int32_t batch, float *grads, float *loss, void *stream);
+int csd_mlp_train_step(csd_mlp *model, const float *q, const float *target,
+                       int32_t batch, float learning_rate, float *loss,
+                       void *stream);
+/* Labels: the smallest signed distance (metres, FP32, negative in
+ * penetration) over the plant's collision pairs, for plants whose moving
+ * collision geometry is all spheres (CSD_ERR_UNSUPPORTED otherwise);
+ * voxels are not included. out (device) [n]. */
+int csd_signed_distance(const csd_plant *plant, const float *q, int64_t n,
+                        int64_t stride_n, int64_t stride_j, float *out,
+                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif
