@@ -69,6 +69,96 @@ def cpu_baseline(mp, q, seconds):
                       f"CPU rule of collision_checker.cpp with early exit"}
 
 
+def train_surrogate(plant, n=2_000_000, batch=8192, epochs=3, seed=7):
+    """Surrogate for the plant: n exact FP32 signed-distance labels (clipped to
+    +-0.2 m, scaled to +-1) at uniform configurations, Adam lr 1e-3."""
+    import torch
+    from csdecomp_amd import capi
+    mp = plant.getMinimalPlant()
+    lo = np.asarray(plant.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(plant.getPositionUpperLimits(), np.float32)
+    dof = len(lo)
+    t0 = time.perf_counter()
+    q = torch.from_numpy(np.random.default_rng(seed).uniform(lo, hi, (n, dof)).astype(np.float32)).cuda()
+    sd = torch.empty(n, device="cuda")
+    capi.signed_distance(capi.DevicePlant(mp), q, n, dof, 1, sd)
+    target = (sd.clamp(-0.2, 0.2) / 0.2).contiguous()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    model = capi.Mlp(dof, lo, hi, 1)
+    loss = torch.empty(1, device="cuda")
+    steps = epochs * n // batch
+    for s in range(steps):
+        k = (s * batch) % (n - batch)
+        model.train_step(q[k:k + batch], target[k:k + batch], batch, 1e-3, loss)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return model, {"labels": n, "label_s": round(t1 - t0, 3), "train_steps": steps,
+                   "train_s": round(t2 - t1, 3), "final_mse": round(loss.item(), 5)}
+
+
+def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_budget=100):
+    """Regions/s of batched EI-ZO (epsilon 1e-3, delta 0.05, tau 0.5, 10
+    bisection steps, 40 mixing steps, <= 10 faces per round, <= face_budget
+    faces beyond the domain) in each mode."""
+    import torch
+    import csdecomp_amd as csd
+    mp = plant.getMinimalPlant()
+    dom = csd.HPolyhedron()
+    dom.MakeBox(plant.getPositionLowerLimits(), plant.getPositionUpperLimits())
+    nd = dom.A().shape[0]
+    opt = csd.EizoOptions()
+    opt.epsilon, opt.delta, opt.tau = 1e-3, 0.05, 0.5
+    opt.bisection_steps, opt.num_particles, opt.mixing_steps = 10, particles, 40
+    opt.max_hyperplanes_per_iteration = 10
+    opt.verbose = False
+    R = starts.shape[1]
+    out = {"segments": R, "dof": int(starts.shape[0])}
+    for mode in modes:
+        inf = csd.CudaEdgeInflator(mp, list(plant.getRobotGeometryIds()), opt, dom, R, 10, nd + face_budget)
+        if mode == "guided":
+            inf.set_guidance(pilot_samples=4096, pilot_factor=10.0, surrogate=model.address,
+                             keep_particles=64, line_points=4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        regions, infos = inf.inflateEdges(starts, ends, None, 0.0, budgets)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        conv = np.array([i.converged for i in infos])
+        out[mode] = {"seconds": round(el, 3), "regions_per_s": round(R / el, 2),
+                     "certified_per_s": round(float(conv.sum()) / el, 2),
+                     "certified_fraction": round(float(conv.mean()), 4),
+                     "rounds_mean": round(float(np.mean([i.iterations for i in infos])), 2),
+                     "faces_mean": round(float(np.mean([r.A().shape[0] - nd for r in regions])), 1),
+                     "samples": int(sum(i.samples_drawn for i in infos))}
+    if "plain" in out and "guided" in out:
+        out["guided_vs_plain_regions_per_s"] = round(out["guided"]["regions_per_s"] / out["plain"]["regions_per_s"], 3)
+        out["guided_vs_plain_certified_per_s"] = round(
+            out["guided"]["certified_per_s"] / max(out["plain"]["certified_per_s"], 1e-9), 3)
+    return out
+
+
+def eizo_benchmarks(segments=2048):
+    """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
+    segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
+    dual arm, 96 obstacles, 2048 segments, lengths log-U[0.05, 2] rad, budgets
+    log-U[512, 16384]); plain rounds vs surrogate-guided rounds."""
+    from csdecomp_amd import scenes
+    res = {}
+    arm = scenes.arm_scene(1001, 48, 16)
+    s1, e1 = scenes.free_segments(arm, 256, 3001, (0.2, 1.5))
+    m7, tr7 = train_surrogate(arm)
+    res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, ("plain", "guided"), m7)
+    res["single_edge_arm7"]["surrogate"] = tr7
+    dual = scenes.dual_arm_scene(1002, 72, 24)
+    s2, e2 = scenes.free_segments(dual, segments, 3002, (0.05, 2.0), log_uniform=True)
+    b2 = scenes.log_uniform_budgets(3003, segments)
+    m14, tr14 = train_surrogate(dual)
+    res["batched_dualarm14"] = eizo_leg("batched", dual, s2, e2, b2, 16384, ("plain", "guided"), m14)
+    res["batched_dualarm14"]["surrogate"] = tr14
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +167,8 @@ def main():
     ap.add_argument("--num-configs", type=int, default=65536)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-eizo", action="store_true", help="skip the EI-ZO regions/s legs")
+    ap.add_argument("--eizo-segments", type=int, default=2048)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -166,6 +258,8 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mp, q, args.cpu_seconds)
+    if not args.no_eizo:
+        line["eizo"] = eizo_benchmarks(args.eizo_segments)
     print(json.dumps(line), flush=True)
 
 

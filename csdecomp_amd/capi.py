@@ -247,6 +247,11 @@ class Mlp:
         check(lib().csd_mlp_num_params(self.handle, C.byref(cnt)), "csd_mlp_num_params")
         self.num_params = int(cnt.value)
 
+    @property
+    def address(self):
+        """The csd_mlp* as an integer (for CudaEdgeInflator.set_guidance)."""
+        return int(self.handle.value)
+
     def get_params(self):
         p = np.zeros(self.num_params, np.float32)
         check(lib().csd_mlp_get_params(self.handle, _ptr(p)), "csd_mlp_get_params")

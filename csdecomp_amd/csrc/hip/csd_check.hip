@@ -23,8 +23,10 @@ __global__ __launch_bounds__(kCheckBlock) void k_check(PlantImage P, const float
   const bool valid = gi < n;
   const int64_t i = valid ? gi : n - 1;  // idle tail lanes re-read the last config
   const QMem qa{q, i * sn, sj};
+  extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  const float *tab = stage_tables<VAR>(P, csd_smem);
   // idle lanes start "done" so they never hold the wave back
-  const bool col = collides_variant<VAR>(P, qa, V, R, !valid);
+  const bool col = collides_variant<VAR>(P, qa, V, R, !valid, tab);
   if (valid) out[gi] = col ? 0 : 1;
 }
 
@@ -154,8 +156,8 @@ int launch_check(const csd_plant *pl, const float *q, int64_t n, int64_t sn, int
   bool all_slots = fill_robot_list(pl, robot, nvox > 0 ? nr : 0, R, V, &rc);
   if (rc) return rc;
   const int var = check_variant(pl->img, nvox > 0 && !all_slots);
-  CSD_DISPATCH_VARIANT(var, k_check, grid_for(n), dim3(kCheckBlock), 0, st, pl->img, q, n, sn,
-                       sj, V, R, out);
+  CSD_DISPATCH_VARIANT(var, k_check, grid_for(n), dim3(kCheckBlock),
+                       check_lds_bytes(pl->img, var), st, pl->img, q, n, sn, sj, V, R, out);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

@@ -83,17 +83,22 @@ __device__ __forceinline__ void fast_spheres(const PlantImage &P, const Q &qa,
 // Fast path: every moving geometry is a sphere (PlantImage::fast).
 // `col` enters as the lane's initial state (true = nothing left to decide,
 // e.g. an idle tail lane) and is returned OR-ed with the result.
+// `tab`: for 64 sphere slots, the plant's contact-distance tables staged in
+// LDS by the kernel (stage_tables); unused otherwise.
 template <int NB, int KM, bool ANY = false, class Q>
 __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
-                                              const VoxelArgs &V, bool col) {
+                                              const VoxelArgs &V, bool col, const float *tab) {
   constexpr int NS = NB * KM;
-  // Up to 32 sphere slots, contact distances come from per-pair tables held
-  // in scalar registers (no extra vector work per pair; the tables fit the
-  // scalar cache). Above that they are formed from the slot radii and pair
-  // masks, which keeps the per-obstacle data small.
+  // Contact distances of the pair tests, by plant size:
+  //  * up to 32 sphere slots: per-pair tables read into scalar registers (no
+  //    extra vector work per pair; the tables fit the scalar cache);
+  //  * 64 slots: the same tables, staged in LDS once per workgroup;
+  //  * 128 slots: formed from the slot radii and pair masks.
   constexpr bool kTab = NS <= 32;
+  constexpr bool kLds = NS == 64;
+  constexpr bool kMask = !kTab && !kLds;
   float cx[NS], cy[NS], cz[NS], rad[NS];  // sphere centres (world) and radii
-  fast_spheres<NB, KM, !kTab>(P, qa, cx, cy, cz, rad);
+  fast_spheres<NB, KM, kMask>(P, qa, cx, cy, cz, rad);
   col = col || P.always_colliding;
 
   // moving sphere x moving sphere (self collision) as body-pair blocks. With
@@ -110,8 +115,8 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
 #pragma unroll
     for (int bb = ba + 1; bb < NB; ++bb) {
       if ((act >> bb) & 1) {
-        if constexpr (kTab) {
-          const float *r2 = P.self_r2 + ((size_t)ba * NB + bb) * (KM * KM);
+        if constexpr (kTab || kLds) {
+          const float *r2 = (kLds ? tab : P.self_r2) + ((size_t)ba * NB + bb) * (KM * KM);
 #pragma unroll
           for (int ka = 0; ka < KM; ++ka)
 #pragma unroll
@@ -184,13 +189,55 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
     }
   }
 
+  // moving sphere x world-fixed obstacle, LDS tables: the record in scalar
+  // registers, each body's row of contact distances read from LDS just before
+  // its tests (one 16-byte read per 4 slots).
+  if constexpr (kLds) {
+    const float *orow = tab + NB * NB * KM * KM;
+    for (int o = 0; o < P.num_obstacles; ++o) {
+      if (wave_done<ANY>(col)) break;
+      const int t = P.obstacle_type[o];
+      float ov[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+      const float *r2 = orow + (size_t)o * NS;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (b >= P.num_bodies) continue;
+        float rb[KM];
+#pragma unroll
+        for (int k = 0; k < KM; ++k) rb[k] = r2[b * KM + k];
+        if (t == CSD_BOX) {
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= hit_sphere_obox(cx[s], cy[s], cz[s], rb[k], ov);
+          }
+        } else if (t == CSD_SPHERE) {
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], rb[k]);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                                ov[6]) < rb[k];
+          }
+        }
+      }
+    }
+  }
+
   // moving sphere x world-fixed obstacle, masks. Per obstacle only its 64-byte record and two mask words are read (scalar
   // loads, issued one obstacle ahead so their latency hides behind the
   // current obstacle's tests). Contact distances are formed in FP32 as
   // r * r (box) and (r + r_o) * (r + r_o) (sphere, capsule); a slot not
   // paired with the obstacle gets -1, which never reports a hit, so there is
   // no per-slot branch. Slots of absent bodies are skipped per body.
-  if (!kTab && P.num_obstacles > 0) {
+  if (kMask && P.num_obstacles > 0) {
     int tn = P.obstacle_type[0];
     float ovn[16];
 #pragma unroll
@@ -380,13 +427,51 @@ __device__ __forceinline__ bool collides_generic(const PlantImage &P, const Q &q
 // needs. 0..3 fast <8,4>,<8,8>,<16,4>,<16,8>; 4,5 generic <8>,<16>.
 enum CheckVariant { kFast8x4 = 0, kFast8x8, kFast16x4, kFast16x8, kGeneric8, kGeneric16 };
 
+__host__ __device__ constexpr int variant_slots(int var) {
+  return var == kFast8x4 ? 32 : (var == kFast8x8 || var == kFast16x4) ? 64
+                                : var == kFast16x8 ? 128 : 0;
+}
+
+// LDS bytes of the contact-distance tables a 64-slot variant stages
+constexpr size_t kCheckLdsMax = 56 * 1024;
+inline size_t check_lds_bytes(const PlantImage &P, int var) {
+  if (variant_slots(var) != 64) return 0;
+  const size_t nbt = P.nslots / P.kmax;
+  return (nbt * nbt * P.kmax * P.kmax + (size_t)P.num_obstacles * 64) * sizeof(float);
+}
+
+// the fast shape of a fast-path plant
+inline int fast_variant(const PlantImage &P) {
+  const int nbt = P.nslots / P.kmax;
+  if (nbt == 8) return P.kmax == 4 ? kFast8x4 : kFast8x8;
+  return P.kmax == 4 ? kFast16x4 : kFast16x8;
+}
+
+// the checker a launch uses: the fast shape unless generic geometry is
+// needed or its staged tables would not fit in LDS
 inline int check_variant(const PlantImage &P, bool need_generic) {
   if (P.fast && !need_generic) {
-    const int nbt = P.nslots / P.kmax;
-    if (nbt == 8) return P.kmax == 4 ? kFast8x4 : kFast8x8;
-    return P.kmax == 4 ? kFast16x4 : kFast16x8;
+    const int v = fast_variant(P);
+    if (check_lds_bytes(P, v) <= kCheckLdsMax) return v;
   }
   return P.num_bodies <= 8 ? kGeneric8 : kGeneric16;
+}
+
+// Copies the contact-distance tables of a 64-slot variant into LDS (self
+// pairs, then obstacle rows) and returns them; every thread of the block
+// must call it. Other variants need nothing (nullptr).
+template <int VAR>
+__device__ __forceinline__ const float *stage_tables(const PlantImage &P, float *smem) {
+  if constexpr (variant_slots(VAR) == 64) {
+    constexpr int NB = VAR == kFast8x8 ? 8 : 16, KM = VAR == kFast8x8 ? 8 : 4;
+    constexpr int ns = NB * NB * KM * KM;
+    const int no = P.num_obstacles * 64;
+    for (int e = threadIdx.x; e < ns; e += blockDim.x) smem[e] = P.self_r2[e];
+    for (int e = threadIdx.x; e < no; e += blockDim.x) smem[ns + e] = P.obstacle_r2[e];
+    __syncthreads();
+    return smem;
+  }
+  return nullptr;
 }
 
 // dispatch a device-side call on the variant (VAR is a compile-time value
@@ -394,11 +479,11 @@ inline int check_variant(const PlantImage &P, bool need_generic) {
 template <int VAR, bool ANY = false, class Q>
 __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &qa,
                                                  const VoxelArgs &V, const RobotGeomList &R,
-                                                 bool col) {
-  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY>(P, qa, V, col);
-  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY>(P, qa, V, col);
-  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY>(P, qa, V, col);
-  else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY>(P, qa, V, col);
+                                                 bool col, const float *tab) {
+  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY>(P, qa, V, col, tab);
+  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY>(P, qa, V, col, tab);
+  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY>(P, qa, V, col, tab);
+  else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY>(P, qa, V, col, tab);
   else if constexpr (VAR == kGeneric8) return collides_generic<8, ANY>(P, qa, V, R, col);
   else return collides_generic<16, ANY>(P, qa, V, R, col);
 }

@@ -41,6 +41,8 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
   const VoxelArgs V{nullptr, 0, 0.f, 0, 0};
   RobotGeomList R;
   R.n = 0;
+  extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  const float *tab = stage_tables<VAR>(P, csd_smem);
   for (int64_t e = wave; e < E; e += nwaves) {
     const int a = edges[2 * e], b = edges[2 * e + 1];
     if (a < 0 || a >= num_nodes || b < 0 || b >= num_nodes) {
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
 #pragma unroll
       for (int d = 0; d < kEdgeDim; ++d)
         q[d] = __fadd_rn(__fmul_rn(t, q2[d]), __fmul_rn(__fsub_rn(1.f, t), q1[d]));
-      const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, false);
+      const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, false, tab);
       any = __any(col);
     }
     if (lane == 0) out[e] = any ? 0 : 1;
@@ -238,8 +240,9 @@ int csd_validate_edges(const csd_plant *plant, const float *nodes, int32_t num_n
   // persistent grid: enough waves to fill 256 CUs several times over
   const int64_t waves = std::min<int64_t>(num_edges, 256 * 64);
   dim3 g((unsigned)((waves * 64 + kEdgeBlock - 1) / kEdgeBlock));
-  CSD_DISPATCH_VARIANT(var, k_validate_edges, g, dim3(kEdgeBlock), 0, (hipStream_t)stream,
-                       plant->img, nodes, num_nodes, dim, edges, num_edges, step, rule, out);
+  CSD_DISPATCH_VARIANT(var, k_validate_edges, g, dim3(kEdgeBlock),
+                       check_lds_bytes(plant->img, var), (hipStream_t)stream, plant->img, nodes,
+                       num_nodes, dim, edges, num_edges, step, rule, out);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

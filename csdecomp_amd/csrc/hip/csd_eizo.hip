@@ -5,6 +5,8 @@
 // geometry/cuda_hit_and_run_sampling.cu. Differences, all deliberate:
 //  * many regions (edges) advance round by round together; one launch per
 //    stage per round serves every active region;
+//  * chain i of N starts at fraction (i + 1/2) / N of the segment instead of
+//    at line_start;
 //  * a lane runs one hit-and-run chain; the samples of a block stay in that
 //    block's staging segment, where the collision check compacts the
 //    colliding ones in place;
@@ -21,6 +23,7 @@
 #include <vector>
 
 #include "csd_eizo_device.h"
+#include "csd_mlp_internal.h"
 
 namespace csd {
 
@@ -30,15 +33,15 @@ namespace csd {
 // occupancy their small register footprint allows.
 template <int DPL>
 __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim, int cap,
-                                                              int nblk, uint32_t M, int mixing,
-                                                              uint32_t k0, uint32_t k1,
-                                                              uint32_t round) {
+                                                              int nblk, int mixing, uint32_t k0,
+                                                              uint32_t k1, uint32_t round,
+                                                              uint32_t stream_off) {
   constexpr int DP = SamplerShape<DPL>::DP;
   constexpr bool LDS = SamplerShape<DPL>::LDS;
   extern __shared__ float csd_smem[];
   const int slot = blockIdx.y;
   const int r = D.active[slot];
-  const uint32_t N = max(M, D.budget[r]);
+  const uint32_t N = D.nsamp[r];
   if (blockIdx.x * kSampleBlock >= N) return;  // whole block past this region's samples
   const int F = D.nf[r];
   const float *A = D.A + (size_t)r * cap * dim, *b = D.b + (size_t)r * cap;
@@ -49,10 +52,18 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
     b = csd_smem + (size_t)F * DP;
   }
   const uint32_t i = blockIdx.x * kSampleBlock + threadIdx.x;
+  // chain i starts at fraction (i + 1/2) / N of the segment, which lies in
+  // the region: spread starts cover elongated regions better than starts at
+  // line_start only
+  const float f = ((float)i + 0.5f) / (float)N;
   float x[DP];
 #pragma unroll
-  for (int d = 0; d < DP; ++d) x[d] = d < dim ? D.ls[(size_t)r * dim + d] : 0.f;
-  hit_and_run<DP, LDS>(A, b, F, dim, x, i, (uint32_t)r, round, mixing, k0, k1);
+  for (int d = 0; d < DP; ++d) {
+    const float a = d < dim ? D.ls[(size_t)r * dim + d] : 0.f;
+    const float e = d < dim ? D.le[(size_t)r * dim + d] : 0.f;
+    x[d] = fmaf(f, e - a, a);
+  }
+  hit_and_run<DP, LDS>(A, b, F, dim, x, i, (uint32_t)r + stream_off, round, mixing, k0, k1);
   float *seg = D.stage + ((size_t)slot * nblk + blockIdx.x) * kSampleBlock * dim;
 #pragma unroll
   for (int d = 0; d < DP; ++d)
@@ -69,7 +80,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
                                                              int nblk, uint32_t M) {
   const int slot = blockIdx.y;
   const int r = D.active[slot];
-  const uint32_t N = max(M, D.budget[r]);
+  const uint32_t N = D.nsamp[r];
   if (blockIdx.x * kSampleBlock >= N) {
     if (threadIdx.x == 0) {
       D.blk_cnt[(size_t)slot * nblk + blockIdx.x] = 0;
@@ -77,13 +88,15 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
     }
     return;
   }
+  extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  const float *tab = stage_tables<VAR>(P, csd_smem);
   const uint32_t i = blockIdx.x * kSampleBlock + threadIdx.x;
   const bool valid = i < N;
   float *seg = D.stage + ((size_t)slot * nblk + blockIdx.x) * kSampleBlock * dim;
   float x[DP];
 #pragma unroll
   for (int d = 0; d < DP; ++d) x[d] = d < dim ? seg[(size_t)d * kSampleBlock + threadIdx.x] : 0.f;
-  const bool col = collides_variant<VAR>(P, QReg<DP>{x}, V, R, !valid) && valid;
+  const bool col = collides_variant<VAR>(P, QReg<DP>{x}, V, R, !valid, tab) && valid;
 
   __shared__ uint32_t wcnt[kSampleBlock / 64], wfirst[kSampleBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -114,13 +127,14 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
   }
 }
 
-// Stage 2: unadaptive test (cuda_edge_inflation_zero_order.cu:165-199) and
-// gather of the first budget_r colliding samples, one workgroup per region.
+// Stage 2: unadaptive test (cuda_edge_inflation_zero_order.cu:165-199), when
+// `test`, and gather of the first budget_r colliding samples, one workgroup
+// per region. Without the test the collision count goes to pilot_col.
 __global__ __launch_bounds__(256) void k_eizo_compact(EizoDev D, int dim, int nblk, uint32_t M,
-                                                      float thresh, int U) {
+                                                      float thresh, int U, int test) {
   const int slot = blockIdx.x;
   const int r = D.active[slot];
-  const uint32_t N = max(M, D.budget[r]);
+  const uint32_t N = D.nsamp[r];
   __shared__ uint32_t s_off[1024];
   __shared__ uint32_t s_tot, s_first;
   __shared__ int s_u;
@@ -135,9 +149,9 @@ __global__ __launch_bounds__(256) void k_eizo_compact(EizoDev D, int dim, int nb
     s_first = first;
     D.samples[r] += N;
     D.collisions[r] += acc;
-    D.iters[r] += 1;
-    const bool certified = (float)first <= thresh;
+    const bool certified = test && (float)first <= thresh;
     if (certified) D.status[r] = 1;
+    if (!test) D.pilot_col[r] = acc;
     s_u = certified ? 0 : (int)min(acc, min(D.budget[r], (uint32_t)U));
     D.u[r] = s_u;
   }
@@ -156,6 +170,106 @@ __global__ __launch_bounds__(256) void k_eizo_compact(EizoDev D, int dim, int nb
   }
 }
 
+// Surrogate ranking: from the surrogate's values at L points between each
+// particle's projection and the particle (pred [slot][cand][L], positive =
+// free), the first point predicted in collision gives an estimate of how far
+// along the way the boundary lies (linear interpolation; the particle itself
+// collides). Particles are ordered by that estimated distance to the segment
+// and the `keep` nearest of the first `cand` stay, in that order (ties by
+// index), as the particles to bisect.
+constexpr int kRankBlock = 1024;
+constexpr int kRankMax = 1024;  // particles ranked per region (the first ones)
+
+__global__ __launch_bounds__(kRankBlock) void k_eizo_rank(EizoDev D, const float *__restrict__ pred,
+                                                          int dim, int U, int L, int cand,
+                                                          int keep, float *__restrict__ tmp) {
+  const int slot = blockIdx.x;
+  const int r = D.active[slot];
+  if (D.status[r] != 0) return;
+  const int u = D.u[r];
+  if (u <= keep) return;
+  const int n = min(u, cand);
+  __shared__ float key[kRankMax];
+  __shared__ float s_v[kRankBlock / 64];
+  __shared__ int s_i[kRankBlock / 64];
+  __shared__ int sel[256];
+  const float *q1 = D.ls + (size_t)r * dim, *q2 = D.le + (size_t)r * dim;
+  for (int i = threadIdx.x; i < n; i += kRankBlock) {
+    const float *p = D.part + ((size_t)r * U + i) * dim;
+    float a = 0.f, bb = 0.f;
+    for (int d = 0; d < dim; ++d) {
+      a = fmaf(p[d] - q1[d], q2[d] - q1[d], a);
+      bb = fmaf(q2[d] - q1[d], q2[d] - q1[d], bb);
+    }
+    const float t = fminf(fmaxf(a / bb, 0.f), 1.f);
+    float len2 = 0.f;
+    for (int d = 0; d < dim; ++d) {
+      const float e = p[d] - ((1.f - t) * q1[d] + t * q2[d]);
+      len2 = fmaf(e, e, len2);
+    }
+    const float *pv = pred + ((size_t)slot * cand + i) * L;
+    float prev = 1.f, frac = 1.f;  // the projection is taken as free
+    for (int l = 0; l < L; ++l) {
+      const float v = pv[l];
+      if (v < 0.f) {
+        const float f0 = (float)l / (float)(L + 1), f1 = (float)(l + 1) / (float)(L + 1);
+        frac = f0 + (f1 - f0) * (prev / (prev - v));
+        break;
+      }
+      prev = v;
+    }
+    key[i] = frac * sqrtf(len2);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = min(keep, 256);
+  for (int s = 0; s < k; ++s) {
+    float v = INFINITY;
+    int vi = 0x7fffffff;
+    for (int i = threadIdx.x; i < n; i += kRankBlock)
+      if (key[i] < v || (key[i] == v && i < vi)) {
+        v = key[i];
+        vi = i;
+      }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float ov = __shfl_xor(v, o);
+      const int oi = __shfl_xor(vi, o);
+      if (ov < v || (ov == v && oi < vi)) {
+        v = ov;
+        vi = oi;
+      }
+    }
+    if (lane == 0) {
+      s_v[wave] = v;
+      s_i[wave] = vi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float bv = s_v[0];
+      int bi = s_i[0];
+      for (int w = 1; w < kRankBlock / 64; ++w)
+        if (s_v[w] < bv || (s_v[w] == bv && s_i[w] < bi)) {
+          bv = s_v[w];
+          bi = s_i[w];
+        }
+      sel[s] = bi;
+      key[bi] = INFINITY;
+    }
+    __syncthreads();
+  }
+  // reorder: selected particles to the front (through tmp [slot][256][dim])
+  float *t = tmp + (size_t)slot * 256 * dim;
+  for (int e = threadIdx.x; e < k * dim; e += kRankBlock) {
+    const int j = e / dim, d = e - j * dim;
+    t[e] = D.part[((size_t)r * U + sel[j]) * dim + d];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < k * dim; e += kRankBlock)
+    D.part[(size_t)r * U * dim + e] = t[e];
+  if (threadIdx.x == 0) D.u[r] = k;
+}
+
 // Stage 3: projection + all bisection steps per lane; FP64 distance from
 // the last colliding point to the (pre-bisection) projection.
 template <int VAR, int DP>
@@ -167,6 +281,8 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
   const int u = D.u[r];
   const int i = blockIdx.x * kSampleBlock + threadIdx.x;
   if ((int)(blockIdx.x * kSampleBlock) >= u) return;
+  extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  const float *tab = stage_tables<VAR>(P, csd_smem);
   const bool valid = i < u;
   const int ii = valid ? i : u - 1;
   float p[DP], q1[DP], q2[DP], sc[DP];
@@ -176,7 +292,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
     q1[d] = d < dim ? D.ls[(size_t)r * dim + d] : 0.f;
     q2[d] = d < dim ? D.le[(size_t)r * dim + d] : 0.f;
   }
-  const float t = bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc);
+  const float t = bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc, tab);
   if (!valid) return;
   double dd = 0.0;
 #pragma unroll
@@ -207,6 +323,13 @@ struct csd_eizo {
   void *arena = nullptr;
   size_t stage_bytes = 0, blk_entries = 0;
   std::vector<void *> owned;
+  // guidance (csd_eizo_set_guidance)
+  uint32_t pilot = 0;
+  float pilot_factor = 0.f;
+  const csd_mlp *surrogate = nullptr;
+  int keep = 64, line_points = 4;
+  float *pred = nullptr;  // [slots][cand][L] surrogate outputs
+  size_t pred_bytes = 0;
 };
 
 using namespace csd;
@@ -258,6 +381,8 @@ int csd_eizo_create(const csd_plant *plant, const int32_t *robot_geoms, int32_t 
   if (!rc) rc = dalloc(w, &D.ls, R * dim);
   if (!rc) rc = dalloc(w, &D.le, R * dim);
   if (!rc) rc = dalloc(w, &D.budget, R);
+  if (!rc) rc = dalloc(w, &D.nsamp, R);
+  if (!rc) rc = dalloc(w, &D.pilot_col, R);
   if (!rc) rc = dalloc(w, &D.active, R);
   if (!rc) rc = dalloc(w, &D.part, R * U * dim);
   if (!rc) rc = dalloc(w, &D.scol, R * U * dim);
@@ -284,7 +409,27 @@ int csd_eizo_destroy(csd_eizo *w) {
   if (w->D.stage) hipFree(w->D.stage);
   if (w->D.blk_cnt) hipFree(w->D.blk_cnt);
   if (w->D.blk_first) hipFree(w->D.blk_first);
+  if (w->pred) hipFree(w->pred);
   delete w;
+  return CSD_OK;
+}
+
+int csd_eizo_set_guidance(csd_eizo *w, uint32_t pilot_samples, float pilot_factor,
+                          const csd_mlp *surrogate, int32_t keep_particles, int32_t line_points) {
+  if (!w) return fail(CSD_ERR_INVALID_ARGUMENT, "null workspace");
+  if (pilot_samples > 262144) return fail(CSD_ERR_CAPACITY, "at most 262144 pilot samples");
+  if (pilot_samples && !(pilot_factor > 0.f))
+    return fail(CSD_ERR_INVALID_ARGUMENT, "pilot_factor must be positive");
+  if (surrogate && (keep_particles < 1 || keep_particles > 256 || line_points < 1 ||
+                    line_points > 64))
+    return fail(CSD_ERR_INVALID_ARGUMENT, "keep_particles must be 1..256 and line_points 1..64");
+  w->pilot = pilot_samples;
+  w->pilot_factor = pilot_factor;
+  w->surrogate = surrogate;
+  if (surrogate) {
+    w->keep = keep_particles;
+    w->line_points = line_points;
+  }
   return CSD_OK;
 }
 
@@ -344,18 +489,26 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
   if (rc) return rc;
   const PlantImage &P = w->plant->img;
   const int var = check_variant(P, num_voxels > 0 && !all_slots);
+  const size_t tab_bytes = check_lds_bytes(P, var);
   const uint32_t k0 = (uint32_t)o.seed, k1 = (uint32_t)(o.seed >> 32);
   std::vector<int32_t> status(Rn), nfh(nf);  // nfh: face counts on the host
 
-  for (uint32_t it = 0; it < o.max_iterations && !active.empty(); ++it) {
-    const int na = (int)active.size();
-    // cuda_edge_inflation_zero_order.cu:113-119 (float arithmetic)
-    const float delta_k = (float)(6 * o.delta / (M_PI * M_PI * (it + 1) * (it + 1)));
-    const uint32_t M = (uint32_t)ceilf(2 * logf(1 / delta_k) / (o.epsilon * o.tau * o.tau));
-    const uint32_t N = std::max(M, max_budget);
-    const int nblk = (int)((N + kSampleBlock - 1) / kSampleBlock);
+  std::vector<int32_t> rounds(Rn, 0), pcol(Rn, 0);
+  std::vector<uint32_t> nsamp(Rn, 0);
+  const int dp = dim <= 8 ? 8 : 16;
+
+  // one sampling phase over the regions in `slots`: nsamp[r] samples each
+  // (stream offset: independent random numbers for the pilot), collision
+  // check, compaction (+ the unadaptive test when `test`)
+  auto phase = [&](const std::vector<int32_t> &slots, uint32_t it, uint32_t M, bool test,
+                   uint32_t stream_off) -> int {
+    const int na = (int)slots.size();
+    uint32_t nmax = 0;
+    for (int r : slots) nmax = std::max(nmax, nsamp[r]);
+    const int nblk = (int)((nmax + kSampleBlock - 1) / kSampleBlock);
     if (nblk > 1024) return fail(CSD_ERR_CAPACITY, "more than 262144 samples in one round");
-    const size_t stage_need = (size_t)na * nblk * kSampleBlock * dim * sizeof(float);
+    const size_t stage_need =
+        std::max((size_t)na * nblk * kSampleBlock, (size_t)na * 256) * dim * sizeof(float);
     if (stage_need > w->stage_bytes) {
       if (D.stage) hipFree(D.stage);
       D.stage = nullptr;
@@ -373,33 +526,84 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       CSD_HIP(hipMalloc(&D.blk_first, blk_need * 4));
       w->blk_entries = blk_need;
     }
-    CSD_HIP(hipMemcpy(D.active, active.data(), na * 4, hipMemcpyHostToDevice));
+    CSD_HIP(hipMemcpy(D.active, slots.data(), na * 4, hipMemcpyHostToDevice));
+    CSD_HIP(hipMemcpy(D.nsamp, nsamp.data(), Rn * 4, hipMemcpyHostToDevice));
     const float thresh = (1 - o.tau) * o.epsilon * M;
-    // the largest polytope of the round decides whether they are staged in LDS
+    // the largest polytope of the phase decides whether they are staged in LDS
     int nfmax = 0;
-    for (int r : active) nfmax = std::max(nfmax, nfh[r]);
+    for (int r : slots) nfmax = std::max(nfmax, nfh[r]);
     const dim3 sg(nblk, na), sb(kSampleBlock);
     if (dim <= 8 && polytope_lds_bytes<8>(nfmax) <= kLdsLimit)
       hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, polytope_lds_bytes<8>(nfmax), 0, D, dim,
-                         cap, nblk, M, (int)o.mixing_steps, k0, k1, it);
+                         cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
     else if (polytope_lds_bytes<16>(nfmax) <= kLdsLimit)
       hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, polytope_lds_bytes<16>(nfmax), 0, D, dim,
-                         cap, nblk, M, (int)o.mixing_steps, k0, k1, it);
+                         cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
     else
-      hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, 0, D, dim, cap, nblk, M,
-                         (int)o.mixing_steps, k0, k1, it);
-    if (dim <= 8)
-      CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 8, sg, sb, 0, 0, P, V, RL, D, dim, nblk, M);
+      hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, 0, D, dim, cap, nblk,
+                         (int)o.mixing_steps, k0, k1, it, stream_off);
+    if (dp == 8)
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 8, sg, sb, tab_bytes, 0, P, V, RL, D, dim, nblk,
+                             M);
     else
-      CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, 0, 0, P, V, RL, D, dim, nblk, M);
-    hipLaunchKernelGGL(k_eizo_compact, dim3(na), dim3(256), 0, 0, D, dim, nblk, M, thresh, U);
-    const int ublk = (int)((max_budget + kSampleBlock - 1) / kSampleBlock);
-    if (dim <= 8)
-      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 8, dim3(ublk, na), dim3(kSampleBlock), 0, 0, P,
-                             V, RL, D, dim, U, (int)o.bisection_steps);
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, 0, P, V, RL, D, dim,
+                             nblk, M);
+    hipLaunchKernelGGL(k_eizo_compact, dim3(na), dim3(256), 0, 0, D, dim, nblk, M, thresh, U,
+                       test ? 1 : 0);
+    CSD_HIP(hipGetLastError());
+    return CSD_OK;
+  };
+
+  for (uint32_t it = 0; it < o.max_iterations && !active.empty(); ++it) {
+    const int na = (int)active.size();
+    for (int r : active) rounds[r] += 1;
+    // cuda_edge_inflation_zero_order.cu:113-119 (float arithmetic)
+    const float delta_k = (float)(6 * o.delta / (M_PI * M_PI * (it + 1) * (it + 1)));
+    const uint32_t M = (uint32_t)ceilf(2 * logf(1 / delta_k) / (o.epsilon * o.tau * o.tau));
+    // pilot: regions whose pilot collision fraction is far above epsilon skip
+    // this round's test and take their particles from the pilot
+    std::vector<int32_t> full = active;
+    if (w->pilot) {
+      for (int r : active) nsamp[r] = w->pilot;
+      rc = phase(active, it, M, false, 0x40000000u);
+      if (rc) return rc;
+      CSD_HIP(hipMemcpy(pcol.data(), D.pilot_col, Rn * 4, hipMemcpyDeviceToHost));
+      full.clear();
+      const float limit = w->pilot_factor * o.epsilon * (float)w->pilot;
+      for (int r : active)
+        if ((float)pcol[r] <= limit) full.push_back(r);
+    }
+    if (!full.empty()) {
+      for (int r : full) nsamp[r] = std::max(M, budget[r]);
+      rc = phase(full, it, M, true, 0u);
+      if (rc) return rc;
+    }
+    if (w->pilot || full.size() != active.size())
+      CSD_HIP(hipMemcpy(D.active, active.data(), na * 4, hipMemcpyHostToDevice));
+    if (w->surrogate) {
+      const int L = w->line_points, cand = kRankMax;
+      const size_t need = (size_t)na * cand * L * sizeof(float);
+      if (need > w->pred_bytes) {
+        if (w->pred) hipFree(w->pred);
+        w->pred = nullptr;
+        w->pred_bytes = 0;
+        CSD_HIP(hipMalloc(&w->pred, need));
+        w->pred_bytes = need;
+      }
+      SegmentPoints sp{D.part, D.ls, D.le, D.active, D.u, U, dim, L, cand};
+      rc = mlp_predict_segment_points(w->surrogate, sp, na, w->pred, 0);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_eizo_rank, dim3(na), dim3(kRankBlock), 0, 0, D, w->pred, dim, U, L,
+                         cand, w->keep, D.stage);
+    }
+    const uint32_t bmax = w->surrogate ? (uint32_t)w->keep : max_budget;
+    const int ublk = (int)((bmax + kSampleBlock - 1) / kSampleBlock);
+    if (dp == 8)
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 8, dim3(ublk, na), dim3(kSampleBlock),
+                             tab_bytes, 0, P, V, RL, D, dim, U, (int)o.bisection_steps);
     else
-      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 16, dim3(ublk, na), dim3(kSampleBlock), 0, 0, P,
-                             V, RL, D, dim, U, (int)o.bisection_steps);
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 16, dim3(ublk, na), dim3(kSampleBlock),
+                             tab_bytes, 0, P, V, RL, D, dim, U, (int)o.bisection_steps);
     hipLaunchKernelGGL(k_eizo_faces, dim3(na), dim3(kFaceBlock), 0, 0, D, dim, U, cap,
                        o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1);
     CSD_HIP(hipGetLastError());
@@ -413,16 +617,15 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
 
   CSD_HIP(hipMemcpy(out_A, D.A, (size_t)Rn * cap * dim * 4, hipMemcpyDeviceToHost));
   CSD_HIP(hipMemcpy(out_b, D.b, (size_t)Rn * cap * 4, hipMemcpyDeviceToHost));
-  std::vector<int32_t> iters(Rn), fa(Rn);
+  std::vector<int32_t> fa(Rn);
   std::vector<int64_t> smp(Rn), colls(Rn);
   CSD_HIP(hipMemcpy(nf.data(), D.nf, Rn * 4, hipMemcpyDeviceToHost));
-  CSD_HIP(hipMemcpy(iters.data(), D.iters, Rn * 4, hipMemcpyDeviceToHost));
   CSD_HIP(hipMemcpy(fa.data(), D.faces_added, Rn * 4, hipMemcpyDeviceToHost));
   CSD_HIP(hipMemcpy(smp.data(), D.samples, Rn * 8, hipMemcpyDeviceToHost));
   CSD_HIP(hipMemcpy(colls.data(), D.collisions, Rn * 8, hipMemcpyDeviceToHost));
   CSD_HIP(hipMemcpy(status.data(), D.status, Rn * 4, hipMemcpyDeviceToHost));
   for (int r = 0; r < Rn; ++r) {
-    infos[r].iterations = iters[r];
+    infos[r].iterations = rounds[r];
     infos[r].converged = status[r] == 1;
     infos[r].capacity_reached = status[r] == 2;
     infos[r].pad = 0;

@@ -148,7 +148,9 @@ struct EizoDev {
   int32_t *nf;             // [R]
   float *ls, *le;          // [R][dim]
   uint32_t *budget;        // [R]
-  int32_t *active;         // [R] region ids of this round
+  uint32_t *nsamp;         // [R] samples drawn by the current sampling phase
+  uint32_t *pilot_col;     // [R] colliding samples of the last untested phase
+  int32_t *active;         // [R] region ids of the current launch
   float *stage;            // [nactive][nblk] segments of 256*dim: samples ([dim][256]), then
                            // the colliding ones compacted in place ([count][dim])
   uint32_t *blk_cnt;       // [nactive][nblk]
@@ -173,7 +175,8 @@ template <int VAR, int DP>
 __device__ __forceinline__ float bisect_particle(const PlantImage &P, const VoxelArgs &V,
                                                  const RobotGeomList &R, const float (&p)[DP],
                                                  const float (&q1)[DP], const float (&q2)[DP],
-                                                 int steps, bool idle, float (&sc)[DP]) {
+                                                 int steps, bool idle, float (&sc)[DP],
+                                                 const float *tab) {
   float a = 0.f, bb = 0.f;
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
@@ -191,7 +194,7 @@ __device__ __forceinline__ float bisect_particle(const PlantImage &P, const Voxe
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
     for (int d = 0; d < DP; ++d) mid[d] = 0.5f * (hi[d] + lo[d]);
-    const bool col = collides_variant<VAR>(P, QReg<DP>{mid}, V, R, idle);
+    const bool col = collides_variant<VAR>(P, QReg<DP>{mid}, V, R, idle, tab);
 #pragma unroll
     for (int d = 0; d < DP; ++d) {
       hi[d] = col ? mid[d] : hi[d];

@@ -88,16 +88,16 @@ struct PlantImage {
   // bit s of obstacle_mask[2o + s/64]: sphere slot s is paired with
   // obstacle o (all 64-byte records and masks together fit the scalar cache)
   const uint64_t *obstacle_mask;  // [num_obstacles][2]
-  // plants of at most 32 sphere slots also get contact-distance tables:
+  // plants of at most 64 sphere slots also get contact-distance tables:
   // obstacle_r2[o][s] = r^2 (box) or (r + r_o)^2 (sphere, capsule) in FP32 as
   // the device computes it, -1 if slot s is not paired with obstacle o
-  const float *obstacle_r2;       // [num_obstacles][nslots] (nslots <= 32)
+  const float *obstacle_r2;       // [num_obstacles][nslots] (nslots <= 64)
   // self collision of the fast path as body-pair blocks: bit bb of
   // self_active[ba] says bodies ba < bb have sphere pairs; bit ka*kmax+kb
   // of self_bits[ba][bb] says slots (ba, ka) and (bb, kb) form a pair
   const uint64_t *self_active;    // [nbt]
   const uint64_t *self_bits;      // [nbt][nbt]
-  const float *self_r2;           // [nbt][nbt][kmax][kmax] (nslots <= 32), 0 if absent
+  const float *self_r2;           // [nbt][nbt][kmax][kmax] (nslots <= 64), 0 if absent
   const int32_t *geom_slot;       // [G] slot of moving sphere geometry, -1 else
 };
 

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "csd_collide.h"
+#include "csd_mlp_internal.h"
 
 namespace csd {
 namespace {
@@ -101,8 +102,52 @@ __device__ __forceinline__ void load_chunk(uint16_t *sw, const uint16_t *W, int 
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_mlp_infer(MlpDev M, const float *__restrict__ q,
-                                                      int64_t n, int64_t sn, int64_t sj,
+// raw inputs of row r: q[r * sn + k * sj]
+struct StridedInput {
+  const float *q;
+  int64_t sn, sj;
+  int in_dim;
+  __device__ __forceinline__ void load(int64_t r, float (&x)[kIn]) const {
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) x[k] = k < in_dim ? q[r * sn + (int64_t)k * sj] : 0.f;
+  }
+};
+
+// raw inputs of row (slot, i, l): a point between a particle's projection
+// onto its segment and the particle (projection as in the bisection)
+struct SegmentInput {
+  SegmentPoints s;
+  __device__ __forceinline__ void load(int64_t row, float (&x)[kIn]) const {
+    const int64_t per = (int64_t)s.cand * s.L;
+    const int slot = (int)(row / per);
+    const int rem = (int)(row - (int64_t)slot * per);
+    const int i = rem / s.L, l = rem - i * s.L;
+    const int r = s.active[slot];
+    const int n = min(s.u[r], s.cand);
+    const int ii = n > 0 ? min(i, n - 1) : 0;
+    const float *p = s.part + ((size_t)r * s.U + ii) * s.dim;
+    const float *q1 = s.ls + (size_t)r * s.dim, *q2 = s.le + (size_t)r * s.dim;
+    float a = 0.f, bb = 0.f;
+    for (int d = 0; d < s.dim; ++d) {
+      a = fmaf(p[d] - q1[d], q2[d] - q1[d], a);
+      bb = fmaf(q2[d] - q1[d], q2[d] - q1[d], bb);
+    }
+    const float t = fminf(fmaxf(a / bb, 0.f), 1.f);
+    const float f = (float)(l + 1) / (float)(s.L + 1);
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) {
+      float v = 0.f;
+      if (k < s.dim) {
+        const float pr = (1.f - t) * q1[k] + t * q2[k];
+        v = n > 0 ? fmaf(f, p[k] - pr, pr) : q1[k];
+      }
+      x[k] = v;
+    }
+  }
+};
+
+template <class In>
+__global__ __launch_bounds__(kBlock) void k_mlp_infer(MlpDev M, In in, int64_t n,
                                                       float *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) uint16_t sw[kW * kChunkPad];
   __shared__ __attribute__((aligned(16))) uint16_t st[4][32 * kTilePad];
@@ -114,15 +159,18 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(MlpDev M, const float *__r
   bf16x8 af[kW / 16];
   {
     const int64_t r = min(row0 + c, n - 1);
+    float raw[kIn];
+    in.load(r, raw);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       uint16_t h[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int k = 16 * s + 8 * hr + j;
-        float x = 0.f;
-        if (k < M.in_dim) x = (q[r * sn + (int64_t)k * sj] - M.lo[k]) * M.scale[k] - 1.f;
-        h[j] = f2bf(x);
+        // lane-dependent feature index as a select between two static ones
+        const int k0 = 16 * s + j, k1 = 16 * s + 8 + j;
+        const float x0 = k0 < M.in_dim ? (raw[k0] - M.lo[k0]) * M.scale[k0] - 1.f : 0.f;
+        const float x1 = k1 < M.in_dim ? (raw[k1] - M.lo[k1]) * M.scale[k1] - 1.f : 0.f;
+        h[j] = f2bf(hr ? x1 : x0);
       }
       af[s] = *reinterpret_cast<const bf16x8 *>(h);
     }
@@ -734,8 +782,9 @@ int csd_mlp_predict(const csd_mlp *m, const float *q, int64_t n, int64_t stride_
   if (!q || !out) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
   const int64_t blocks = (n + 4 * kRowsWave - 1) / (4 * kRowsWave);
   if (blocks > 0x7fffffffLL) return fail(CSD_ERR_CAPACITY, "too many rows");
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(kBlock), 0, (hipStream_t)stream,
-                     m->dev(), q, n, stride_n, stride_j, out);
+  hipLaunchKernelGGL((k_mlp_infer<StridedInput>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                     (hipStream_t)stream, m->dev(), StridedInput{q, stride_n, stride_j, m->in_dim},
+                     n, out);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }
@@ -779,6 +828,24 @@ int csd_mlp_train_step(csd_mlp *m, const float *q, const float *target, int32_t 
   return CSD_OK;
 }
 
+}  // extern "C"
+
+int csd::mlp_predict_segment_points(const csd_mlp *m, const SegmentPoints &s, int num_slots,
+                                    float *out, hipStream_t st) {
+  if (!m) return fail(CSD_ERR_INVALID_ARGUMENT, "null surrogate");
+  if (s.dim != m->in_dim) return fail(CSD_ERR_INVALID_ARGUMENT, "surrogate input dimension differs from the plant's");
+  const int64_t n = (int64_t)num_slots * s.cand * s.L;
+  if (n == 0) return CSD_OK;
+  const int64_t blocks = (n + 4 * kRowsWave - 1) / (4 * kRowsWave);
+  if (blocks > 0x7fffffffLL) return fail(CSD_ERR_CAPACITY, "too many rows");
+  hipLaunchKernelGGL((k_mlp_infer<SegmentInput>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                     m->dev(), SegmentInput{s}, n, out);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+extern "C" {
+
 int csd_signed_distance(const csd_plant *plant, const float *q, int64_t n, int64_t stride_n,
                         int64_t stride_j, float *out, void *stream) {
   if (!plant) return fail(CSD_ERR_INVALID_ARGUMENT, "null plant");
@@ -789,7 +856,7 @@ int csd_signed_distance(const csd_plant *plant, const float *q, int64_t n, int64
   if (!P.fast)
     return fail(CSD_ERR_UNSUPPORTED,
                 "signed distance needs every moving collision geometry to be a sphere");
-  const int var = check_variant(P, false);
+  const int var = fast_variant(P);
   const int64_t blocks = (n + 255) / 256;
   if (blocks > 0x7fffffffLL) return fail(CSD_ERR_CAPACITY, "too many configurations");
   hipStream_t st = (hipStream_t)stream;

@@ -1,0 +1,24 @@
+// csd_mlp_internal.h -- surrogate inference on points generated on the
+// device, used by the EI-ZO driver to rank particles.
+#pragma once
+#include "csd_internal.h"
+
+struct csd_mlp;
+
+namespace csd {
+
+// Row (slot, i, l), slot < num_slots, i < cand, l < L: the point at fraction
+// (l + 1) / (L + 1) from the projection of particle i of region
+// active[slot] onto its segment towards the particle. Rows past a region's
+// particle count repeat its last particle (their outputs are unused).
+struct SegmentPoints {
+  const float *part, *ls, *le;  // [R][U][dim], [R][dim], [R][dim]
+  const int32_t *active, *u;    // [slots], [R]
+  int U, dim, L, cand;
+};
+
+// out [num_slots][cand][L]: the surrogate's output at each point
+int mlp_predict_segment_points(const csd_mlp *m, const SegmentPoints &s, int num_slots,
+                               float *out, hipStream_t st);
+
+}  // namespace csd

@@ -415,7 +415,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   const int nbt_alloc = fast ? nbt : 1;
   std::vector<uint64_t> self_active(nbt_alloc, 0);
   std::vector<uint64_t> self_bits((size_t)nbt_alloc * nbt_alloc, 0);
-  const bool tables = fast && nslots <= 32;
+  const bool tables = fast && nslots <= 64;
   std::vector<float> self_r2(tables ? (size_t)nbt * nbt * kmax * kmax : 1, 0.f);
   int num_self_pairs = 0;
   if (fast) {

@@ -44,6 +44,8 @@ __global__ __launch_bounds__(kSampleBlock) void k_bisect_standalone(
     const float *__restrict__ ls, const float *__restrict__ le,
     const uint32_t *__restrict__ seg, int n, int dim, int steps, float *__restrict__ scol,
     float *__restrict__ proj) {
+  extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  const float *tab = stage_tables<VAR>(P, csd_smem);
   const int i = blockIdx.x * kSampleBlock + threadIdx.x;
   const bool valid = i < n;
   const int ii = valid ? i : n - 1;
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_bisect_standalone(
     q1[d] = d < dim ? ls[(size_t)s_idx * dim + d] : 0.f;
     q2[d] = d < dim ? le[(size_t)s_idx * dim + d] : 0.f;
   }
-  const float t = bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc);
+  const float t = bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc, tab);
   if (!valid) return;
 #pragma unroll
   for (int d = 0; d < DP; ++d)
@@ -135,12 +137,12 @@ int csd_bisect(const csd_plant *plant, const float *particles, const float *line
   const int var = check_variant(plant->img, num_voxels > 0 && !all_slots);
   dim3 g((num_particles + kSampleBlock - 1) / kSampleBlock);
   if (dim <= 8)
-    CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, 8, g, dim3(kSampleBlock), 0,
-                           (hipStream_t)stream, plant->img, V, R, particles, line_start, line_end,
+    CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, 8, g, dim3(kSampleBlock),
+                           check_lds_bytes(plant->img, var), (hipStream_t)stream, plant->img, V, R, particles, line_start, line_end,
                            segment_idx, num_particles, dim, steps, scol, proj);
   else
-    CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, 16, g, dim3(kSampleBlock), 0,
-                           (hipStream_t)stream, plant->img, V, R, particles, line_start, line_end,
+    CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, 16, g, dim3(kSampleBlock),
+                           check_lds_bytes(plant->img, var), (hipStream_t)stream, plant->img, V, R, particles, line_start, line_end,
                            segment_idx, num_particles, dim, steps, scol, proj);
   CSD_HIP(hipGetLastError());
   return CSD_OK;

@@ -449,6 +449,17 @@ PYBIND11_MODULE(_pycsdecomp, m) {
           },
           py::arg("line_starts"), py::arg("line_ends"), py::arg("voxels"),
           py::arg("voxel_radius"), py::arg("particle_budgets") = std::vector<uint32_t>())
+      .def(
+          "set_guidance",
+          [](CudaEdgeInflator &self, uint32_t pilot_samples, float pilot_factor,
+             uintptr_t surrogate, int keep_particles, int line_points) {
+            self.setGuidance(pilot_samples, pilot_factor,
+                             reinterpret_cast<const void *>(surrogate), keep_particles,
+                             line_points);
+          },
+          py::arg("pilot_samples") = 0, py::arg("pilot_factor") = 10.0f,
+          py::arg("surrogate") = 0, py::arg("keep_particles") = 64, py::arg("line_points") = 4,
+          "surrogate: the address of a csd_mlp (csdecomp_amd.capi.Mlp.address), 0 for none")
       .def("last_info", &CudaEdgeInflator::lastInfo)
       .def("face_capacity", &CudaEdgeInflator::faceCapacity);
 

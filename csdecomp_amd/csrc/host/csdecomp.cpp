@@ -940,6 +940,14 @@ CudaEdgeInflator::CudaEdgeInflator(const MinimalPlant &plant,
                "EI-ZO workspace");
 }
 
+void CudaEdgeInflator::setGuidance(uint32_t pilot_samples, float pilot_factor,
+                                   const void *surrogate, int keep_particles, int line_points) {
+  check_status(csd_eizo_set_guidance(ws_, pilot_samples, pilot_factor,
+                                     static_cast<const csd_mlp *>(surrogate), keep_particles,
+                                     line_points),
+               "EI-ZO guidance");
+}
+
 CudaEdgeInflator::~CudaEdgeInflator() {
   if (ws_) csd_eizo_destroy(ws_);
 }

@@ -324,6 +324,11 @@ class CudaEdgeInflator {
                                         const MatrixXf &voxels, float voxel_radius,
                                         const std::vector<uint32_t> &particle_budgets,
                                         std::vector<EizoInfo> *infos = nullptr);
+  // EI-ZO guidance (csd_eizo_set_guidance): an exact pilot that lets clearly
+  // colliding regions skip a round's test, and surrogate ranking of the
+  // particles to bisect. surrogate: a csd_mlp* (or null).
+  void setGuidance(uint32_t pilot_samples, float pilot_factor, const void *surrogate,
+                   int keep_particles = 64, int line_points = 4);
   const EizoInfo &lastInfo() const { return last_info_; }
   int faceCapacity() const { return face_capacity_; }
 

@@ -213,6 +213,24 @@ int csd_eizo_run(csd_eizo *ws, const float *domain_A, const float *domain_b,
                  int32_t num_voxels, float voxel_radius, float *out_A,
                  float *out_b, csd_eizo_region_info *infos);
 
+struct csd_mlp;
+
+/* Guidance for later csd_eizo_run calls (no counterpart in the reference;
+ * all off by default, which is the reference's round structure):
+ *  pilot_samples > 0: each round first checks pilot_samples samples of an
+ *    independent random stream; a region whose pilot collision fraction
+ *    exceeds pilot_factor * epsilon skips that round's unadaptive test and
+ *    takes its particles from the pilot. The test, when run, draws its own
+ *    samples, so certification keeps the reference's guarantee.
+ *  surrogate != NULL: particles are ranked by the surrogate's predicted
+ *    boundary distance (signed distance at line_points points between each
+ *    particle's projection and the particle) and only the keep_particles best
+ *    of the first 1024 are bisected; the exact checker still judges every
+ *    bisection step and every sample. */
+int csd_eizo_set_guidance(csd_eizo *ws, uint32_t pilot_samples, float pilot_factor,
+                          const struct csd_mlp *surrogate, int32_t keep_particles,
+                          int32_t line_points);
+
 /* EI-ZO stage kernels exposed for stage-wise parity tests (device arrays).
  * Replaces executeProjectSamplesOntoLineSegmentsKernel and the bisection
  * kernels (planning/cuda_set_builder_utils.h:25-131), fused: per particle,

@@ -334,3 +334,78 @@ def test_knn_vs_brute_force(gpu):
         order = np.lexsort((np.arange(3000), dd))[:k]
         assert list(order) == list(idx[i]), i
     assert 17 not in idx[5] and 5 not in idx[17]
+
+
+def _train_surrogate(gpu, pl, steps=300, n=262144, seed=5):
+    import torch
+    from csdecomp_amd import capi
+    mp = pl.getMinimalPlant()
+    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
+    dof = len(lo)
+    q = torch.from_numpy(np.random.default_rng(seed).uniform(lo, hi, (n, dof)).astype(np.float32)).to(gpu)
+    sd = torch.empty(n, device=gpu)
+    capi.signed_distance(capi.DevicePlant(mp), q, n, dof, 1, sd)
+    t = (sd.clamp(-0.2, 0.2) / 0.2).contiguous()
+    m = capi.Mlp(dof, lo, hi, seed)
+    B = 8192
+    for s in range(steps):
+        k = (s * B) % n
+        m.train_step(q[k:k + B], t[k:k + B], B, 1e-3)
+    torch.cuda.synchronize()
+    return m
+
+
+def test_guided_eizo_certifies(gpu, csd, orc):
+    """Pilot rounds and surrogate-ranked particles: every certified region
+    contains its segment and passes an independent collision-fraction check;
+    runs are reproducible."""
+    from csdecomp_amd import scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    op = orc.Plant(mp)
+    ls, le = free_edges(orc, op, lo, hi, 6, (0.1, 0.4), seed=23)
+    R = ls.shape[1]
+    dom = csd.HPolyhedron()
+    dom.MakeBox(lo, hi)
+    opt = csd.EizoOptions()
+    opt.epsilon = 0.01
+    opt.num_particles = 2000
+    opt.bisection_steps = 9
+    opt.max_iterations = 80
+    rg = list(pl.getRobotGeometryIds())
+    model = _train_surrogate(gpu, pl)
+    results = {}
+    for mode in ("plain", "pilot", "guided"):
+        inf = csd.CudaEdgeInflator(mp, rg, opt, dom, R, 10)
+        if mode == "pilot":
+            inf.set_guidance(pilot_samples=2048, pilot_factor=10.0)
+        elif mode == "guided":
+            inf.set_guidance(pilot_samples=2048, pilot_factor=10.0, surrogate=model.address,
+                             keep_particles=32, line_points=4)
+        regions, infos = inf.inflateEdges(ls, le, None, 0.0, [])
+        again, _ = inf.inflateEdges(ls, le, None, 0.0, [])
+        results[mode] = infos
+        for r in range(R):
+            P = regions[r]
+            assert P.PointInSet(ls[:, r], 1e-5) and P.PointInSet(le[:, r], 1e-5)
+            np.testing.assert_array_equal(P.A(), again[r].A())
+            if infos[r].converged:
+                c = ((ls[:, r] + le[:, r]) / 2).astype(np.float32)
+                s = csd.UniformSampleInHPolyhedraCuda([P], c[:, None], 3000, 50, seed=99 + r)[0]
+                frac = 1 - np.mean(csd.CheckCollisionFreeCuda(s, mp))
+                assert frac < 2.5 * opt.epsilon, (mode, r, frac)
+        assert sum(i.converged for i in infos) >= R - 1, mode
+
+
+def test_guidance_argument_checks(gpu, csd):
+    from csdecomp_amd import scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    dom = csd.HPolyhedron()
+    dom.MakeBox(pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
+    inf = csd.CudaEdgeInflator(pl.getMinimalPlant(), list(pl.getRobotGeometryIds()), csd.EizoOptions(), dom)
+    with pytest.raises((ValueError, RuntimeError)):
+        inf.set_guidance(pilot_samples=1000, pilot_factor=0.0)
+    with pytest.raises((ValueError, RuntimeError)):
+        inf.set_guidance(surrogate=1, keep_particles=0)
