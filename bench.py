@@ -138,6 +138,47 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
     return out
 
 
+MLP_FLOPS_PER_ROW = 2 * (256 * 32 + 3 * 256 * 256 + 256)  # padded input, 4 hidden layers, head
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense
+
+
+def mlp_benchmark(rows=1 << 20, batch=8192, reps=20):
+    """Surrogate throughput: fused MFMA inference on `rows` configurations
+    (7 inputs) and one training step (forward, backward, Adam) at `batch`,
+    both timed with HIP events on the launch stream."""
+    import torch
+    from csdecomp_amd import capi
+    lo, hi = np.full(7, -np.pi, np.float32), np.full(7, np.pi, np.float32)
+    m = capi.Mlp(7, lo, hi, 3)
+    q = torch.rand(rows, 7, device="cuda") * 6.28 - 3.14
+    out = torch.empty(rows, device="cuda")
+    t = torch.rand(batch, device="cuda") * 2 - 1
+    s = torch.cuda.current_stream()
+    for _ in range(3):
+        m.predict(q, rows, 7, 1, out, s)
+        m.train_step(q[:batch], t, batch, 1e-3, None, s)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        m.predict(q, rows, 7, 1, out, s)
+    b.record(s)
+    torch.cuda.synchronize()
+    inf_ms = a.elapsed_time(b) / reps
+    a.record(s)
+    for _ in range(reps):
+        m.train_step(q[:batch], t, batch, 1e-3, None, s)
+    b.record(s)
+    torch.cuda.synchronize()
+    train_ms = a.elapsed_time(b) / reps
+    tflops = rows * MLP_FLOPS_PER_ROW / (inf_ms * 1e-3) / 1e12
+    return {"inference_rows": rows, "inference_ms": round(inf_ms, 4),
+            "inference_rows_per_s": round(rows / (inf_ms * 1e-3)),
+            "inference_tflops": round(tflops, 1), "bf16_peak_tflops": BF16_MFMA_PEAK_TFLOPS,
+            "inference_frac": round(tflops / BF16_MFMA_PEAK_TFLOPS, 4),
+            "train_batch": batch, "train_step_ms": round(train_ms, 4),
+            "train_tflops": round(3 * batch * MLP_FLOPS_PER_ROW / (train_ms * 1e-3) / 1e12, 1)}
+
+
 def eizo_benchmarks(segments=2048):
     """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
     segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
@@ -260,6 +301,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline(mp, q, args.cpu_seconds)
     if not args.no_eizo:
         line["eizo"] = eizo_benchmarks(args.eizo_segments)
+        line["surrogate_mlp"] = mlp_benchmark()
     print(json.dumps(line), flush=True)
 
 
