@@ -179,6 +179,55 @@ def mlp_benchmark(rows=1 << 20, batch=8192, reps=20):
             "train_tflops": round(3 * batch * MLP_FLOPS_PER_ROW / (train_ms * 1e-3) / 1e12, 1)}
 
 
+def roadmap_benchmark(nodes=200_000, k=10, step=0.01):
+    """Roadmap construction on the 7-DOF arm with 48 boxes + 16 spheres:
+    `nodes` configurations uniform in the limits, brute-force k-NN (squared L2
+    in joint space), every undirected k-NN edge validated at `step` rad with
+    the checkEdgeCollisionFree discretisation, and the robot-swept occupancy
+    of all nodes on a 64^3 grid over [-1, 1]^3 m. Timed with HIP events."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    dp = capi.DevicePlant(mp)
+    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
+    q = torch.from_numpy(np.random.default_rng(2003).uniform(lo, hi, (nodes, 7)).astype(np.float32)).cuda()
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    idx = torch.empty(nodes, k, dtype=torch.int32, device="cuda")
+    ev[0].record(s)
+    capi.knn(q, nodes, 7, k, idx, None, s)
+    ev[1].record(s)
+    # undirected, de-duplicated edge list (i < j)
+    src = torch.arange(nodes, device="cuda", dtype=torch.int64).repeat_interleave(k)
+    dst = idx.reshape(-1).to(torch.int64)
+    a, b = torch.minimum(src, dst), torch.maximum(src, dst)
+    keys = torch.unique(a * nodes + b)
+    edges = torch.stack([keys // nodes, keys % nodes], dim=1).to(torch.int32).contiguous()
+    E = edges.shape[0]
+    free = torch.empty(E, dtype=torch.uint8, device="cuda")
+    length = torch.linalg.vector_norm(q[edges[:, 0].long()] - q[edges[:, 1].long()], dim=1)
+    points = int(torch.ceil(length / step).sum().item())
+    torch.cuda.synchronize()
+    ev[2].record(s)
+    capi.validate_edges(dp, q, nodes, edges, E, step, 0, free, s)
+    ev[3].record(s)
+    grid = torch.zeros(64, 64, 64, dtype=torch.uint8, device="cuda")
+    ev[4].record(s)
+    capi.occupancy(dp, q, nodes, 7, 1, list(pl.getRobotGeometryIds()), [0, 0, 0], [2, 2, 2], 1.0 / 64,
+                   grid, s)
+    ev[5].record(s)
+    torch.cuda.synchronize()
+    knn_ms, edge_ms, occ_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3]), ev[4].elapsed_time(ev[5])
+    return {"nodes": nodes, "k": k, "knn_ms": round(knn_ms, 3), "edges": E,
+            "edge_points_max": points, "edges_free": int(free.sum().item()),
+            "edge_validation_ms": round(edge_ms, 3),
+            "edges_per_s": round(E / (edge_ms * 1e-3)),
+            "occupancy_voxels": int((grid > 0).sum().item()), "occupancy_ms": round(occ_ms, 3),
+            "occupancy_configs_per_s": round(nodes / (occ_ms * 1e-3))}
+
+
 def eizo_benchmarks(segments=2048):
     """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
     segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
@@ -245,6 +294,9 @@ def main():
     def step():
         capi.check_collision_free(dp, dq, n, 1, n, out, stream)
 
+    # autotuning (outside the timed region): waves per group of 64 configurations
+    split = capi.tune_check(dp, dq, n, 1, n, out, stream)
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -292,7 +344,7 @@ def main():
         "config": {"workload": "FK + collision check, 7-DOF arm, 32 spheres x 16 boxes, 784 pairs",
                    "num_configs": n, "num_collision_pairs": mp.num_collision_pairs,
                    "layout": "SoA [7][N] f32 in HBM", "collision_free_fraction": round(free_frac, 4),
-                   "kernel": "k_check<fast 8x4>"},
+                   "kernel": "k_check<fast 8x4>", "waves_per_64_configs": split},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_VECTOR_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP32_VECTOR_PEAK_TFLOPS,
                      "traffic": traffic, "flops_per_config": f_cfg, "kernel_ms": kern_ms},
@@ -302,6 +354,7 @@ def main():
     if not args.no_eizo:
         line["eizo"] = eizo_benchmarks(args.eizo_segments)
         line["surrogate_mlp"] = mlp_benchmark()
+        line["roadmap"] = roadmap_benchmark()
     print(json.dumps(line), flush=True)
 
 

@@ -51,6 +51,7 @@ _SIGS = {
     "csd_mlp_gradients": (C.c_int, [_VP, _VP, _VP, _I32, _VP, _VP, _VP]),
     "csd_mlp_train_step": (C.c_int, [_VP, _VP, _VP, _I32, _F, _VP, _VP]),
     "csd_signed_distance": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _VP]),
+    "csd_tune_check": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _VP, C.POINTER(_I32)]),
 }
 
 _lib = None
@@ -225,6 +226,15 @@ def knn(nodes, num_nodes, dim, k, out_idx, out_d2=None, stream=None):
 
 def synchronize(stream=None):
     check(lib().csd_stream_synchronize(_stream(stream)), "csd_stream_synchronize")
+
+
+def tune_check(plant, q, n, stride_n, stride_j, out, stream=None):
+    """Autotunes the checker on this batch (see csd_tune_check); returns the
+    number of waves that share each group of 64 configurations."""
+    s = _I32(0)
+    check(lib().csd_tune_check(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+                               _stream(stream), C.byref(s)), "csd_tune_check")
+    return int(s.value)
 
 
 def signed_distance(plant, q, n, stride_n, stride_j, out, stream=None):

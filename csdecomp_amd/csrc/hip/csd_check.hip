@@ -14,20 +14,39 @@ namespace csd {
 
 constexpr int kCheckBlock = 256;
 
-template <int VAR>
+// S waves share each group of 64 configurations: wave w of a block takes
+// part w % S of the pair tests of group w / S (all of them when S = 1), and
+// the parts are OR-ed through LDS. S > 1 fills the chip when the batch is
+// small (65,536 configurations are 1,024 waves, one per SIMD).
+template <int VAR, int S>
 __global__ __launch_bounds__(kCheckBlock) void k_check(PlantImage P, const float *__restrict__ q,
                                                        int64_t n, int64_t sn, int64_t sj,
                                                        VoxelArgs V, RobotGeomList R,
                                                        uint8_t *__restrict__ out) {
-  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int kPerBlock = kCheckBlock / S;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = wave % S, grp = wave / S;
+  const int64_t gi = (int64_t)blockIdx.x * kPerBlock + grp * 64 + lane;
   const bool valid = gi < n;
   const int64_t i = valid ? gi : n - 1;  // idle tail lanes re-read the last config
   const QMem qa{q, i * sn, sj};
   extern __shared__ __attribute__((aligned(16))) float csd_smem[];
   const float *tab = stage_tables<VAR>(P, csd_smem);
   // idle lanes start "done" so they never hold the wave back
-  const bool col = collides_variant<VAR>(P, qa, V, R, !valid, tab);
-  if (valid) out[gi] = col ? 0 : 1;
+  const bool col = collides_variant<VAR>(P, qa, V, R, !valid, tab, sub, S);
+  if constexpr (S == 1) {
+    if (valid) out[gi] = col ? 0 : 1;
+  } else {
+    __shared__ uint8_t part[kCheckBlock];
+    part[threadIdx.x] = col ? 1 : 0;
+    __syncthreads();
+    if (sub == 0 && valid) {
+      bool c = false;
+#pragma unroll
+      for (int s = 0; s < S; ++s) c |= part[(grp * S + s) * 64 + lane] != 0;
+      out[gi] = c ? 0 : 1;
+    }
+  }
 }
 
 // per (config, voxel): 1 iff no robot geometry touches the voxel
@@ -108,6 +127,34 @@ int check_args(const csd_plant *pl, const float *q, int64_t n, int64_t sn, int64
 }
 
 dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kCheckBlock - 1) / kCheckBlock)); }
+
+int size_class(int64_t n) { return n <= 0 ? 0 : 64 - __builtin_clzll((unsigned long long)n); }
+
+// waves per configuration group: the tuned value for this batch-size class,
+// else enough to put about four waves on each of the 1,024 SIMDs
+int check_split(const csd_plant *pl, int64_t n, int var) {
+  if (var >= kGeneric8) return 1;
+  const int t = pl->check_split[size_class(n)];
+  if (t) return t;
+  const int64_t waves = (n + 63) / 64;
+  return waves >= 4096 ? 1 : waves >= 2048 ? 2 : 4;
+}
+
+void launch_k_check(int var, int split, const csd_plant *pl, const float *q, int64_t n,
+                    int64_t sn, int64_t sj, const VoxelArgs &V, const RobotGeomList &R,
+                    uint8_t *out, hipStream_t st) {
+  const size_t lds = check_lds_bytes(pl->img, var);
+  const dim3 b(kCheckBlock);
+  if (split == 4)
+    CSD_DISPATCH_VARIANT_X(var, k_check, 4, dim3((unsigned)((n + 63) / 64)), b, lds, st, pl->img,
+                           q, n, sn, sj, V, R, out);
+  else if (split == 2)
+    CSD_DISPATCH_VARIANT_X(var, k_check, 2, dim3((unsigned)((n + 127) / 128)), b, lds, st,
+                           pl->img, q, n, sn, sj, V, R, out);
+  else
+    CSD_DISPATCH_VARIANT_X(var, k_check, 1, grid_for(n), b, lds, st, pl->img, q, n, sn, sj, V,
+                           R, out);
+}
 }  // namespace
 
 bool fill_robot_list(const csd_plant *pl, const int32_t *robot, int32_t nr, RobotGeomList &R,
@@ -156,8 +203,7 @@ int launch_check(const csd_plant *pl, const float *q, int64_t n, int64_t sn, int
   bool all_slots = fill_robot_list(pl, robot, nvox > 0 ? nr : 0, R, V, &rc);
   if (rc) return rc;
   const int var = check_variant(pl->img, nvox > 0 && !all_slots);
-  CSD_DISPATCH_VARIANT(var, k_check, grid_for(n), dim3(kCheckBlock),
-                       check_lds_bytes(pl->img, var), st, pl->img, q, n, sn, sj, V, R, out);
+  launch_k_check(var, check_split(pl, n, var), pl, q, n, sn, sj, V, R, out, st);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }
@@ -181,6 +227,56 @@ int csd_check_collision_free_voxels(const csd_plant *plant, const float *q, int6
                                     uint8_t *out, void *stream) {
   return launch_check(plant, q, n, stride_n, stride_j, voxels, num_voxels, voxel_radius,
                       robot_geoms, num_robot_geoms, out, (hipStream_t)stream);
+}
+
+int csd_tune_check(const csd_plant *plant, const float *q, int64_t n, int64_t stride_n,
+                   int64_t stride_j, uint8_t *out, void *stream, int32_t *split) {
+  int rc = check_args(plant, q, n, stride_n, stride_j, out);
+  if (rc) return rc;
+  if (n == 0) return CSD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  RobotGeomList R;
+  memset(&R, 0, sizeof(R));
+  const VoxelArgs V{nullptr, 0, 0.f, 0, 0};
+  const int var = check_variant(plant->img, false);
+  int best = 1;
+  if (var < kGeneric8) {
+    hipEvent_t e0, e1;
+    CSD_HIP(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+      hipEventDestroy(e0);
+      return hip_fail(e, "hipEventCreate");
+    }
+    float best_ms = INFINITY;
+    for (int s : {1, 2, 4}) {
+      launch_k_check(var, s, plant, q, n, stride_n, stride_j, V, R, out, st);  // warm-up
+      e = hipEventRecord(e0, st);
+      for (int k = 0; k < 5 && e == hipSuccess; ++k)
+        launch_k_check(var, s, plant, q, n, stride_n, stride_j, V, R, out, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = s;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "check autotuning");
+    const_cast<csd_plant *>(plant)->check_split[size_class(n)] = (int8_t)best;
+    launch_k_check(var, best, plant, q, n, stride_n, stride_j, V, R, out, st);
+  }
+  else {
+    rc = launch_check(plant, q, n, stride_n, stride_j, nullptr, 0, 0.f, nullptr, 0, out, st);
+    if (rc) return rc;
+  }
+  CSD_HIP(hipGetLastError());
+  if (split) *split = best;
+  return CSD_OK;
 }
 
 int csd_voxel_flags(const csd_plant *plant, const float *q, int64_t n, int64_t stride_n,

@@ -84,10 +84,14 @@ __device__ __forceinline__ void fast_spheres(const PlantImage &P, const Q &qa,
 // `col` enters as the lane's initial state (true = nothing left to decide,
 // e.g. an idle tail lane) and is returned OR-ed with the result.
 // `tab`: for 64 sphere slots, the plant's contact-distance tables staged in
-// LDS by the kernel (stage_tables); unused otherwise.
+// LDS by the kernel (stage_tables); unused otherwise. (sub, nsub): this call
+// tests only part `sub` of `nsub` of the work -- self-collision blocks of
+// bodies ba = sub mod nsub, obstacles and voxels o = sub mod nsub -- so that
+// nsub waves can share one configuration (the caller ORs the parts).
 template <int NB, int KM, bool ANY = false, class Q>
 __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
-                                              const VoxelArgs &V, bool col, const float *tab) {
+                                              const VoxelArgs &V, bool col, const float *tab,
+                                              int sub = 0, int nsub = 1) {
   constexpr int NS = NB * KM;
   // Contact distances of the pair tests, by plant size:
   //  * up to 32 sphere slots: per-pair tables read into scalar registers (no
@@ -111,7 +115,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
 #pragma unroll
   for (int ba = 0; ba < NB; ++ba) {
     const uint64_t act = P.self_active[ba];
-    if (act == 0 || wave_done<ANY>(col)) continue;
+    if (act == 0 || ba % nsub != sub || wave_done<ANY>(col)) continue;
 #pragma unroll
     for (int bb = ba + 1; bb < NB; ++bb) {
       if ((act >> bb) & 1) {
@@ -146,7 +150,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // (r + r_o)^2, -1 if the slot is not paired with o) are read once per
   // obstacle with a few wide scalar loads; the slot loop has no branch.
   if constexpr (kTab) {
-    for (int o = 0; o < P.num_obstacles; ++o) {
+    for (int o = sub; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
       float ov[16], r2[NS];
@@ -194,7 +198,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // its tests (one 16-byte read per 4 slots).
   if constexpr (kLds) {
     const float *orow = tab + NB * NB * KM * KM;
-    for (int o = 0; o < P.num_obstacles; ++o) {
+    for (int o = sub; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
       float ov[16];
@@ -237,20 +241,20 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // r * r (box) and (r + r_o) * (r + r_o) (sphere, capsule); a slot not
   // paired with the obstacle gets -1, which never reports a hit, so there is
   // no per-slot branch. Slots of absent bodies are skipped per body.
-  if (kMask && P.num_obstacles > 0) {
-    int tn = P.obstacle_type[0];
+  if (kMask && sub < P.num_obstacles) {
+    int tn = P.obstacle_type[sub];
     float ovn[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) ovn[k] = P.obstacles[0].v[k];
-    uint64_t m0n = P.obstacle_mask[0], m1n = P.obstacle_mask[1];
-    for (int o = 0; o < P.num_obstacles; ++o) {
+    for (int k = 0; k < 16; ++k) ovn[k] = P.obstacles[sub].v[k];
+    uint64_t m0n = P.obstacle_mask[2 * sub], m1n = P.obstacle_mask[2 * sub + 1];
+    for (int o = sub; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = tn;
       float ov[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) ov[k] = ovn[k];
       const uint64_t m0 = m0n, m1 = m1n;
-      const int onext = o + 1 < P.num_obstacles ? o + 1 : o;
+      const int onext = o + nsub < P.num_obstacles ? o + nsub : o;
       tn = P.obstacle_type[onext];
 #pragma unroll
       for (int k = 0; k < 16; ++k) ovn[k] = P.obstacles[onext].v[k];
@@ -307,7 +311,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
       const float rs = V.radius + P.slot_sphere[s].w;
       vr2[s] = on ? rs * rs : -1.f;
     }
-    for (int v = 0; v < V.num; ++v) {
+    for (int v = sub; v < V.num; v += nsub) {
       if (wave_done<ANY>(col)) break;
       const float vx = V.vox[3 * v], vy = V.vox[3 * v + 1], vz = V.vox[3 * v + 2];
 #pragma unroll
@@ -476,14 +480,18 @@ __device__ __forceinline__ const float *stage_tables(const PlantImage &P, float 
 
 // dispatch a device-side call on the variant (VAR is a compile-time value
 // inside the generic lambda-free macro body)
+// (sub, nsub): see collides_fast; the generic checker ignores the split
+// (part 0 does all of the work, other parts report no collision).
 template <int VAR, bool ANY = false, class Q>
 __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &qa,
                                                  const VoxelArgs &V, const RobotGeomList &R,
-                                                 bool col, const float *tab) {
-  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY>(P, qa, V, col, tab);
-  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY>(P, qa, V, col, tab);
-  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY>(P, qa, V, col, tab);
-  else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY>(P, qa, V, col, tab);
+                                                 bool col, const float *tab, int sub = 0,
+                                                 int nsub = 1) {
+  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY>(P, qa, V, col, tab, sub, nsub);
+  else if (sub != 0) return col;
   else if constexpr (VAR == kGeneric8) return collides_generic<8, ANY>(P, qa, V, R, col);
   else return collides_generic<16, ANY>(P, qa, V, R, col);
 }

@@ -4,6 +4,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <utility>
+
 #include "csd_collide.h"
 
 namespace csd {
@@ -153,44 +155,49 @@ __global__ __launch_bounds__(kEdgeBlock) void k_occupancy(PlantImage P, const fl
 // k nearest neighbours by squared L2 distance (sequential, uncontracted
 // float sum); candidates with d2 <= FLT_EPSILON (the node itself and exact
 // duplicates) are skipped as in roadmap_builder.cpp:190-197; ties by index.
+// K (= k) is a template value, so the running threshold is the k-th entry of
+// a register list with static indices. Node coordinates are staged in LDS in
+// rows zero-padded to DP floats; the padding adds +0 to each sum.
 constexpr int kKnnMax = 16;
+template <int K, int DP>
 __global__ __launch_bounds__(256) void k_knn(const float *__restrict__ nodes, int N, int dim,
-                                             int k, int32_t *__restrict__ out_idx,
+                                             int32_t *__restrict__ out_idx,
                                              float *__restrict__ out_d2) {
-  __shared__ float tile[256 * kEdgeDim];
+  __shared__ __attribute__((aligned(16))) float tile[256 * DP];
   const int i = blockIdx.x * 256 + threadIdx.x;
   const bool valid = i < N;
-  float qv[kEdgeDim];
+  float qv[DP];
 #pragma unroll
-  for (int d = 0; d < kEdgeDim; ++d) qv[d] = (valid && d < dim) ? nodes[(size_t)i * dim + d] : 0.f;
-  float bd[kKnnMax];
-  int bi[kKnnMax];
+  for (int d = 0; d < DP; ++d) qv[d] = (valid && d < dim) ? nodes[(size_t)i * dim + d] : 0.f;
+  float bd[K];
+  int bi[K];
 #pragma unroll
-  for (int s = 0; s < kKnnMax; ++s) {
+  for (int s = 0; s < K; ++s) {
     bd[s] = INFINITY;
     bi[s] = 0x7fffffff;
   }
   for (int t0 = 0; t0 < N; t0 += 256) {
     const int cnt = min(256, N - t0);
     __syncthreads();
-    for (int e = threadIdx.x; e < cnt * dim; e += 256) tile[e] = nodes[(size_t)t0 * dim + e];
+    for (int e = threadIdx.x; e < 256 * DP; e += 256) {
+      const int j = e / DP, d = e - j * DP;
+      tile[e] = (j < cnt && d < dim) ? nodes[(size_t)(t0 + j) * dim + d] : 0.f;
+    }
     __syncthreads();
     for (int j = 0; j < cnt; ++j) {
       float s = 0.f;
 #pragma unroll
-      for (int d = 0; d < kEdgeDim; ++d)
-        if (d < dim) {
-          const float e = __fsub_rn(tile[j * dim + d], qv[d]);
-          s = __fadd_rn(s, __fmul_rn(e, e));
-        }
+      for (int d = 0; d < DP; ++d) {
+        const float e = __fsub_rn(tile[j * DP + d], qv[d]);
+        s = __fadd_rn(s, __fmul_rn(e, e));
+      }
       const int jj = t0 + j;
-      const float kth = bd[kKnnMax - 1];
-      if (s > 1.1920928955078125e-07f && (s < kth || (s == kth && jj < bi[kKnnMax - 1]))) {
+      if (s > 1.1920928955078125e-07f && (s < bd[K - 1] || (s == bd[K - 1] && jj < bi[K - 1]))) {
         // insertion into the sorted list (static indices)
         float cd = s;
         int ci = jj;
 #pragma unroll
-        for (int p = 0; p < kKnnMax; ++p) {
+        for (int p = 0; p < K; ++p) {
           const bool lt = cd < bd[p] || (cd == bd[p] && ci < bi[p]);
           const float td = bd[p];
           const int ti = bi[p];
@@ -204,11 +211,25 @@ __global__ __launch_bounds__(256) void k_knn(const float *__restrict__ nodes, in
   }
   if (!valid) return;
 #pragma unroll
-  for (int s = 0; s < kKnnMax; ++s)
-    if (s < k) {
-      out_idx[(size_t)i * k + s] = bi[s] == 0x7fffffff ? -1 : bi[s];
-      if (out_d2) out_d2[(size_t)i * k + s] = bd[s];
-    }
+  for (int s = 0; s < K; ++s) {
+    out_idx[(size_t)i * K + s] = bi[s] == 0x7fffffff ? -1 : bi[s];
+    if (out_d2) out_d2[(size_t)i * K + s] = bd[s];
+  }
+}
+
+template <int K>
+void launch_knn(const float *nodes, int N, int dim, int32_t *idx, float *d2, hipStream_t st) {
+  const dim3 g((N + 255) / 256), b(256);
+  if (dim <= 8)
+    hipLaunchKernelGGL((k_knn<K, 8>), g, b, 0, st, nodes, N, dim, idx, d2);
+  else
+    hipLaunchKernelGGL((k_knn<K, 16>), g, b, 0, st, nodes, N, dim, idx, d2);
+}
+
+template <int... Ks>
+void launch_knn_k(int k, const float *nodes, int N, int dim, int32_t *idx, float *d2,
+                  hipStream_t st, std::integer_sequence<int, Ks...>) {
+  ((k == Ks + 1 ? launch_knn<Ks + 1>(nodes, N, dim, idx, d2, st) : void()), ...);
 }
 
 namespace {
@@ -289,8 +310,8 @@ int csd_knn(const float *nodes, int32_t num_nodes, int32_t dim, int32_t k, int32
   if (num_nodes < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative size");
   if (num_nodes == 0) return CSD_OK;
   if (!nodes || !out_idx) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
-  hipLaunchKernelGGL(k_knn, dim3((num_nodes + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                     nodes, num_nodes, dim, k, out_idx, out_d2);
+  launch_knn_k(k, nodes, num_nodes, dim, out_idx, out_d2, (hipStream_t)stream,
+               std::make_integer_sequence<int, kKnnMax>{});
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

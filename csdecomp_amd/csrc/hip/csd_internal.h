@@ -111,6 +111,9 @@ struct csd_plant {
   std::vector<int32_t> geom_type, geom_link, geom_body, geom_slot;
   int32_t num_configuration_variables = 0;
   int32_t num_links = 0;
+  // waves sharing one configuration in the checker, tuned per batch-size
+  // class (index: bit length of n); 0 = not tuned
+  int8_t check_split[64] = {0};
 };
 
 namespace csd {

@@ -118,6 +118,16 @@ int csd_check_collision_free(const csd_plant *plant, const float *q, int64_t n,
                              int64_t stride_n, int64_t stride_j, uint8_t *out,
                              void *stream);
 
+/* Autotuner of csd_check_collision_free (no counterpart in the reference):
+ * times the kernel on the caller's batch with 1, 2 and 4 waves sharing each
+ * group of 64 configurations (each wave taking part of the pair tests; small
+ * batches otherwise leave SIMDs idle) and keeps the fastest for this plant
+ * and batch-size class (same bit length of n). out receives the flags.
+ * Synchronizes `stream`; *split (may be NULL) reports the choice. */
+int csd_tune_check(const csd_plant *plant, const float *q, int64_t n,
+                   int64_t stride_n, int64_t stride_j, uint8_t *out, void *stream,
+                   int32_t *split);
+
 /* Replaces executeCollisionFreeVoxelsKernel (cuda_collision_checker.cu:255-307)
  * / checkCollisionFreeVoxelsCuda (:377-435). voxels (device) [V][3] centres,
  * robot_geoms (HOST array, at most 128 entries) [R] geometry indices checked

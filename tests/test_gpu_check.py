@@ -238,3 +238,32 @@ def test_empty_and_errors(gpu, csd):
     assert e.value.status == 1
     with pytest.raises(capi.CsdError):
         capi.check_collision_free(dp, out, -1, 1, 4, out)
+
+
+def test_split_checker_and_tuner(gpu, orc):
+    """Batches that share each 64-configuration group between 2 or 4 waves
+    (chosen by size or by the autotuner) give the oracle's flags."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    from conftest import assert_flags_match
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    dp = capi.DevicePlant(mp)
+    op = orc.Plant(mp)
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    for n in (140000, 30000):  # 2 waves per group, then 4, by the size rule
+        q = scenes.uniform_configs(n, n, lo, hi)
+        dq = torch.from_numpy(np.ascontiguousarray(q)).to(gpu)
+        out = torch.empty(n, dtype=torch.uint8, device=gpu)
+        capi.check_collision_free(dp, dq, n, 1, n, out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        ref = orc.check(op, q, orc.RULE_GPU)
+        assert_flags_match(got, ref, orc.min_sd(op, q), f"n={n}")
+        s = capi.tune_check(dp, dq, n, 1, n, out)
+        assert s in (1, 2, 4)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), got)
+        capi.check_collision_free(dp, dq, n, 1, n, out)  # uses the tuned split
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), got)
