@@ -118,7 +118,7 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         inf = csd.CudaEdgeInflator(mp, list(plant.getRobotGeometryIds()), opt, dom, R, 10, nd + face_budget)
         if mode == "guided":
             inf.set_guidance(pilot_samples=4096, pilot_factor=10.0, surrogate=model.address,
-                             keep_particles=64, line_points=4)
+                             keep_particles=256, line_points=4)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         regions, infos = inf.inflateEdges(starts, ends, None, 0.0, budgets)

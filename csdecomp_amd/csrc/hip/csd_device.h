@@ -152,9 +152,13 @@ __device__ __forceinline__ bool hit_sphere_obox(float cx, float cy, float cz, fl
   const float lx = fmaf(o[0], cx, fmaf(o[1], cy, fmaf(o[2], cz, o[9])));
   const float ly = fmaf(o[3], cx, fmaf(o[4], cy, fmaf(o[5], cz, o[10])));
   const float lz = fmaf(o[6], cx, fmaf(o[7], cy, fmaf(o[8], cz, o[11])));
-  const float ex = fmaxf(fabsf(lx) - o[12], 0.f);
-  const float ey = fmaxf(fabsf(ly) - o[13], 0.f);
-  const float ez = fmaxf(fabsf(lz) - o[14], 0.f);
+  // per-axis distance outside the box, clamped to [0, 1]: one v_sub with the
+  // clamp modifier per axis. The decision d2 < r2 is unchanged as long as
+  // r2 < 1 (an axis distance above 1 already exceeds r); the fast path only
+  // takes robot spheres of radius below 1 m.
+  const float ex = fminf(fmaxf(fabsf(lx) - o[12], 0.f), 1.f);
+  const float ey = fminf(fmaxf(fabsf(ly) - o[13], 0.f), 1.f);
+  const float ez = fminf(fmaxf(fabsf(lz) - o[14], 0.f), 1.f);
   const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
   return d2 < r2;
 }

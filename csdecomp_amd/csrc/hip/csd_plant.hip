@@ -393,10 +393,13 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   for (int b = 0; b < NB && fast; ++b)
     if (bodies[b].parent != -1 && bodies[b].parent != b - 1) fast = false;
   std::vector<int> per_body(NB, 0);
+  // moving pair geometry must be spheres of radius below 1 m (the fast
+  // sphere-box test clamps axis distances to [0, 1])
   for (int p = 0; p < NP && fast; ++p)
     for (int s = 0; s < 2; ++s) {
       int g = pairs[2 * p + s];
-      if (gbody[g] >= 0 && geoms[g].type != CSD_SPHERE) fast = false;
+      if (gbody[g] >= 0 && (geoms[g].type != CSD_SPHERE || !(geoms[g].dims[0] < 1.f)))
+        fast = false;
     }
   for (int g = 0; g < G; ++g)
     if (gbody[g] >= 0 && geoms[g].type == CSD_SPHERE) per_body[gbody[g]]++;

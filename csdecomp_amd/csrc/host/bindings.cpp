@@ -458,7 +458,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
                              line_points);
           },
           py::arg("pilot_samples") = 0, py::arg("pilot_factor") = 10.0f,
-          py::arg("surrogate") = 0, py::arg("keep_particles") = 64, py::arg("line_points") = 4,
+          py::arg("surrogate") = 0, py::arg("keep_particles") = 256, py::arg("line_points") = 4,
           "surrogate: the address of a csd_mlp (csdecomp_amd.capi.Mlp.address), 0 for none")
       .def("last_info", &CudaEdgeInflator::lastInfo)
       .def("face_capacity", &CudaEdgeInflator::faceCapacity);

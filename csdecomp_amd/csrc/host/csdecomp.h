@@ -328,7 +328,7 @@ class CudaEdgeInflator {
   // colliding regions skip a round's test, and surrogate ranking of the
   // particles to bisect. surrogate: a csd_mlp* (or null).
   void setGuidance(uint32_t pilot_samples, float pilot_factor, const void *surrogate,
-                   int keep_particles = 64, int line_points = 4);
+                   int keep_particles = 256, int line_points = 4);
   const EizoInfo &lastInfo() const { return last_info_; }
   int faceCapacity() const { return face_capacity_; }
 
