@@ -228,6 +228,83 @@ def roadmap_benchmark(nodes=200_000, k=10, step=0.01):
             "occupancy_configs_per_s": round(nodes / (occ_ms * 1e-3))}
 
 
+def edit_regions_benchmark(regions=2048, per=32, steps=9, voxels=64, reps=20):
+    """EditRegions on the 7-DOF arm with 48 boxes + 16 spheres and `voxels`
+    voxel spheres (radius 0.02 m): `regions` boxes around free segments
+    (0.2-0.8 rad) widened by 0.4 rad, and the first `per` colliding of 256
+    configurations sampled uniformly in each box. One call of
+    csd_edit_regions = projection + `steps` bisection steps per collision,
+    then the faces of every region. Timed with HIP events on the launch
+    stream; each call first restores the regions (3 small device copies)."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    dp = capi.DevicePlant(mp)
+    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
+    ls, le = scenes.free_segments(pl, regions, 4001, (0.2, 0.8))
+    blo = np.maximum(lo[:, None], np.minimum(ls, le) - 0.4).T.copy()  # [R][7]
+    bhi = np.minimum(hi[:, None], np.maximum(ls, le) + 0.4).T.copy()
+    rng = np.random.default_rng(4002)
+    dev = torch.device("cuda:0")
+    vox = torch.from_numpy(rng.uniform(-0.8, 0.8, (voxels, 3)).astype(np.float32)).to(dev)
+    rg = list(pl.getRobotGeometryIds())
+    cand = 256
+    u = rng.uniform(0, 1, (regions, cand, 7)).astype(np.float32)
+    samples = torch.from_numpy(blo[:, None, :] + u * (bhi - blo)[:, None, :]).to(dev).contiguous()
+    flags = torch.empty(regions * cand, dtype=torch.uint8, device=dev)
+    capi.check_collision_free_voxels(dp, samples, regions * cand, 7, 1, vox, voxels, 0.02, rg, flags)
+    col = flags.view(regions, cand) == 0
+    keep = col & (torch.cumsum(col.int(), dim=1) <= per)
+    cols = samples[keep].contiguous()  # region-major
+    seg = torch.nonzero(keep)[:, 0].to(torch.int32).contiguous()
+    offsets = torch.zeros(regions + 1, dtype=torch.int32, device=dev)
+    offsets[1:] = torch.cumsum(keep.sum(dim=1), dim=0).to(torch.int32)
+    C = cols.shape[0]
+    F0, cap = 14, 14 + per
+    A0 = torch.zeros(regions, cap, 7, device=dev)
+    b0 = torch.zeros(regions, cap, device=dev)
+    for i in range(7):  # MakeBox rows, interleaved (-e_i, +e_i)
+        A0[:, 2 * i, i], A0[:, 2 * i + 1, i] = -1.0, 1.0
+        b0[:, 2 * i] = -torch.from_numpy(blo[:, i]).to(dev)
+        b0[:, 2 * i + 1] = torch.from_numpy(bhi[:, i]).to(dev)
+    nf0 = torch.full((regions,), F0, dtype=torch.int32, device=dev)
+    A, b, nf = A0.clone(), b0.clone(), nf0.clone()
+    dls = torch.from_numpy(np.ascontiguousarray(ls.T)).to(dev)
+    dle = torch.from_numpy(np.ascontiguousarray(le.T)).to(dev)
+    scol = torch.empty(C * 7, device=dev)
+    proj = torch.empty(C * 7, device=dev)
+    status = torch.empty(regions, dtype=torch.int32, device=dev)
+    s = torch.cuda.current_stream()
+
+    def run():
+        A.copy_(A0)
+        b.copy_(b0)
+        nf.copy_(nf0)
+        capi.edit_regions(dp, cols, seg, offsets, C, dls, dle, regions, A, b, nf, cap, 0.01, steps,
+                          scol, proj, status, voxels=vox, num_voxels=voxels, radius=0.02,
+                          robot_geoms=rg, stream=s)
+
+    for _ in range(2):
+        run()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        run()
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    st = status.cpu().numpy()
+    if (st < 0).any():
+        raise RuntimeError(f"region editing failed for {(st < 0).sum()} regions")
+    return {"regions": regions, "collisions": C, "bisection_steps": steps, "voxels": voxels,
+            "ms": round(ms, 4), "regions_per_s": round(regions / (ms * 1e-3)),
+            "collisions_per_s": round(C / (ms * 1e-3)),
+            "checks_per_s": round(C * steps / (ms * 1e-3)),
+            "faces_added": int(st.sum()), "faces_max": int(st.max())}
+
+
 def eizo_benchmarks(segments=2048):
     """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
     segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
@@ -261,7 +338,16 @@ def main():
     ap.add_argument("--eizo-segments", type=int, default=2048)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions"],
+                    help="only run this secondary leg and print its JSON (for profiling)")
     args = ap.parse_args()
+    if args.leg:
+        from csdecomp_amd import capi
+        capi.init(0)
+        fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments), "mlp": mlp_benchmark,
+              "roadmap": roadmap_benchmark, "edit_regions": edit_regions_benchmark}[args.leg]
+        print(json.dumps({args.leg: fn()}), flush=True)
+        return
     if args.copy_calibration:
         import torch
         src = torch.ones(args.copy_calibration * (1 << 18), device="cuda:0")  # 4-byte floats
@@ -355,6 +441,7 @@ def main():
         line["eizo"] = eizo_benchmarks(args.eizo_segments)
         line["surrogate_mlp"] = mlp_benchmark()
         line["roadmap"] = roadmap_benchmark()
+        line["edit_regions"] = edit_regions_benchmark()
     print(json.dumps(line), flush=True)
 
 

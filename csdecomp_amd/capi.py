@@ -38,6 +38,8 @@ _SIGS = {
     "csd_hit_and_run": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _I32, _VP, _I64, _I32, _U64, _U32, _VP, _VP]),
     "csd_bisect": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _I32, _I32, _VP, _I32, _F, _VP, _I32, _VP, _VP, _VP]),
     "csd_place_faces": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _VP, _VP, _I32, _VP, _VP, _F, _I32, _VP, _VP]),
+    "csd_edit_regions": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _VP, _VP, _I32, _VP, _VP, _VP, _I32,
+                                   _VP, _I32, _F, _VP, _I32, _F, _I32, _VP, _VP, _VP, _VP]),
     "csd_validate_edges": (C.c_int, [_VP, _VP, _I32, _VP, _I64, _F, _I32, _VP, _VP]),
     "csd_occupancy": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _VP, _VP, _F, _VP, _VP]),
     "csd_knn": (C.c_int, [_VP, _I32, _I32, _I32, _VP, _VP, _VP]),
@@ -202,6 +204,20 @@ def place_faces(A, b, nf, cap, dim, scol, proj, n, ls, le, margin, max_faces, ad
     check(lib().csd_place_faces(_ptr(A), _ptr(b), _ptr(nf), cap, dim, _ptr(scol), _ptr(proj), n,
                                 _ptr(ls), _ptr(le), margin, max_faces, _ptr(added), _stream(stream)),
           "csd_place_faces")
+
+
+def edit_regions(plant, collisions, seg_idx, offsets, num_collisions, ls, le, num_regions, A, b,
+                 nf, cap, margin, steps, scol, proj, status, voxels=None, num_voxels=0,
+                 radius=0.0, robot_geoms=(), stream=None):
+    """csd_edit_regions on device arrays: collisions grouped by region
+    (offsets [R+1]); A [R][cap][dim], b [R][cap], nf [R] edited in place."""
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    check(lib().csd_edit_regions(plant.handle, _ptr(collisions), _ptr(seg_idx), _ptr(offsets),
+                                 num_collisions, _ptr(ls), _ptr(le), num_regions, _ptr(A), _ptr(b),
+                                 _ptr(nf), cap, _ptr(voxels), num_voxels, radius, _ptr(rg), len(rg),
+                                 margin, steps, _ptr(scol), _ptr(proj), _ptr(status),
+                                 _stream(stream)),
+          "csd_edit_regions")
 
 
 def validate_edges(plant, nodes, num_nodes, edges, num_edges, step, rule, out, stream=None):

@@ -166,35 +166,68 @@ struct EizoDev {
   int64_t *collisions;     // [R]
 };
 
-// Projection of particle p onto the segment q1-q2 and `steps` bisection
-// steps between the projection (assumed free) and p (colliding), with FK +
-// collision check inline (cuda_set_builder_utils.cu:25-67, 239-357). sc is
-// the last colliding midpoint (p itself if none collides); returns the
-// projection parameter t. Idle lanes pass idle = true.
-template <int VAR, int DP>
-__device__ __forceinline__ float bisect_particle(const PlantImage &P, const VoxelArgs &V,
-                                                 const RobotGeomList &R, const float (&p)[DP],
-                                                 const float (&q1)[DP], const float (&q2)[DP],
-                                                 int steps, bool idle, float (&sc)[DP],
-                                                 const float *tab) {
+// Projection parameter of p on the segment q1-q2 and the projected point
+// (projectSamplesOntoLineSegmentsKernel, cuda_set_builder_utils.cu:25-58),
+// as separately rounded float operations in the reference's order: every
+// kernel that projects obtains the same bits, and so does a plain C
+// restatement. Padding dimensions (zero in p, q1, q2) add exact zeros.
+template <int DP>
+__device__ __forceinline__ float seg_param(const float (&p)[DP], const float (&q1)[DP],
+                                           const float (&q2)[DP]) {
+#pragma clang fp contract(off)
   float a = 0.f, bb = 0.f;
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
-    a = fmaf(p[d] - q1[d], q2[d] - q1[d], a);
-    bb = fmaf(q2[d] - q1[d], q2[d] - q1[d], bb);
+    a = a + (p[d] - q1[d]) * (q2[d] - q1[d]);
+    bb = bb + (q2[d] - q1[d]) * (q2[d] - q1[d]);
   }
-  const float t = fminf(fmaxf(a / bb, 0.f), 1.f);
+  return fminf(fmaxf(a / bb, 0.f), 1.f);
+}
+
+__device__ __forceinline__ float seg_point(float t, float q1, float q2) {
+#pragma clang fp contract(off)
+  return (1.f - t) * q1 + t * q2;
+}
+
+// Projection of particle p onto the segment q1-q2 (written to pr) and
+// `steps` bisection steps between the projection (assumed free) and p
+// (colliding), with FK + collision check inline (cuda_set_builder_utils.cu
+// :25-67, 239-357). sc is the last colliding midpoint (p itself if none
+// collides). Idle lanes pass idle = true.
+// With S > 1, S consecutive waves of the workgroup hold the same 64
+// particles; wave `sub` of the group checks its share of the pairs and the
+// verdicts are OR-ed through `part` (one byte per thread) at every step, so
+// all S waves follow the same bisection. Every thread of the workgroup must
+// call this with the same `steps`.
+template <int VAR, int DP, int S = 1>
+__device__ __forceinline__ void bisect_particle(const PlantImage &P, const VoxelArgs &V,
+                                                const RobotGeomList &R, const float (&p)[DP],
+                                                const float (&q1)[DP], const float (&q2)[DP],
+                                                int steps, bool idle, float (&sc)[DP],
+                                                float (&pr)[DP], const float *tab, int sub = 0,
+                                                uint8_t *part = nullptr) {
+  const float t = seg_param<DP>(p, q1, q2);
   float lo[DP], hi[DP], mid[DP];
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
-    lo[d] = (1.f - t) * q1[d] + t * q2[d];  // projection
+    pr[d] = seg_point(t, q1[d], q2[d]);
+    lo[d] = pr[d];
     hi[d] = p[d];
     sc[d] = p[d];
   }
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
     for (int d = 0; d < DP; ++d) mid[d] = 0.5f * (hi[d] + lo[d]);
-    const bool col = collides_variant<VAR>(P, QReg<DP>{mid}, V, R, idle, tab);
+    bool col = collides_variant<VAR>(P, QReg<DP>{mid}, V, R, idle, tab, sub, S);
+    if constexpr (S > 1) {
+      part[threadIdx.x] = col ? 1 : 0;
+      __syncthreads();
+      const int base = ((int)(threadIdx.x >> 6) / S) * S * 64 + (int)(threadIdx.x & 63);
+      col = false;
+#pragma unroll
+      for (int k = 0; k < S; ++k) col |= part[base + k * 64] != 0;
+      __syncthreads();
+    }
 #pragma unroll
     for (int d = 0; d < DP; ++d) {
       hi[d] = col ? mid[d] : hi[d];
@@ -202,7 +235,6 @@ __device__ __forceinline__ float bisect_particle(const PlantImage &P, const Voxe
       lo[d] = col ? lo[d] : mid[d];
     }
   }
-  return t;
 }
 
 // Stage 4: greedy face placement in FP64 (maybePlaceFace +

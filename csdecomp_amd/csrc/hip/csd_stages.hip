@@ -38,32 +38,38 @@ __global__ __launch_bounds__(kSampleBlock) void k_hit_and_run(
     if (d < dim) out[((size_t)p * n + i) * dim + d] = x[d];
 }
 
-template <int VAR, int DP>
+// DPS = 8 * DP + S: DP padded dimensions, S waves per 64 particles
+template <int VAR, int DPS>
 __global__ __launch_bounds__(kSampleBlock) void k_bisect_standalone(
     PlantImage P, VoxelArgs V, RobotGeomList R, const float *__restrict__ part,
     const float *__restrict__ ls, const float *__restrict__ le,
     const uint32_t *__restrict__ seg, int n, int dim, int steps, float *__restrict__ scol,
     float *__restrict__ proj) {
+  constexpr int DP = DPS / 8, S = DPS % 8;
+  constexpr int kPer = kSampleBlock / S;
   extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  __shared__ uint8_t verdict[kSampleBlock];
   const float *tab = stage_tables<VAR>(P, csd_smem);
-  const int i = blockIdx.x * kSampleBlock + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = wave % S, grp = wave / S;
+  const int i = blockIdx.x * kPer + grp * 64 + lane;
   const bool valid = i < n;
   const int ii = valid ? i : n - 1;
   const uint32_t s_idx = seg[ii];
-  float p[DP], q1[DP], q2[DP], sc[DP];
+  float p[DP], q1[DP], q2[DP], sc[DP], pr[DP];
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
     p[d] = d < dim ? part[(size_t)ii * dim + d] : 0.f;
     q1[d] = d < dim ? ls[(size_t)s_idx * dim + d] : 0.f;
     q2[d] = d < dim ? le[(size_t)s_idx * dim + d] : 0.f;
   }
-  const float t = bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc, tab);
-  if (!valid) return;
+  bisect_particle<VAR, DP, S>(P, V, R, p, q1, q2, steps, !valid, sc, pr, tab, sub, verdict);
+  if (!valid || sub != 0) return;
 #pragma unroll
   for (int d = 0; d < DP; ++d)
     if (d < dim) {
       scol[(size_t)i * dim + d] = sc[d];
-      proj[(size_t)i * dim + d] = (1.f - t) * q1[d] + t * q2[d];
+      proj[(size_t)i * dim + d] = pr[d];
     }
 }
 
@@ -77,6 +83,191 @@ __global__ void k_face_prep(const float *__restrict__ scol, const float *__restr
     s += e * e;
   }
   dist[i] = sqrt(s);
+}
+
+// Region editing (planning/cuda_edit_regions.cu:236-293), one workgroup per
+// region, FP64 arithmetic on the stored FP32 faces. The region's collisions
+// are [off[r], off[r+1]):
+//  1. keep those whose optimised point is strictly inside the region, keyed by
+//     |scol - proj|;
+//  2. repeatedly take the smallest key (lower index on ties), add its face, and
+//     drop every remaining point the new face leaves outside (a.x - b >= 0),
+//     so each point is tested against the region as edited so far.
+constexpr int kEditBlock = 256;  // 4 waves: four regions share a CU (32 KB of keys each)
+constexpr int kEditWaves = kEditBlock / 64;
+constexpr int kEditMax = 4096;  // collisions per region (LDS keys)
+
+__global__ __launch_bounds__(kEditBlock) void k_edit_faces(
+    const float *__restrict__ scol, const float *__restrict__ proj,
+    const int32_t *__restrict__ off, int C, const float *__restrict__ ls,
+    const float *__restrict__ le, float *__restrict__ A, float *__restrict__ b,
+    int32_t *__restrict__ nf, int cap, int dim, float margin, int32_t *__restrict__ status) {
+  __shared__ double s_key[kEditMax];
+  __shared__ double s_v[kEditWaves];
+  __shared__ int s_i[kEditWaves];
+  __shared__ double s_a[kDimMax];
+  __shared__ double s_b;
+  __shared__ int s_stop, s_nf;
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int first = off[r], last = off[r + 1];
+  if (first < 0 || last < first || last > C) {  // offsets outside the batch
+    if (tid == 0) status[r] = -4;
+    return;
+  }
+  const int total = last - first;
+  if (total > kEditMax) {
+    if (tid == 0) status[r] = -1;
+    return;
+  }
+  if (total == 0) {
+    if (tid == 0) status[r] = 0;
+    return;
+  }
+  const int F0 = nf[r];
+  if (F0 < 0 || F0 > cap) {  // a face count the buffers cannot hold
+    if (tid == 0) status[r] = -3;
+    return;
+  }
+  float *Ar = A + (size_t)r * cap * dim;
+  float *br = b + (size_t)r * cap;
+  scol += (size_t)first * dim;
+  proj += (size_t)first * dim;
+  // 1. keys: distance for points strictly inside the region, +inf otherwise
+  for (int k = tid; k < total; k += kEditBlock) {
+    float o[kDimMax];
+    double dd = 0.0;
+#pragma unroll
+    for (int d = 0; d < kDimMax; ++d) {
+      o[d] = d < dim ? scol[(size_t)k * dim + d] : 0.f;
+      const double e = d < dim ? (double)o[d] - (double)proj[(size_t)k * dim + d] : 0.0;
+      dd += e * e;
+    }
+    bool inside = true;
+    for (int f = 0; f < F0 && inside; ++f) {
+      double s = -(double)br[f];
+#pragma unroll
+      for (int d = 0; d < kDimMax; ++d)
+        if (d < dim) s += (double)Ar[(size_t)f * dim + d] * (double)o[d];
+      inside = s < 0.0;
+    }
+    s_key[k] = inside ? sqrt(dd) : INFINITY;
+  }
+  if (tid == 0) s_nf = F0;
+  __syncthreads();
+  // 2. greedy placement
+  int added = 0, full = 0;
+  for (;;) {
+    double v = INFINITY;
+    int vi = 0x7fffffff;
+    for (int k = tid; k < total; k += kEditBlock) argmin_pair(v, vi, s_key[k], k);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(v, o);
+      const int oi = __shfl_xor(vi, o);
+      argmin_pair(v, vi, ov, oi);
+    }
+    if (lane == 0) {
+      s_v[wave] = v;
+      s_i[wave] = vi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double bv = s_v[0];
+      int bi = s_i[0];
+      for (int w = 1; w < kEditWaves; ++w) argmin_pair(bv, bi, s_v[w], s_i[w]);
+      s_stop = 0;
+      if (!(bv < INFINITY)) {
+        s_stop = 1;  // no contained point left
+      } else if (s_nf >= cap) {
+        s_stop = 2;  // face capacity exhausted
+      } else {
+        const float *sc = scol + (size_t)bi * dim, *pr = proj + (size_t)bi * dim;
+        const float *q1 = ls + (size_t)r * dim, *q2 = le + (size_t)r * dim;
+        double a[kDimMax], nrm = 0.0;
+        for (int d = 0; d < dim; ++d) {
+          a[d] = (double)sc[d] - (double)pr[d];
+          nrm += a[d] * a[d];
+        }
+        nrm = sqrt(nrm);
+        const int f = s_nf;
+        double bb = -(double)margin, v1 = 0.0, v2 = 0.0;
+        for (int d = 0; d < dim; ++d) {
+          // a zero direction stays zero (Eigen's normalize leaves it alone)
+          const float af = nrm > 0.0 ? (float)(a[d] / nrm) : 0.f;
+          Ar[(size_t)f * dim + d] = af;
+          s_a[d] = af;
+          bb += (double)af * sc[d];
+          v1 += (double)af * q1[d];
+          v2 += (double)af * q2[d];
+        }
+        const double relax = fmax(v1 - bb, v2 - bb);
+        if (relax > 0.0) bb += relax + 2e-6;  // keep the segment inside the face
+        const float bf = __double2float_ru(bb);
+        br[f] = bf;
+        s_b = bf;
+        s_nf = f + 1;
+        s_key[bi] = INFINITY;
+      }
+    }
+    __syncthreads();
+    if (s_stop) {
+      full = s_stop == 2;
+      break;
+    }
+    ++added;
+    for (int k = tid; k < total; k += kEditBlock) {
+      if (!(s_key[k] < INFINITY)) continue;
+      double s = -s_b;
+      for (int d = 0; d < dim; ++d) s += s_a[d] * (double)scol[(size_t)k * dim + d];
+      if (!(s < 0.0)) s_key[k] = INFINITY;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    nf[r] = s_nf;
+    status[r] = full ? -2 : added;
+  }
+}
+
+// fused projection + bisection for a batch of particles (shared by csd_bisect
+// and csd_edit_regions); arguments validated by the callers
+int launch_bisect(const csd_plant *plant, const float *particles, const float *line_start,
+                  const float *line_end, const uint32_t *segment_idx, int num_particles,
+                  int steps, const float *voxels, int num_voxels, float voxel_radius,
+                  const int32_t *robot_geoms, int num_robot_geoms, float *scol, float *proj,
+                  hipStream_t st) {
+  const int dim = plant->img.dof;
+  RobotGeomList R;
+  VoxelArgs V{voxels, num_voxels, voxel_radius, 0, 0};
+  int rc = CSD_OK;
+  bool all_slots = fill_robot_list(plant, robot_geoms, num_voxels ? num_robot_geoms : 0, R, V, &rc);
+  if (rc) return rc;
+  const int var = check_variant(plant->img, num_voxels > 0 && !all_slots);
+  // waves per 64 particles: small batches spread each particle's pairs over
+  // several waves so that the chip holds enough waves to hide latency
+  const int64_t waves = ((int64_t)num_particles + 63) / 64;
+  const bool generic = var == kGeneric8 || var == kGeneric16;  // no split of its pairs
+  const int split = generic || waves >= 4096 ? 1 : waves >= 2048 ? 2 : 4;
+  const size_t lds = check_lds_bytes(plant->img, var);
+  const dim3 g((unsigned)((num_particles + kSampleBlock / split - 1) / (kSampleBlock / split)));
+  const dim3 blk(kSampleBlock);
+#define CSD_BISECT(DPS)                                                                         \
+  CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, DPS, g, blk, lds, st, plant->img, V, R,      \
+                         particles, line_start, line_end, segment_idx, num_particles, dim, steps, \
+                         scol, proj)
+  if (dim <= 8) {
+    if (split == 4) CSD_BISECT(68);
+    else if (split == 2) CSD_BISECT(66);
+    else CSD_BISECT(65);
+  } else {
+    if (split == 4) CSD_BISECT(132);
+    else if (split == 2) CSD_BISECT(130);
+    else CSD_BISECT(129);
+  }
+#undef CSD_BISECT
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
 }
 }  // namespace csd
 
@@ -129,21 +320,41 @@ int csd_bisect(const csd_plant *plant, const float *particles, const float *line
   if (!particles || !line_start || !line_end || !segment_idx || !scol || !proj ||
       (num_voxels && !voxels))
     return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
-  RobotGeomList R;
-  VoxelArgs V{voxels, num_voxels, voxel_radius, 0, 0};
-  int rc = CSD_OK;
-  bool all_slots = fill_robot_list(plant, robot_geoms, num_voxels ? num_robot_geoms : 0, R, V, &rc);
-  if (rc) return rc;
-  const int var = check_variant(plant->img, num_voxels > 0 && !all_slots);
-  dim3 g((num_particles + kSampleBlock - 1) / kSampleBlock);
-  if (dim <= 8)
-    CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, 8, g, dim3(kSampleBlock),
-                           check_lds_bytes(plant->img, var), (hipStream_t)stream, plant->img, V, R, particles, line_start, line_end,
-                           segment_idx, num_particles, dim, steps, scol, proj);
-  else
-    CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, 16, g, dim3(kSampleBlock),
-                           check_lds_bytes(plant->img, var), (hipStream_t)stream, plant->img, V, R, particles, line_start, line_end,
-                           segment_idx, num_particles, dim, steps, scol, proj);
+  return launch_bisect(plant, particles, line_start, line_end, segment_idx, num_particles, steps,
+                       voxels, num_voxels, voxel_radius, robot_geoms, num_robot_geoms, scol, proj,
+                       (hipStream_t)stream);
+}
+
+int csd_edit_regions(const csd_plant *plant, const float *collisions, const uint32_t *segment_idx,
+                     const int32_t *region_offsets, int32_t num_collisions,
+                     const float *line_start, const float *line_end,
+                     int32_t num_regions, float *A, float *b, int32_t *num_faces,
+                     int32_t capacity, const float *voxels, int32_t num_voxels,
+                     float voxel_radius, const int32_t *robot_geoms, int32_t num_robot_geoms,
+                     float margin, int32_t bisection_steps, float *scol, float *proj,
+                     int32_t *status, void *stream) {
+  if (!plant) return fail(CSD_ERR_INVALID_ARGUMENT, "null plant");
+  const int dim = plant->img.dof;
+  if (dim < 1 || dim > kDimMax) return fail(CSD_ERR_CAPACITY, "region editing supports 1..16 dims");
+  if (plant->img.num_bodies > 16) return fail(CSD_ERR_CAPACITY, "more than 16 moving joints");
+  if (num_collisions < 0 || num_regions < 0 || bisection_steps < 0 || num_voxels < 0 ||
+      capacity < 1)
+    return fail(CSD_ERR_INVALID_ARGUMENT, "bad size");
+  if (num_regions == 0) return CSD_OK;
+  if (!line_start || !line_end || !A || !b || !num_faces || !status || !region_offsets ||
+      (num_collisions && (!collisions || !segment_idx || !scol || !proj)) ||
+      (num_voxels && !voxels))
+    return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (num_collisions) {
+    const int rc = launch_bisect(plant, collisions, line_start, line_end, segment_idx,
+                                 num_collisions, bisection_steps, voxels, num_voxels,
+                                 voxel_radius, robot_geoms, num_robot_geoms, scol, proj, st);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_edit_faces, dim3(num_regions), dim3(kEditBlock), 0, st, scol, proj,
+                     region_offsets, num_collisions, line_start, line_end, A, b, num_faces,
+                     capacity, dim, margin, status);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

@@ -15,7 +15,7 @@ using namespace csdecomp;
 namespace {
 using FArr = py::array_t<float, py::array::f_style | py::array::forcecast>;
 
-MatrixXf to_mat(const py::array &a_in) {
+MatrixXf to_mat(const py::object &a_in) {
   FArr a = FArr::ensure(a_in);
   if (!a) throw std::invalid_argument("expected a float array");
   MatrixXf m;
@@ -36,7 +36,7 @@ py::array_t<float> from_mat(const MatrixXf &m) {
   return a;
 }
 
-std::vector<float> to_vec(const py::array &a_in) {
+std::vector<float> to_vec(const py::object &a_in) {
   py::array_t<float, py::array::c_style | py::array::forcecast> a =
       py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(a_in);
   if (!a) throw std::invalid_argument("expected a float vector");
@@ -49,7 +49,7 @@ py::array_t<float> from_vec(const std::vector<float> &v) {
   return a;
 }
 
-Matrix4f to_m4(const py::array &a_in) {
+Matrix4f to_m4(const py::object &a_in) {
   MatrixXf m = to_mat(a_in);
   if (m.rows != 4 || m.cols != 4) throw std::invalid_argument("expected a 4x4 matrix");
   Matrix4f r;
@@ -63,7 +63,7 @@ py::array_t<float> from_m4(const Matrix4f &x) {
   return from_mat(m);
 }
 
-Vector3f to_v3(const py::array &a) {
+Vector3f to_v3(const py::object &a) {
   std::vector<float> v = to_vec(a);
   if (v.size() != 3) throw std::invalid_argument("expected 3 values");
   return {v[0], v[1], v[2]};
@@ -76,7 +76,7 @@ struct Voxels {
 MatrixXf voxels_of(const py::object &o) {
   if (o.is_none()) return MatrixXf(3, 0);
   if (py::isinstance<Voxels>(o)) return o.cast<const Voxels &>().matrix;
-  return to_mat(o.cast<py::array>());
+  return to_mat(o);
 }
 }  // namespace
 
@@ -96,12 +96,12 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def_readwrite("type", &Joint::type)
       .def_property(
           "X_PL_J", [](const Joint &j) { return from_m4(j.X_PL_J); },
-          [](Joint &j, const py::array &a) { j.X_PL_J = to_m4(a); })
+          [](Joint &j, const py::object &a) { j.X_PL_J = to_m4(a); })
       .def_readwrite("parent_link", &Joint::parent_link)
       .def_readwrite("child_link", &Joint::child_link)
       .def_property(
           "axis", [](const Joint &j) { return from_vec({j.axis[0], j.axis[1], j.axis[2]}); },
-          [](Joint &j, const py::array &a) { j.axis = to_v3(a); })
+          [](Joint &j, const py::object &a) { j.axis = to_v3(a); })
       .def_readwrite("position_lower_limit", &Joint::position_lower_limit)
       .def_readwrite("position_upper_limit", &Joint::position_upper_limit);
 
@@ -130,7 +130,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def("get_position_upper_limits",
            [](const KinematicTree &t) { return from_vec(t.getPositionUpperLimits()); })
       .def("compute_link_frame_to_world_transforms",
-           [](const KinematicTree &t, const py::array &q) {
+           [](const KinematicTree &t, const py::object &q) {
              std::vector<py::array_t<float>> out;
              for (auto &x : t.computeLinkFrameToWorldTransforms(to_vec(q))) out.push_back(from_m4(x));
              return out;
@@ -145,13 +145,13 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def_readwrite("type", &MinimalJoint::type)
       .def_property(
           "X_PL_J", [](const MinimalJoint &j) { return from_m4(j.X_PL_J); },
-          [](MinimalJoint &j, const py::array &a) { j.X_PL_J = to_m4(a); })
+          [](MinimalJoint &j, const py::object &a) { j.X_PL_J = to_m4(a); })
       .def_readwrite("index", &MinimalJoint::index)
       .def_readwrite("parent_link", &MinimalJoint::parent_link)
       .def_readwrite("child_link", &MinimalJoint::child_link)
       .def_property(
           "axis", [](const MinimalJoint &j) { return from_vec({j.axis[0], j.axis[1], j.axis[2]}); },
-          [](MinimalJoint &j, const py::array &a) { j.axis = to_v3(a); });
+          [](MinimalJoint &j, const py::object &a) { j.axis = to_v3(a); });
 
   py::class_<MinimalLink>(m, "MinimalLink")
       .def(py::init<>())
@@ -173,7 +173,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
 
   m.def(
       "computeLinkFrameToWorldTransformsMinimal",
-      [](const py::array &q, const MinimalKinematicTree &tree) {
+      [](const py::object &q, const MinimalKinematicTree &tree) {
         auto tfs = computeLinkFrameToWorldTransformsMinimal(to_vec(q), tree);
         MatrixXf out(4 * tree.num_links, 4);  // link l at rows 4l..4l+3
         for (int l = 0; l < tree.num_links; ++l)
@@ -202,12 +202,12 @@ PYBIND11_MODULE(_pycsdecomp, m) {
           [](const CollisionGeometry &g) {
             return from_vec({g.dimensions[0], g.dimensions[1], g.dimensions[2]});
           },
-          [](CollisionGeometry &g, const py::array &a) { g.dimensions = to_v3(a); })
+          [](CollisionGeometry &g, const py::object &a) { g.dimensions = to_v3(a); })
       .def_readwrite("link_index", &CollisionGeometry::link_index)
       .def_readwrite("group", &CollisionGeometry::group)
       .def_property(
           "X_L_B", [](const CollisionGeometry &g) { return from_m4(g.X_L_B); },
-          [](CollisionGeometry &g, const py::array &a) { g.X_L_B = to_m4(a); });
+          [](CollisionGeometry &g, const py::object &a) { g.X_L_B = to_m4(a); });
 
   py::class_<SceneInspector>(m, "SceneInspector")
       .def(py::init<>())
@@ -237,7 +237,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
 
   py::class_<Voxels>(m, "Voxels")
       .def(py::init<>())
-      .def(py::init([](const py::array &a) {
+      .def(py::init([](const py::object &a) {
         Voxels v;
         v.matrix = to_mat(a);
         if (v.matrix.cols > 0 && v.matrix.rows != 3)
@@ -245,7 +245,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
         return v;
       }))
       .def("set_matrix",
-           [](Voxels &v, const py::array &a) {
+           [](Voxels &v, const py::object &a) {
              v.matrix = to_mat(a);
              if (v.matrix.cols > 0 && v.matrix.rows != 3)
                throw std::invalid_argument("voxels must be a 3 x V matrix");
@@ -267,7 +267,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
              p.num_scene_geometries = (int)g.size();
            })
       .def("set_collision_pairs_from_matrix",
-           [](MinimalPlant &p, const py::array &mat) {
+           [](MinimalPlant &p, const py::object &mat) {
              auto a = py::array_t<int32_t, py::array::forcecast>::ensure(mat);
              if (!a || a.ndim() != 2 || a.shape(0) != 2)
                throw std::runtime_error("Matrix must be shape (2, N)");
@@ -344,24 +344,24 @@ PYBIND11_MODULE(_pycsdecomp, m) {
   py::class_<HPolyhedron>(m, "HPolyhedron")
       .def(py::init<>())
       .def(py::init<const MinimalHPolyhedron &>())
-      .def(py::init([](const py::array &A, const py::array &b) {
+      .def(py::init([](const py::object &A, const py::object &b) {
              return HPolyhedron(to_mat(A), to_vec(b));
            }),
            py::arg("A"), py::arg("b"))
       .def("A", [](const HPolyhedron &h) { return from_mat(h.A()); })
       .def("b", [](const HPolyhedron &h) { return from_vec(h.b()); })
-      .def("MakeBox", [](HPolyhedron &h, const py::array &lo,
-                         const py::array &hi) { h.MakeBox(to_vec(lo), to_vec(hi)); })
+      .def("MakeBox", [](HPolyhedron &h, const py::object &lo,
+                         const py::object &hi) { h.MakeBox(to_vec(lo), to_vec(hi)); })
       .def("GetMyMinimalHPolyhedron", &HPolyhedron::GetMyMinimalHPolyhedron)
       .def(
           "PointInSet",
-          [](const HPolyhedron &h, const py::array &x, float tol) {
+          [](const HPolyhedron &h, const py::object &x, float tol) {
             return h.PointInSet(to_vec(x), tol);
           },
           py::arg("previous_sample"), py::arg("tol") = 1e-9f)
       .def(
           "UniformSample",
-          [](const HPolyhedron &h, const py::array &x, int mixing) {
+          [](const HPolyhedron &h, const py::object &x, int mixing) {
             return from_vec(h.UniformSample(to_vec(x), mixing));
           },
           py::arg("previous_sample"), py::arg("mixing_steps"))
@@ -372,16 +372,16 @@ PYBIND11_MODULE(_pycsdecomp, m) {
   // ---------------------------------------------------------------- checker
   m.def(
       "PairCollisionFree",
-      [](const CollisionGeometry &a, const py::array &xa, const CollisionGeometry &b,
-         const py::array &xb) { return pairCollisionFree(a, to_m4(xa), b, to_m4(xb)); },
+      [](const CollisionGeometry &a, const py::object &xa, const CollisionGeometry &b,
+         const py::object &xb) { return pairCollisionFree(a, to_m4(xa), b, to_m4(xb)); },
       py::arg("geom_a"), py::arg("x_w_la"), py::arg("geom_b"), py::arg("x_w_lb"));
   m.def(
       "CheckCollisionFree",
-      [](const py::array &q, const MinimalPlant &p) { return checkCollisionFree(to_vec(q), p); },
+      [](const py::object &q, const MinimalPlant &p) { return checkCollisionFree(to_vec(q), p); },
       py::arg("configuration"), py::arg("plant"));
   m.def(
       "CheckCollisionFreeVoxels",
-      [](const py::array &q, const py::object &vox, float r, const MinimalPlant &p,
+      [](const py::object &q, const py::object &vox, float r, const MinimalPlant &p,
          const std::vector<int> &ids) {
         for (int id : ids)
           if (id < 0 || id >= p.num_scene_geometries)
@@ -394,7 +394,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       py::arg("robot_geometry_index"));
   m.def(
       "CheckEdgeCollisionFree",
-      [](const py::array &a, const py::array &b, const MinimalPlant &p, float step) {
+      [](const py::object &a, const py::object &b, const MinimalPlant &p, float step) {
         return checkEdgeCollisionFree(to_vec(a), to_vec(b), p, step);
       },
       py::arg("configuration_1"), py::arg("configuration_2"), py::arg("plant"),
@@ -432,7 +432,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
            py::arg("face_capacity") = 0)
       .def(
           "inflateEdge",
-          [](CudaEdgeInflator &self, const py::array &ls, const py::array &le,
+          [](CudaEdgeInflator &self, const py::object &ls, const py::object &le,
              const py::object &vox, float r, bool verbose) {
             return self.inflateEdge(to_vec(ls), to_vec(le), voxels_of(vox), r, verbose);
           },
@@ -440,7 +440,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
           py::arg("verbose") = true)
       .def(
           "inflateEdges",
-          [](CudaEdgeInflator &self, const py::array &ls, const py::array &le,
+          [](CudaEdgeInflator &self, const py::object &ls, const py::object &le,
              const py::object &vox, float r, const std::vector<uint32_t> &budgets) {
             std::vector<EizoInfo> infos;
             auto regions = self.inflateEdges(to_mat(ls), to_mat(le), voxels_of(vox), r, budgets,
@@ -465,7 +465,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
 
   m.def(
       "EizoCuda",
-      [](const py::array &ls, const py::array &le, const py::object &vox, float r,
+      [](const py::object &ls, const py::object &le, const py::object &vox, float r,
          const HPolyhedron &domain, const MinimalPlant &plant, const std::vector<GeometryIndex> &ids,
          const EizoOptions &opt) {
         EizoInfo info;
@@ -478,7 +478,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
 
   m.def(
       "UniformSampleInHPolyhedraCuda",
-      [](const std::vector<HPolyhedron> &polys, const py::array &starts, int n, int mixing,
+      [](const std::vector<HPolyhedron> &polys, const py::object &starts, int n, int mixing,
          uint64_t seed) {
         std::vector<py::array_t<float>> out;
         for (auto &s : UniformSampleInHPolyhedronCuda(polys, to_mat(starts), n, mixing, seed))
@@ -488,9 +488,37 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       py::arg("polyhedra"), py::arg("starting_points"), py::arg("num_samples_per_hpolyhedron"),
       py::arg("mixing_steps"), py::arg("seed") = 1337);
 
+  py::class_<EditRegionsOptions>(m, "EditRegionsOptions")
+      .def(py::init<>())
+      .def_readwrite("configuration_margin", &EditRegionsOptions::configuration_margin)
+      .def_readwrite("bisection_steps", &EditRegionsOptions::bisection_steps)
+      .def_readwrite("verbose", &EditRegionsOptions::verbose);
+
+  m.def(
+      "EditRegionsCuda",
+      [](const py::object &collisions, const std::vector<uint32_t> &line_segment_idxs,
+         const py::object &line_start_points, const py::object &line_end_points,
+         const std::vector<HPolyhedron> &regions, const MinimalPlant &plant,
+         const std::vector<GeometryIndex> &robot_geometry_ids, const py::object &voxels,
+         float voxel_radius, const EditRegionsOptions &options) {
+        auto res = EditRegionsCuda(to_mat(collisions), line_segment_idxs,
+                                   to_mat(line_start_points), to_mat(line_end_points), regions,
+                                   plant, robot_geometry_ids, voxels_of(voxels), voxel_radius,
+                                   options);
+        return py::make_tuple(res.first, py::make_tuple(from_mat(res.second.first),
+                                                        from_mat(res.second.second)));
+      },
+      py::arg("collisions"), py::arg("line_segment_idxs"), py::arg("line_start_points"),
+      py::arg("line_end_points"), py::arg("regions"), py::arg("plant"),
+      py::arg("robot_geometry_ids"), py::arg("voxels"), py::arg("voxel_radius"),
+      py::arg("options"),
+      "Cuts colliding configurations out of regions while keeping each region's line segment "
+      "inside it. Returns (edited regions, (projections, optimized collisions)), the last two "
+      "(dim, C) with the collisions grouped by line segment.");
+
   m.def(
       "CheckCollisionFreeVoxelsCuda",
-      [](const py::array &q, const py::object &vox, float r, const MinimalPlant &p,
+      [](const py::object &q, const py::object &vox, float r, const MinimalPlant &p,
          const std::vector<GeometryIndex> &ids) {
         return checkCollisionFreeVoxelsCuda(to_mat(q), voxels_of(vox), r, p, ids);
       },
@@ -499,12 +527,12 @@ PYBIND11_MODULE(_pycsdecomp, m) {
 
   m.def(
       "CheckCollisionFreeCuda",
-      [](const py::array &q, const MinimalPlant &p) { return checkCollisionFreeCuda(to_mat(q), p); },
+      [](const py::object &q, const MinimalPlant &p) { return checkCollisionFreeCuda(to_mat(q), p); },
       py::arg("configurations"), py::arg("plant"));
 
   m.def(
       "CheckCollisionFreeVoxelsWithoutSelfCollisionCuda",
-      [](const py::array &q, const py::object &vox, float r, const MinimalPlant &p,
+      [](const py::object &q, const py::object &vox, float r, const MinimalPlant &p,
          const std::vector<GeometryIndex> &ids) {
         MatrixXf qm = to_mat(q), vm = voxels_of(vox);
         std::vector<uint8_t> f = checkCollisionFreeVoxelsWithoutSelfCollisionCuda(qm, vm, r, p, ids);
@@ -517,14 +545,14 @@ PYBIND11_MODULE(_pycsdecomp, m) {
 
   m.def(
       "computeForwardKinematicsCuda",
-      [](const py::array &q, const MinimalKinematicTree &t) {
+      [](const py::object &q, const MinimalKinematicTree &t) {
         return from_mat(computeForwardKinematicsCuda(to_mat(q), t));
       },
       py::arg("configurations"), py::arg("tree"));
 
   m.def(
       "_visibility_graph_edges",
-      [](const py::array &q, const MinimalPlant &p, float step, int64_t batch) {
+      [](const py::object &q, const MinimalPlant &p, float step, int64_t batch) {
         auto e = VisibilityGraph(to_mat(q), p, step, batch);
         py::array_t<int32_t> rows((py::ssize_t)e.size()), cols((py::ssize_t)e.size());
         for (size_t k = 0; k < e.size(); ++k) {

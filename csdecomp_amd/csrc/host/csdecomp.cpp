@@ -1056,4 +1056,153 @@ MinimalHPolyhedron EizoCuda(const std::vector<float> &line_start, const std::vec
   return P.GetMyMinimalHPolyhedron();
 }
 
+// ---------------------------------------------------------------------------
+// region editing
+// ---------------------------------------------------------------------------
+std::pair<std::vector<HPolyhedron>, std::pair<MatrixXf, MatrixXf>> EditRegionsCuda(
+    const MatrixXf &collisions, const std::vector<uint32_t> &line_segment_idxs,
+    const MatrixXf &line_start_points, const MatrixXf &line_end_points,
+    const std::vector<HPolyhedron> &regions, const MinimalPlant &plant,
+    const std::vector<GeometryIndex> &robot_geometry_ids, const MatrixXf &voxels,
+    float voxel_radius, const EditRegionsOptions &options) {
+  // argument checks of cuda_edit_regions.cu:55-86
+  require(!regions.empty(), "Regions vector cannot be empty");
+  const int R = (int)regions.size();
+  require(line_start_points.cols == R && line_end_points.cols == R,
+          "Number of line segments must match number of regions");
+  require(line_start_points.rows == line_end_points.rows,
+          "Line start and end points must have same dimension");
+  const int dim = regions[0].ambient_dimension();
+  for (const auto &r : regions)
+    require(r.ambient_dimension() == dim, "all regions must have the same dimension");
+  require(collisions.rows == dim, "collisions must live in the same space as the regions");
+  require(line_start_points.rows == dim,
+          "the linesegments must live in the same dimension as the regions");
+  require(dim == plant.kin_tree.num_configuration_variables,
+          "the regions must live in the plant's configuration space");
+  require(!robot_geometry_ids.empty(), "Robot geometry IDs cannot be empty");
+  for (auto g : robot_geometry_ids)
+    require(g >= 0 && g < plant.num_scene_geometries, "invalid robot geometry id");
+  require(voxels.cols == 0 || voxels.rows == 3, "voxels must be a 3 x V matrix");
+  require(voxel_radius >= 0.f, "Voxel radius must not be negative");
+  auto column = [dim](const MatrixXf &m, int c) {
+    return std::vector<float>(m.data.begin() + (size_t)c * dim,
+                              m.data.begin() + (size_t)(c + 1) * dim);
+  };
+  for (int r = 0; r < R; ++r) {
+    if (!regions[r].PointInSet(column(line_start_points, r)))
+      throw std::runtime_error("line start is not contained in region " + std::to_string(r));
+    if (!regions[r].PointInSet(column(line_end_points, r)))
+      throw std::runtime_error("line end is not contained in region " + std::to_string(r));
+  }
+  // group the collisions by segment, keeping their order (cuda_edit_regions.cu:88-113)
+  const int C = collisions.cols;
+  if ((int)line_segment_idxs.size() < C)
+    throw std::out_of_range("[EditRegionsCuda] fewer line segment indices than collisions");
+  std::vector<uint32_t> count(R, 0), start(R, 0);
+  for (int c = 0; c < C; ++c) {
+    if (line_segment_idxs[c] >= (uint32_t)R)
+      throw std::out_of_range("[EditRegionsCuda] line segment index " +
+                              std::to_string(line_segment_idxs[c]) + " has no region");
+    ++count[line_segment_idxs[c]];
+  }
+  for (int r = 1; r < R; ++r) start[r] = start[r - 1] + count[r - 1];
+  if (options.verbose) printf("[EditRegionsCuda] Num collisions to be optimized %d\n", C);
+  if (C == 0)
+    throw std::runtime_error(
+        "[EditRegionsCuda] No collisions are contained in the regions, this is not supported.\n");
+  for (int r = 0; r < R; ++r)
+    if (count[r] > 4096)
+      throw std::invalid_argument("[EditRegionsCuda] region " + std::to_string(r) + " has " +
+                                  std::to_string(count[r]) +
+                                  " collisions; at most 4096 per region are supported");
+  MatrixXf grouped(dim, C);
+  std::vector<uint32_t> seg(C);
+  {
+    std::vector<uint32_t> next(start);
+    for (int c = 0; c < C; ++c) {
+      const uint32_t s = line_segment_idxs[c], k = next[s]++;
+      std::memcpy(&grouped.data[(size_t)k * dim], &collisions.data[(size_t)c * dim],
+                  sizeof(float) * dim);
+      seg[k] = s;
+    }
+  }
+  // regions packed [R][cap][dim] with room for one face per collision
+  int cap = 1;
+  for (int r = 0; r < R; ++r) cap = std::max(cap, regions[r].A().rows + (int)count[r]);
+  std::vector<float> A((size_t)R * cap * dim, 0.f), b((size_t)R * cap, 0.f);
+  std::vector<int32_t> nf(R);
+  for (int r = 0; r < R; ++r) {
+    const MatrixXf &Ar = regions[r].A();
+    nf[r] = Ar.rows;
+    for (int f = 0; f < Ar.rows; ++f) {
+      for (int d = 0; d < dim; ++d) A[((size_t)r * cap + f) * dim + d] = Ar(f, d);
+      b[(size_t)r * cap + f] = regions[r].b()[f];
+    }
+  }
+  csd_plant *p = plant.device();
+  std::vector<int32_t> offsets(R + 1);
+  for (int r = 0; r < R; ++r) offsets[r] = (int32_t)start[r];
+  offsets[R] = C;
+  DevBuf d_col, d_seg, d_off, d_ls, d_le, d_A, d_b, d_nf, d_vox, d_sc, d_pr, d_st;
+  auto up = [](DevBuf &buf, const void *src, size_t bytes) {
+    void *dst = buf.get(std::max<size_t>(bytes, 4));
+    if (bytes) check_status(csd_memcpy_h2d(dst, src, bytes, nullptr), "upload");
+    return dst;
+  };
+  const size_t cbytes = sizeof(float) * (size_t)C * dim;
+  up(d_col, grouped.data.data(), cbytes);
+  up(d_seg, seg.data(), sizeof(uint32_t) * C);
+  up(d_off, offsets.data(), sizeof(int32_t) * (R + 1));
+  up(d_ls, line_start_points.data.data(), sizeof(float) * (size_t)R * dim);
+  up(d_le, line_end_points.data.data(), sizeof(float) * (size_t)R * dim);
+  up(d_A, A.data(), sizeof(float) * A.size());
+  up(d_b, b.data(), sizeof(float) * b.size());
+  up(d_nf, nf.data(), sizeof(int32_t) * R);
+  if (voxels.cols) up(d_vox, voxels.data.data(), sizeof(float) * voxels.data.size());
+  d_sc.get(cbytes);
+  d_pr.get(cbytes);
+  d_st.get(sizeof(int32_t) * R);
+  check_status(csd_edit_regions(p, (const float *)d_col.ptr, (const uint32_t *)d_seg.ptr,
+                                (const int32_t *)d_off.ptr, C,
+                                (const float *)d_ls.ptr, (const float *)d_le.ptr, R,
+                                (float *)d_A.ptr, (float *)d_b.ptr, (int32_t *)d_nf.ptr, cap,
+                                voxels.cols ? (const float *)d_vox.ptr : nullptr, voxels.cols,
+                                voxel_radius, robot_geometry_ids.data(),
+                                (int)robot_geometry_ids.size(), options.configuration_margin,
+                                (int32_t)options.bisection_steps, (float *)d_sc.ptr,
+                                (float *)d_pr.ptr, (int32_t *)d_st.ptr, nullptr),
+               "region editing");
+  std::vector<int32_t> status(R);
+  MatrixXf projections(dim, C), optimized(dim, C);
+  check_status(csd_memcpy_d2h(A.data(), d_A.ptr, sizeof(float) * A.size(), nullptr), "download");
+  check_status(csd_memcpy_d2h(b.data(), d_b.ptr, sizeof(float) * b.size(), nullptr), "download");
+  check_status(csd_memcpy_d2h(nf.data(), d_nf.ptr, sizeof(int32_t) * R, nullptr), "download");
+  check_status(csd_memcpy_d2h(status.data(), d_st.ptr, sizeof(int32_t) * R, nullptr), "download");
+  check_status(csd_memcpy_d2h(projections.data.data(), d_pr.ptr, cbytes, nullptr), "download");
+  check_status(csd_memcpy_d2h(optimized.data.data(), d_sc.ptr, cbytes, nullptr), "download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  std::vector<HPolyhedron> edited;
+  edited.reserve(R);
+  for (int r = 0; r < R; ++r) {
+    if (status[r] < 0)
+      throw std::runtime_error("[EditRegionsCuda] region " + std::to_string(r) +
+                               " could not be edited (status " + std::to_string(status[r]) + ")");
+    if (count[r] == 0) {
+      if (options.verbose) printf("[EditRegionsCuda] region %d remains unchanged\n", r);
+      edited.push_back(regions[r]);
+      continue;
+    }
+    if (options.verbose) printf("[EditRegionsCuda] region %d added %d faces\n", r, status[r]);
+    MatrixXf Ar(nf[r], dim);
+    std::vector<float> br(nf[r]);
+    for (int f = 0; f < nf[r]; ++f) {
+      for (int d = 0; d < dim; ++d) Ar(f, d) = A[((size_t)r * cap + f) * dim + d];
+      br[f] = b[(size_t)r * cap + f];
+    }
+    edited.emplace_back(Ar, br);
+  }
+  return {std::move(edited), {std::move(projections), std::move(optimized)}};
+}
+
 }  // namespace csdecomp

@@ -348,4 +348,26 @@ MinimalHPolyhedron EizoCuda(const std::vector<float> &line_start, const std::vec
                             const std::vector<GeometryIndex> &robot_geometry_ids,
                             const EizoOptions &options, EizoInfo &eizo_info);
 
+// ---------------------------------------------------------------------------
+// Region editing (planning/cuda_edit_regions.h)
+// ---------------------------------------------------------------------------
+struct EditRegionsOptions {
+  float configuration_margin{0.01f};
+  uint32_t bisection_steps{9};
+  bool verbose{false};
+};
+
+// Cuts trajectory collisions out of regions while keeping each region's
+// segment inside it (planning/cuda_edit_regions.cu:44-300). collisions is
+// (dim, C); line_segment_idxs[c] names the segment / region of collision c.
+// Returns the edited regions and (projections, optimised collisions), both
+// (dim, C) with the collisions grouped by segment in order of appearance, as
+// the reference returns them.
+std::pair<std::vector<HPolyhedron>, std::pair<MatrixXf, MatrixXf>> EditRegionsCuda(
+    const MatrixXf &collisions, const std::vector<uint32_t> &line_segment_idxs,
+    const MatrixXf &line_start_points, const MatrixXf &line_end_points,
+    const std::vector<HPolyhedron> &regions, const MinimalPlant &plant,
+    const std::vector<GeometryIndex> &robot_geometry_ids, const MatrixXf &voxels,
+    float voxel_radius, const EditRegionsOptions &options);
+
 }  // namespace csdecomp

@@ -264,6 +264,45 @@ int csd_place_faces(float *A, float *b, int32_t *num_faces, int32_t capacity,
                     const float *line_end, float margin, int32_t max_faces,
                     int32_t *added, void *stream);
 
+/* ---- region editing ----
+ * Replaces EditRegionsCuda (planning/cuda_edit_regions.cu:44-300; declared
+ * planning/cuda_edit_regions.h:58-67) below its host-side argument checks
+ * and its grouping of the collisions by segment. All arrays are device
+ * arrays:
+ *   collisions [C][dim], segment_idx [C]: the colliding configurations,
+ *     grouped by the segment / region they belong to: those of region r are
+ *     [region_offsets[r], region_offsets[r+1]) and carry segment_idx r (the
+ *     bisection reads segment_idx, the face placement the offsets; their
+ *     agreement is not checked on the device);
+ *   region_offsets [R+1]: non-decreasing, region_offsets[R] <= C;
+ *   line_start, line_end [R][dim]: segment r lies in region r (the caller
+ *     checks that both ends are inside, as the reference does);
+ *   A [R][capacity][dim], b [R][capacity], num_faces [R]: the regions,
+ *     edited in place;
+ *   scol, proj [C][dim] (out): bisected collisions and their projections;
+ *   status [R] (out): faces added, or -1 more than 4096 collisions for the
+ *     region (region untouched), -2 face capacity reached (faces placed
+ *     until then kept), -3 num_faces outside [0, capacity] (untouched),
+ *     -4 offsets outside [0, C] or decreasing (untouched).
+ * Per collision: projection onto its segment and bisection_steps bisection
+ * steps with the full check (plant pairs and, with voxels, robot geometries
+ * against voxel spheres). Per region, in FP64 on the stored FP32 faces: its
+ * collisions whose bisected point is strictly inside the region, in ascending
+ * |scol - proj| (lower index on ties), each add the face a = (scol -
+ * proj)/|scol - proj|, b = a.scol - margin, raised by max(a.ls - b, a.le - b)
+ * + 2e-6 when that is positive, if the point is still strictly inside the
+ * region edited so far. b is stored rounded up. robot_geoms is a host
+ * array. Asynchronous on stream. */
+int csd_edit_regions(const csd_plant *plant, const float *collisions,
+                     const uint32_t *segment_idx, const int32_t *region_offsets,
+                     int32_t num_collisions, const float *line_start,
+                     const float *line_end, int32_t num_regions, float *A,
+                     float *b, int32_t *num_faces, int32_t capacity,
+                     const float *voxels, int32_t num_voxels, float voxel_radius,
+                     const int32_t *robot_geoms, int32_t num_robot_geoms,
+                     float margin, int32_t bisection_steps, float *scol,
+                     float *proj, int32_t *status, void *stream);
+
 /* ---- edge validation and occupancy ----
  * Configurations along each edge are generated in-kernel, never stored.
  * rule 0 -- checkEdgeCollisionFree (plant/collision_checker.cpp:41-53):

@@ -967,6 +967,79 @@ int32_t orc_place_faces(float *A, float *b, int32_t *num_faces, int32_t cap,
   return added;
 }
 
+/* region editing (cuda_edit_regions.cu:236-293) for one region: the
+ * reference's walk over its collisions sorted by distance, each tested for
+ * strict containment in the region as edited so far (A.x - b < 0 on every
+ * face, evaluated in FP64 on the stored FP32 faces), with the face
+ * a = (scol - proj)/|scol - proj| (left zero for a zero difference),
+ * b = a.scol - margin, raised by max(a.ls - b, a.le - b) + 2e-6 when that is
+ * positive, stored rounded up. Ties in distance keep the lower index. */
+typedef struct {
+  double d;
+  int32_t i;
+} orc_key;
+
+static int key_cmp(const void *x, const void *y) {
+  const orc_key *a = (const orc_key *)x, *b = (const orc_key *)y;
+  if (a->d < b->d) return -1;
+  if (a->d > b->d) return 1;
+  return (a->i > b->i) - (a->i < b->i);
+}
+
+int32_t orc_edit_region(float *A, float *b, int32_t *num_faces, int32_t cap,
+                        int32_t dim, const float *scol, const float *proj,
+                        int32_t u, const float *ls, const float *le,
+                        float margin) {
+  orc_key *keys = (orc_key *)malloc(sizeof(orc_key) * (u > 0 ? u : 1));
+  for (int i = 0; i < u; ++i) {
+    double s = 0;
+    for (int d = 0; d < dim; ++d) {
+      double e = (double)scol[i * dim + d] - (double)proj[i * dim + d];
+      s += e * e;
+    }
+    keys[i].d = sqrt(s);
+    keys[i].i = i;
+  }
+  qsort(keys, (size_t)u, sizeof(orc_key), key_cmp);
+  int32_t added = 0;
+  for (int k = 0; k < u; ++k) {
+    const float *sc = scol + keys[k].i * dim, *pr = proj + keys[k].i * dim;
+    int inside = 1;
+    for (int f = 0; f < *num_faces && inside; ++f) {
+      double v = -(double)b[f];
+      for (int d = 0; d < dim; ++d) v += (double)A[f * dim + d] * (double)sc[d];
+      inside = v < 0.0;
+    }
+    if (!inside) continue;
+    if (*num_faces >= cap) {
+      free(keys);
+      return -1;
+    }
+    double a[ORC_MAX_DIM], nrm = 0;
+    for (int d = 0; d < dim; ++d) {
+      a[d] = (double)sc[d] - (double)pr[d];
+      nrm += a[d] * a[d];
+    }
+    nrm = sqrt(nrm);
+    const int f = *num_faces;
+    double bb = -(double)margin, v1 = 0, v2 = 0;
+    for (int d = 0; d < dim; ++d) {
+      float af = nrm > 0 ? (float)(a[d] / nrm) : 0.f;
+      A[f * dim + d] = af;
+      bb += (double)af * sc[d];
+      v1 += (double)af * ls[d];
+      v2 += (double)af * le[d];
+    }
+    double relax = v1 - bb > v2 - bb ? v1 - bb : v2 - bb;
+    if (relax > 0) bb += relax + 2e-6;
+    b[f] = round_up_f(bb);
+    *num_faces = f + 1;
+    ++added;
+  }
+  free(keys);
+  return added;
+}
+
 /* one EI-ZO region (cuda_edge_inflation_zero_order.cu:109-306), with the
  * build's deterministic choices: colliding samples are taken in sample-index
  * order, the sampler draws round-dependent Philox streams, failed

@@ -116,6 +116,14 @@ int32_t orc_place_faces(float *A, float *b, int32_t *num_faces, int32_t cap,
                         int32_t u, const float *ls, const float *le,
                         float margin, int32_t max_faces);
 
+/* region editing for one region (cuda_edit_regions.cu:236-293): u bisected
+ * collisions scol with projections proj; appends faces to A/b (capacity
+ * cap). Returns faces added, or -1 on capacity overflow. */
+int32_t orc_edit_region(float *A, float *b, int32_t *num_faces, int32_t cap,
+                        int32_t dim, const float *scol, const float *proj,
+                        int32_t u, const float *ls, const float *le,
+                        float margin);
+
 typedef struct {
   float configuration_margin, tau, delta, epsilon;
   uint32_t bisection_steps, num_particles, max_hyperplanes_per_iteration;

@@ -77,6 +77,8 @@ def _load():
                               C.c_float, C.c_void_p, C.c_int32, C.c_int, _f32p]),
         "orc_place_faces": (C.c_int32, [_f32p, _f32p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                         _f32p, _f32p, C.c_int32, _f32p, _f32p, C.c_float, C.c_int32]),
+        "orc_edit_region": (C.c_int32, [_f32p, _f32p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                        _f32p, _f32p, C.c_int32, _f32p, _f32p, C.c_float]),
         "orc_eizo": (C.c_int32, [P, _i32p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, _f32p,
                                  _f32p, C.POINTER(C.c_int32), C.c_int32, _f32p, _f32p,
                                  C.POINTER(EizoOptions), C.c_uint32, C.c_int, C.POINTER(EizoInfo)]),
@@ -280,6 +282,71 @@ def place_faces(A, b, nf, cap, scol, proj, ls, le, margin, max_faces):
                                   len(scol), np.ascontiguousarray(ls, np.float32),
                                   np.ascontiguousarray(le, np.float32), margin, max_faces)
     return added, n.value
+
+
+def bisect_voxels(plant, particle, proj, steps, voxels, radius, robot_geoms, rule=RULE_GPU):
+    """orc_bisect with the full check (plant pairs + robot geometries vs voxels)."""
+    sc = np.zeros(plant.dof, np.float32)
+    v, nv = _vox(voxels)
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    lib().orc_bisect(plant.ref, np.ascontiguousarray(particle, np.float32),
+                     np.ascontiguousarray(proj, np.float32), plant.dof, steps,
+                     v.ctypes.data_as(C.c_void_p) if nv else None, nv, radius,
+                     rg.ctypes.data_as(C.c_void_p) if len(rg) else None, len(rg), rule, sc)
+    return sc
+
+
+def edit_region(A, b, scol, proj, ls, le, margin):
+    """Faces of one region after editing (orc_edit_region): A [F][dim], b [F];
+    scol, proj [u][dim]. Returns (A', b', faces added)."""
+    A = np.asarray(A, np.float32)
+    F, dim = A.shape
+    scol = np.ascontiguousarray(scol, np.float32).reshape(-1, dim)
+    u = scol.shape[0]
+    cap = F + u
+    Ab = np.zeros((max(cap, 1), dim), np.float32)
+    bb = np.zeros(max(cap, 1), np.float32)
+    Ab[:F], bb[:F] = A, np.asarray(b, np.float32)
+    n = C.c_int32(F)
+    added = lib().orc_edit_region(Ab.reshape(-1), bb, C.byref(n), max(cap, 1), dim,
+                                  scol.reshape(-1),
+                                  np.ascontiguousarray(proj, np.float32).reshape(-1), u,
+                                  np.ascontiguousarray(ls, np.float32),
+                                  np.ascontiguousarray(le, np.float32), margin)
+    assert added >= 0
+    return Ab[:n.value].copy(), bb[:n.value].copy(), added
+
+
+def edit_regions(plant, collisions, segment_idxs, line_starts, line_ends, regions, robot_geoms,
+                 voxels, radius, margin=0.01, steps=9, rule=RULE_GPU):
+    """EditRegionsCuda restated (planning/cuda_edit_regions.cu:44-300).
+
+    collisions (dim, C); segment_idxs [C]; line_starts / line_ends (dim, R);
+    regions: list of (A [F][dim], b [F]). Returns (list of (A, b),
+    projections (dim, C), optimised collisions (dim, C)), the collisions
+    grouped by segment in order of appearance."""
+    col = np.asarray(collisions, np.float32)
+    seg = np.asarray(segment_idxs, np.int64)
+    ls = np.asarray(line_starts, np.float32)
+    le = np.asarray(line_ends, np.float32)
+    order = np.argsort(seg, kind="stable")
+    col, seg = col[:, order], seg[order]
+    n = col.shape[1]
+    proj = np.zeros_like(col)
+    scol = np.zeros_like(col)
+    for k in range(n):
+        proj[:, k], _ = project(col[:, k], ls[:, seg[k]], le[:, seg[k]])
+        scol[:, k] = bisect_voxels(plant, col[:, k], proj[:, k], steps, voxels, radius,
+                                   robot_geoms, rule)
+    out = []
+    for r, (A, b) in enumerate(regions):
+        m = seg == r
+        if not m.any():
+            out.append((np.asarray(A, np.float32), np.asarray(b, np.float32)))
+            continue
+        A2, b2, _ = edit_region(A, b, scol[:, m].T, proj[:, m].T, ls[:, r], le[:, r], margin)
+        out.append((A2, b2))
+    return out, proj, scol
 
 
 def eizo(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, stream=0, rule=RULE_GPU):
