@@ -430,7 +430,10 @@ def main():
         "config": {"workload": "FK + collision check, 7-DOF arm, 32 spheres x 16 boxes, 784 pairs",
                    "num_configs": n, "num_collision_pairs": mp.num_collision_pairs,
                    "layout": "SoA [7][N] f32 in HBM", "collision_free_fraction": round(free_frac, 4),
-                   "kernel": "k_check<fast 8x4>", "waves_per_64_configs": split},
+                   "kernel": ("k_check<fast 8x4>" if split < 10 else
+                              "k_check2<fast 8x4> (self pairs, then obstacles for survivors)"),
+                   "checker_shape": split,
+                   "waves_per_64_configs": split if split < 10 else split - 10},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_VECTOR_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP32_VECTOR_PEAK_TFLOPS,
                      "traffic": traffic, "flops_per_config": f_cfg, "kernel_ms": kern_ms},

@@ -49,6 +49,76 @@ __global__ __launch_bounds__(kCheckBlock) void k_check(PlantImage P, const float
   }
 }
 
+// Two passes over a tile of kCheckBlock / S1 configurations per workgroup,
+// for batches where most configurations collide with the robot itself:
+//  1. FK + the moving x moving pairs (S1 waves per 64 configurations, parts
+//     OR-ed through LDS); colliding configurations are written at once;
+//  2. the survivors, compacted in LDS, 64 at a time: every wave of the
+//     workgroup takes the same 64 and tests its share of the obstacles and
+//     voxels (o = wave mod 4), verdicts OR-ed through LDS.
+// Pass 2 repeats FK (a few percent of a configuration's work) rather than
+// keeping sphere centres in LDS.
+template <int VAR, int S1>
+__global__ __launch_bounds__(kCheckBlock) void k_check2(PlantImage P, const float *__restrict__ q,
+                                                        int64_t n, int64_t sn, int64_t sj,
+                                                        VoxelArgs V, RobotGeomList R,
+                                                        uint8_t *__restrict__ out) {
+  constexpr int kTile = kCheckBlock / S1;
+  constexpr int kWaves = kCheckBlock / 64;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = wave % S1, grp = wave / S1;
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  __shared__ uint8_t part[kCheckBlock];
+  __shared__ int16_t surv[kTile];
+  __shared__ int s_cnt[kWaves];
+  const float *tab = stage_tables<VAR>(P, csd_smem);
+  // pass 1
+  const int loc = grp * 64 + lane;
+  const int64_t gi = base + loc;
+  const bool valid = gi < n;
+  const int64_t i = valid ? gi : n - 1;
+  bool col = collides_variant<VAR, false, 1>(P, QMem{q, i * sn, sj}, V, R, !valid, tab, sub, S1);
+  if constexpr (S1 > 1) {
+    part[threadIdx.x] = col ? 1 : 0;
+    __syncthreads();
+    col = false;
+#pragma unroll
+    for (int s = 0; s < S1; ++s) col |= part[(grp * S1 + s) * 64 + lane] != 0;
+    __syncthreads();
+  }
+  if (sub == 0 && valid && col) out[gi] = 0;
+  const bool alive = sub == 0 && valid && !col;
+  const uint64_t m = __ballot(alive);
+  if (lane == 0) s_cnt[wave] = __popcll(m);
+  __syncthreads();
+  int off = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    off += w < wave ? s_cnt[w] : 0;
+    total += s_cnt[w];
+  }
+  if (alive) surv[off + __popcll(m & ((1ull << lane) - 1ull))] = (int16_t)loc;
+  __syncthreads();
+  // pass 2 (total is uniform over the workgroup)
+  for (int g = 0; g < total; g += 64) {
+    const int k = g + lane;
+    const bool ok = k < total;
+    const int64_t j = base + surv[ok ? k : total - 1];
+    const bool c =
+        collides_variant<VAR, false, 2>(P, QMem{q, j * sn, sj}, V, R, !ok, tab, wave, kWaves);
+    part[threadIdx.x] = c ? 1 : 0;
+    __syncthreads();
+    if (wave == 0 && ok) {
+      bool cc = false;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) cc |= part[w * 64 + lane] != 0;
+      out[j] = cc ? 0 : 1;
+    }
+    __syncthreads();
+  }
+}
+
 // per (config, voxel): 1 iff no robot geometry touches the voxel
 template <int NB>
 __global__ __launch_bounds__(kCheckBlock) void k_voxel_flags(PlantImage P,
@@ -140,11 +210,35 @@ int check_split(const csd_plant *pl, int64_t n, int var) {
   return waves >= 4096 ? 1 : waves >= 2048 ? 2 : 4;
 }
 
+// split: 1, 2 or 4 -- one pass, that many waves per 64 configurations;
+// 11, 12 or 14 -- two passes (k_check2), pass 1 with 1, 2 or 4 waves per 64
 void launch_k_check(int var, int split, const csd_plant *pl, const float *q, int64_t n,
                     int64_t sn, int64_t sj, const VoxelArgs &V, const RobotGeomList &R,
                     uint8_t *out, hipStream_t st) {
   const size_t lds = check_lds_bytes(pl->img, var);
   const dim3 b(kCheckBlock);
+  if (split > 10 && var < kGeneric8) {
+    const int s1 = split - 10;
+    const int64_t tile = kCheckBlock / s1;
+    const dim3 g((unsigned)((n + tile - 1) / tile));
+#define CSD_CHECK2(V_, S_)                                                                   \
+  hipLaunchKernelGGL((k_check2<V_, S_>), g, b, lds, st, pl->img, q, n, sn, sj, V, R, out)
+#define CSD_CHECK2_S(V_)                 \
+  do {                                   \
+    if (s1 == 4) CSD_CHECK2(V_, 4);      \
+    else if (s1 == 2) CSD_CHECK2(V_, 2); \
+    else CSD_CHECK2(V_, 1);              \
+  } while (0)
+    switch (var) {
+      case kFast8x4: CSD_CHECK2_S(kFast8x4); break;
+      case kFast8x8: CSD_CHECK2_S(kFast8x8); break;
+      case kFast16x4: CSD_CHECK2_S(kFast16x4); break;
+      default: CSD_CHECK2_S(kFast16x8); break;
+    }
+#undef CSD_CHECK2_S
+#undef CSD_CHECK2
+    return;
+  }
   if (split == 4)
     CSD_DISPATCH_VARIANT_X(var, k_check, 4, dim3((unsigned)((n + 63) / 64)), b, lds, st, pl->img,
                            q, n, sn, sj, V, R, out);
@@ -240,7 +334,15 @@ int csd_tune_check(const csd_plant *plant, const float *q, int64_t n, int64_t st
   const VoxelArgs V{nullptr, 0, 0.f, 0, 0};
   const int var = check_variant(plant->img, false);
   int best = 1;
-  if (var < kGeneric8) {
+  const int forced = split ? *split : 0;
+  if (forced != 0) {  // record a given shape instead of timing
+    if (!(forced == 1 || forced == 2 || forced == 4 || forced == 11 || forced == 12 ||
+          forced == 14))
+      return fail(CSD_ERR_INVALID_ARGUMENT, "split must be 1, 2, 4, 11, 12 or 14");
+    best = var < kGeneric8 ? forced : 1;
+    const_cast<csd_plant *>(plant)->check_split[size_class(n)] = (int8_t)best;
+    launch_k_check(var, best, plant, q, n, stride_n, stride_j, V, R, out, st);
+  } else if (var < kGeneric8) {
     hipEvent_t e0, e1;
     CSD_HIP(hipEventCreate(&e0));
     hipError_t e = hipEventCreate(&e1);
@@ -249,7 +351,7 @@ int csd_tune_check(const csd_plant *plant, const float *q, int64_t n, int64_t st
       return hip_fail(e, "hipEventCreate");
     }
     float best_ms = INFINITY;
-    for (int s : {1, 2, 4}) {
+    for (int s : {1, 2, 4, 11, 12, 14}) {
       launch_k_check(var, s, plant, q, n, stride_n, stride_j, V, R, out, st);  // warm-up
       e = hipEventRecord(e0, st);
       for (int k = 0; k < 5 && e == hipSuccess; ++k)

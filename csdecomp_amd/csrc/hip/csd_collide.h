@@ -88,7 +88,10 @@ __device__ __forceinline__ void fast_spheres(const PlantImage &P, const Q &qa,
 // tests only part `sub` of `nsub` of the work -- self-collision blocks of
 // bodies ba = sub mod nsub, obstacles and voxels o = sub mod nsub -- so that
 // nsub waves can share one configuration (the caller ORs the parts).
-template <int NB, int KM, bool ANY = false, class Q>
+// PH selects the pair classes tested: bit 0 the moving x moving (self)
+// pairs, bit 1 the obstacles and voxels; a two-pass checker runs the cheap
+// self pairs first and the obstacles only for configurations still free.
+template <int NB, int KM, bool ANY = false, int PH = 3, class Q>
 __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
                                               const VoxelArgs &V, bool col, const float *tab,
                                               int sub = 0, int nsub = 1) {
@@ -114,6 +117,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // register index stays static.)
 #pragma unroll
   for (int ba = 0; ba < NB; ++ba) {
+    if constexpr (!(PH & 1)) continue;
     const uint64_t act = P.self_active[ba];
     if (act == 0 || ba % nsub != sub || wave_done<ANY>(col)) continue;
 #pragma unroll
@@ -149,7 +153,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // contact distances obstacle_r2[o][s] (box r^2, sphere/capsule
   // (r + r_o)^2, -1 if the slot is not paired with o) are read once per
   // obstacle with a few wide scalar loads; the slot loop has no branch.
-  if constexpr (kTab) {
+  if constexpr (kTab && (PH & 2)) {
     for (int o = sub; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
@@ -196,7 +200,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // moving sphere x world-fixed obstacle, LDS tables: the record in scalar
   // registers, each body's row of contact distances read from LDS just before
   // its tests (one 16-byte read per 4 slots).
-  if constexpr (kLds) {
+  if constexpr (kLds && (PH & 2)) {
     const float *orow = tab + NB * NB * KM * KM;
     for (int o = sub; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
@@ -241,7 +245,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // r * r (box) and (r + r_o) * (r + r_o) (sphere, capsule); a slot not
   // paired with the obstacle gets -1, which never reports a hit, so there is
   // no per-slot branch. Slots of absent bodies are skipped per body.
-  if (kMask && sub < P.num_obstacles) {
+  if (kMask && (PH & 2) && sub < P.num_obstacles) {
     int tn = P.obstacle_type[sub];
     float ovn[16];
 #pragma unroll
@@ -303,7 +307,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
 
   // robot spheres x voxel spheres (executeCollisionFreeVoxelsKernel); the
   // per-slot contact distances are formed once, -1 for slots not checked
-  if (V.num > 0) {
+  if ((PH & 2) && V.num > 0) {
     float vr2[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -480,17 +484,18 @@ __device__ __forceinline__ const float *stage_tables(const PlantImage &P, float 
 
 // dispatch a device-side call on the variant (VAR is a compile-time value
 // inside the generic lambda-free macro body)
-// (sub, nsub): see collides_fast; the generic checker ignores the split
-// (part 0 does all of the work, other parts report no collision).
-template <int VAR, bool ANY = false, class Q>
+// (sub, nsub), PH: see collides_fast; the generic checker ignores the split
+// (part 0 does all of the work, other parts report no collision) and PH
+// (it always tests every pair).
+template <int VAR, bool ANY = false, int PH = 3, class Q>
 __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &qa,
                                                  const VoxelArgs &V, const RobotGeomList &R,
                                                  bool col, const float *tab, int sub = 0,
                                                  int nsub = 1) {
-  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY>(P, qa, V, col, tab, sub, nsub);
-  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY>(P, qa, V, col, tab, sub, nsub);
-  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY>(P, qa, V, col, tab, sub, nsub);
-  else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY>(P, qa, V, col, tab, sub, nsub);
+  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if (sub != 0) return col;
   else if constexpr (VAR == kGeneric8) return collides_generic<8, ANY>(P, qa, V, R, col);
   else return collides_generic<16, ANY>(P, qa, V, R, col);

@@ -119,11 +119,18 @@ int csd_check_collision_free(const csd_plant *plant, const float *q, int64_t n,
                              void *stream);
 
 /* Autotuner of csd_check_collision_free (no counterpart in the reference):
- * times the kernel on the caller's batch with 1, 2 and 4 waves sharing each
- * group of 64 configurations (each wave taking part of the pair tests; small
- * batches otherwise leave SIMDs idle) and keeps the fastest for this plant
- * and batch-size class (same bit length of n). out receives the flags.
- * Synchronizes `stream`; *split (may be NULL) reports the choice. */
+ * times the kernel on the caller's batch in six shapes and keeps the fastest
+ * for this plant and batch-size class (same bit length of n):
+ *   1, 2, 4    -- one pass, that many waves sharing each group of 64
+ *                 configurations (each wave taking part of the pair tests;
+ *                 small batches otherwise leave SIMDs idle);
+ *   11, 12, 14 -- two passes: FK + robot self pairs with 1, 2 or 4 waves per
+ *                 64 configurations, then obstacles and voxels only for the
+ *                 configurations still free, compacted per workgroup (pays
+ *                 when most configurations collide with the robot itself).
+ * out receives the flags. Synchronizes `stream`. *split (may be NULL)
+ * reports the choice; if it holds one of the six shapes on entry, that shape
+ * is recorded without timing. Generic-path plants always use shape 1. */
 int csd_tune_check(const csd_plant *plant, const float *q, int64_t n,
                    int64_t stride_n, int64_t stride_j, uint8_t *out, void *stream,
                    int32_t *split);

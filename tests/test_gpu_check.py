@@ -261,9 +261,62 @@ def test_split_checker_and_tuner(gpu, orc):
         ref = orc.check(op, q, orc.RULE_GPU)
         assert_flags_match(got, ref, orc.min_sd(op, q), f"n={n}")
         s = capi.tune_check(dp, dq, n, 1, n, out)
-        assert s in (1, 2, 4)
+        assert s in (1, 2, 4, 11, 12, 14)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(out.cpu().numpy(), got)
         capi.check_collision_free(dp, dq, n, 1, n, out)  # uses the tuned split
         torch.cuda.synchronize()
         np.testing.assert_array_equal(out.cpu().numpy(), got)
+
+
+@pytest.mark.parametrize("scene", ["arm16", "arm64", "dual"])
+def test_two_pass_checker(gpu, orc, scene):
+    """Every checker shape -- one pass with 1, 2, 4 waves per 64
+    configurations and two passes (self pairs, then obstacles for the
+    survivors) with 1, 2, 4 -- gives the same flags, equal to the oracle's
+    outside the contact band; also with voxels and a ragged batch."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    from conftest import assert_flags_match
+    pl = {"arm16": lambda: scenes.arm_scene(1000, 16, 0),
+          "arm64": lambda: scenes.arm_scene(1001, 48, 16),
+          "dual": lambda: scenes.dual_arm_scene(1002, 72, 24)}[scene]()
+    mp = pl.getMinimalPlant()
+    dp = capi.DevicePlant(mp)
+    op = orc.Plant(mp)
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    n = 20011
+    q = scenes.uniform_configs(5, n, lo, hi)
+    dq = torch.from_numpy(np.ascontiguousarray(q)).to(gpu)
+    out = torch.empty(n, dtype=torch.uint8, device=gpu)
+    ref = orc.check(op, q, orc.RULE_GPU)
+    sd = orc.min_sd(op, q)
+    first = None
+    for shape in (1, 2, 4, 11, 12, 14):
+        out.fill_(7)
+        assert capi.tune_check(dp, dq, n, 1, n, out, force=shape) == shape
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        assert set(np.unique(got)) <= {0, 1}
+        assert_flags_match(got, ref, sd, f"{scene} shape {shape}")
+        out.fill_(7)
+        capi.check_collision_free(dp, dq, n, 1, n, out)  # the recorded shape
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), got)
+        if first is None:
+            first = got
+        np.testing.assert_array_equal(got, first)
+    # voxels ride in pass 2
+    rng = np.random.default_rng(9)
+    vox = torch.from_numpy(rng.uniform(-0.8, 0.8, (40, 3)).astype(np.float32)).to(gpu)
+    rg = list(pl.getRobotGeometryIds())
+    vref = orc.check_voxels(op, q, vox.cpu().numpy().T, 0.03, rg)
+    vsd = orc.min_sd(op, q, vox.cpu().numpy().T, 0.03, rg)
+    for shape in (1, 14):
+        capi.tune_check(dp, dq, n, 1, n, out, force=shape)
+        out.fill_(7)
+        capi.check_collision_free_voxels(dp, dq, n, 1, n, vox, 40, 0.03, rg, out)
+        torch.cuda.synchronize()
+        assert_flags_match(out.cpu().numpy(), vref, vsd, f"{scene} voxels shape {shape}")
+    with pytest.raises(capi.CsdError):
+        capi.tune_check(dp, dq, n, 1, n, out, force=3)
