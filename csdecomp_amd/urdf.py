@@ -1,5 +1,6 @@
-"""URDF loading into a csdecomp plant (the reference's URDFParser,
-plant/urdf_parser.cpp:78-158 and :209-616, on the standard-library XML reader).
+"""URDF and model-directive loading into a csdecomp plant (the reference's
+URDFParser, plant/urdf_parser.cpp:78-158, :209-616 and :618-778, on the
+standard-library XML reader and PyYAML's safe loader).
 
 Link 0 is the implicit ``world`` link; links and joints are indexed in
 document order. Collision primitives: box (full edge lengths), sphere
@@ -13,6 +14,23 @@ import xml.etree.ElementTree as ET
 import numpy as np
 
 from . import _pycsdecomp as _c
+
+
+def yaml_safe_loader():
+    """PyYAML's SafeLoader (constructs plain data only); the directives'
+    `!Rpy {deg: [...]}` tag reads as its mapping."""
+    import yaml
+
+    class Loader(yaml.SafeLoader):
+        pass
+
+    Loader.add_constructor("!Rpy", lambda loader, node: loader.construct_mapping(node, deep=True))
+    return Loader
+
+
+def yaml_load(text, loader):
+    import yaml
+    return yaml.load(text, Loader=loader)  # loader derives from yaml.SafeLoader
 
 FLT_MAX = np.finfo(np.float32).max
 
@@ -41,6 +59,39 @@ def _origin(elem):
     return X.astype(np.float32)
 
 
+def _rename(elem, prefix):
+    """Prefix every name of a model with `prefix::`, and the parent / child
+    links of its joints (urdf_parser.cpp:732-778). A joint whose parent is
+    `world` is refused: models are attached with add_weld."""
+    name = elem.get("name")
+    if name is not None:
+        elem.set("name", f"{prefix}::{name}")
+    if elem.tag == "joint":
+        parent, child = elem.find("parent"), elem.find("child")
+        if parent is not None and parent.get("link") is not None:
+            if parent.get("link") == "world":
+                raise RuntimeError(
+                    f"[CSDECOMP:URDFParser] Directly specifying the world link as parent in one of "
+                    f"the URDFS (found in {name}) is not supported due to potential ambiguity. To "
+                    f"weld the robot to the world specify this in a directive using the -add_weld "
+                    f"directive.")
+            parent.set("link", f"{prefix}::{parent.get('link')}")
+        if child is not None and child.get("link") is not None:
+            child.set("link", f"{prefix}::{child.get('link')}")
+    for c in elem:
+        _rename(c, prefix)
+
+
+def _num(v):
+    """A number as an iostream with default precision writes it (6
+    significant digits), so combined URDFs match the reference's text."""
+    return f"{float(v):g}"
+
+
+class _DirectiveLoader(yaml_safe_loader()):
+    pass
+
+
 class URDFParser:
     """Python counterpart of csdecomp's URDFParser with the same method names."""
 
@@ -57,7 +108,75 @@ class URDFParser:
 
     # -- packages / directives -------------------------------------------
     def register_package(self, name, path):
+        """Maps package://name/... to path (urdf_parser.cpp:780-815); False
+        for an empty name or a path that does not exist."""
+        if not name or not path or not os.path.exists(path):
+            return False
         self._packages[name] = path
+        return True
+
+    def parse_directives_to_urdf_string(self, directives_file):
+        """One URDF combining the models of a directives file
+        (urdf_parser.cpp:618-701): add_model renames the model's links,
+        joints and geometries to `model::name`; add_weld adds a fixed joint;
+        add_frame adds a link on a fixed joint with the given translation and
+        rotation (Rpy degrees). Returns "" if the file has no directives or a
+        model cannot be read."""
+        with open(directives_file) as f:
+            cfg = yaml_load(f.read(), _DirectiveLoader)
+        if not isinstance(cfg, dict) or not cfg.get("directives"):
+            return ""
+        out = ['<robot name="combined_robot">']
+        for d in cfg["directives"]:
+            if "add_model" in d:
+                m = d["add_model"]
+                try:
+                    root = ET.parse(self._resolve_lenient(m["file"])).getroot()
+                except (OSError, ET.ParseError):
+                    return ""
+                _rename(root, m["name"])
+                out += [ET.tostring(c, encoding="unicode") for c in root]
+            elif "add_weld" in d:
+                w = d["add_weld"]
+                out.append(f'  <joint name="weld_{w["parent"]}_to_{w["child"]}" type="fixed">\n'
+                           f'    <parent link="{w["parent"]}"/>\n'
+                           f'    <child link="{w["child"]}"/>\n  </joint>')
+            elif "add_frame" in d:
+                fr = d["add_frame"]
+                X = fr["X_PF"]
+                base, t = X["base_frame"], X["translation"]
+                rot = [float(v) * math.pi / 180.0 for v in X["rotation"]["deg"]]
+                out.append(f'  <link name="{fr["name"]}"/>')
+                out.append(f'  <joint name="{base}_to_{fr["name"]}" type="fixed">\n'
+                           f'    <parent link="{base}"/>\n    <child link="{fr["name"]}"/>\n'
+                           f'    <origin xyz="{_num(t[0])} {_num(t[1])} {_num(t[2])}" '
+                           f'rpy="{_num(rot[0])} {_num(rot[1])} {_num(rot[2])}"/>\n  </joint>')
+        out.append("</robot>")
+        return "\n".join(out)
+
+    def parse_directives(self, directives_file):
+        """Parses the combined URDF of a directives file (urdf_parser.cpp:703-713)."""
+        text = self.parse_directives_to_urdf_string(directives_file)
+        if not text:
+            return False
+        return self.parse_urdf_string(text)
+
+    def override_kinematic_tree(self, kinematic_tree):
+        """Replaces the parsed kinematic tree (urdf_parser.h:127-129)."""
+        self._tree = kinematic_tree
+
+    def override_scene_collision_geometries(self, geometries):
+        """Replaces the parsed collision geometries (urdf_parser.h:116-122)."""
+        self._geoms = list(geometries)
+
+    def _resolve_lenient(self, uri):
+        """package:// paths of registered packages resolve; anything else is
+        used as written (urdf_parser.cpp:716-729)."""
+        if uri.startswith("package://"):
+            pkg, _, rel = uri[len("package://"):].partition("/")
+            if pkg in self._packages:
+                return os.path.join(self._packages[pkg], rel)
+        return uri
 
     def _resolve(self, uri):
         if uri.startswith("package://"):
