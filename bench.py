@@ -305,6 +305,33 @@ def edit_regions_benchmark(regions=2048, per=32, steps=9, voxels=64, reps=20):
             "faces_added": int(st.sum()), "faces_max": int(st.max())}
 
 
+def distance_field_benchmark(grid=512, sites=20000, reps=10):
+    """Distance field of a grid^3 voxel grid (voxel 1 cm) with `sites`
+    occupied voxels at uniform random positions: one computeEDT (row pass +
+    two column passes) timed with HIP events. Per voxel the passes read the
+    uint8 occupancy once and read and write one int32 each (21 B), plus the
+    column passes' stack reads."""
+    import torch
+    from csdecomp_amd import capi
+    rng = np.random.default_rng(5001)
+    pts = torch.from_numpy(rng.uniform(0, grid * 0.01, (sites, 3)).astype(np.float32)).cuda()
+    f = capi.DistanceField([0.0, 0.0, 0.0], grid, 0.01)
+    s = torch.cuda.current_stream()
+    f.set_occupancy(pts, sites, s)
+    f.compute(s)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        f.compute(s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    voxels = grid ** 3
+    return {"grid": grid, "sites": sites, "ms": round(ms, 3),
+            "voxels_per_s": round(voxels / (ms * 1e-3)),
+            "pass_io_GBps": round(21 * voxels / (ms * 1e-3) / 1e9, 1)}
+
+
 def eizo_benchmarks(segments=2048):
     """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
     segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
@@ -338,14 +365,15 @@ def main():
     ap.add_argument("--eizo-segments", type=int, default=2048)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions"],
+    ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions", "distance_field"],
                     help="only run this secondary leg and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.leg:
         from csdecomp_amd import capi
         capi.init(0)
         fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments), "mlp": mlp_benchmark,
-              "roadmap": roadmap_benchmark, "edit_regions": edit_regions_benchmark}[args.leg]
+              "roadmap": roadmap_benchmark, "edit_regions": edit_regions_benchmark,
+              "distance_field": distance_field_benchmark}[args.leg]
         print(json.dumps({args.leg: fn()}), flush=True)
         return
     if args.copy_calibration:
@@ -445,6 +473,7 @@ def main():
         line["surrogate_mlp"] = mlp_benchmark()
         line["roadmap"] = roadmap_benchmark()
         line["edit_regions"] = edit_regions_benchmark()
+        line["distance_field"] = distance_field_benchmark()
     print(json.dumps(line), flush=True)
 
 

@@ -44,6 +44,13 @@ _SIGS = {
     "csd_occupancy": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _VP, _VP, _F, _VP, _VP]),
     "csd_knn": (C.c_int, [_VP, _I32, _I32, _I32, _VP, _VP, _VP]),
     "csd_stream_synchronize": (C.c_int, [_VP]),
+    "csd_distance_field_create": (C.c_int, [_VP, _I32, _F, C.POINTER(_VP)]),
+    "csd_distance_field_destroy": (C.c_int, [_VP]),
+    "csd_distance_field_reset_occupancy": (C.c_int, [_VP, _VP]),
+    "csd_distance_field_set_occupancy": (C.c_int, [_VP, _VP, _I64, _VP]),
+    "csd_distance_field_compute": (C.c_int, [_VP, _VP]),
+    "csd_distance_field_get_distance": (C.c_int, [_VP, _VP, _I64, _VP, _VP]),
+    "csd_distance_field_squared": (C.c_int, [_VP, C.POINTER(_VP)]),
     "csd_mlp_create": (C.c_int, [_I32, _VP, _VP, _U64, C.POINTER(_VP)]),
     "csd_mlp_destroy": (C.c_int, [_VP]),
     "csd_mlp_num_params": (C.c_int, [_VP, C.POINTER(_I64)]),
@@ -312,3 +319,43 @@ class Mlp:
                 self.handle = None
         except Exception:
             pass
+
+
+class DistanceField:
+    """csd_distance_field_*: exact distance field of a G^3 voxel grid."""
+
+    def __init__(self, origin, grid_size, voxel_size):
+        o = np.ascontiguousarray(origin, np.float32)
+        h = _VP()
+        check(lib().csd_distance_field_create(o.ctypes.data_as(_VP), grid_size, voxel_size,
+                                              C.byref(h)), "csd_distance_field_create")
+        self.handle, self.grid_size = h, grid_size
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.csd_distance_field_destroy(self.handle)
+            self.handle = None
+
+    def reset_occupancy(self, stream=None):
+        check(lib().csd_distance_field_reset_occupancy(self.handle, _stream(stream)),
+              "csd_distance_field_reset_occupancy")
+
+    def set_occupancy(self, points, n, stream=None):
+        check(lib().csd_distance_field_set_occupancy(self.handle, _ptr(points), n,
+                                                     _stream(stream)),
+              "csd_distance_field_set_occupancy")
+
+    def compute(self, stream=None):
+        check(lib().csd_distance_field_compute(self.handle, _stream(stream)),
+              "csd_distance_field_compute")
+
+    def get_distance(self, points, n, out, stream=None):
+        check(lib().csd_distance_field_get_distance(self.handle, _ptr(points), n, _ptr(out),
+                                                    _stream(stream)),
+              "csd_distance_field_get_distance")
+
+    def squared_ptr(self):
+        p = _VP()
+        check(lib().csd_distance_field_squared(self.handle, C.byref(p)),
+              "csd_distance_field_squared")
+        return p.value

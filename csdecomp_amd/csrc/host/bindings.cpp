@@ -488,6 +488,23 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       py::arg("polyhedra"), py::arg("starting_points"), py::arg("num_samples_per_hpolyhedron"),
       py::arg("mixing_steps"), py::arg("seed") = 1337);
 
+  py::class_<DistanceField>(m, "DistanceField")
+      .def(py::init([](const py::object &origin, size_t grid_size, float voxel_size) {
+             return new DistanceField(to_v3(origin), grid_size, voxel_size);
+           }),
+           py::arg("origin"), py::arg("grid_size"), py::arg("voxel_size"))
+      .def("computeEDT", &DistanceField::computeEDT, py::arg("phase1Band") = 1,
+           py::arg("phase2Band") = 1, py::arg("phase3Band") = 2)
+      .def("setOccupancy",
+           [](DistanceField &f, const py::object &P) { f.setOccupancy(to_mat(P)); },
+           py::arg("P_VG"))
+      .def("resetOccupancy", &DistanceField::resetOccupancy)
+      .def("getDistance",
+           [](DistanceField &f, const py::object &P) { return from_vec(f.getDistance(to_mat(P))); },
+           py::arg("P_VG"))
+      .def_readonly("gridSize", &DistanceField::gridSize)
+      .def_readonly("numVoxels", &DistanceField::numVoxels);
+
   py::class_<EditRegionsOptions>(m, "EditRegionsOptions")
       .def(py::init<>())
       .def_readwrite("configuration_margin", &EditRegionsOptions::configuration_margin)

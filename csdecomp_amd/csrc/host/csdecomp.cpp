@@ -1057,6 +1057,53 @@ MinimalHPolyhedron EizoCuda(const std::vector<float> &line_start, const std::vec
 }
 
 // ---------------------------------------------------------------------------
+// distance field
+// ---------------------------------------------------------------------------
+DistanceField::DistanceField(const Vector3f &origin, size_t gridSize, float voxelSize)
+    : gridSize((int)gridSize), numVoxels(gridSize * gridSize * gridSize) {
+  ensure_device();
+  check_status(csd_distance_field_create(origin.data(), (int32_t)gridSize, voxelSize, &field_),
+               "distance field");
+}
+
+DistanceField::~DistanceField() { csd_distance_field_destroy(field_); }
+
+void DistanceField::computeEDT(int, int, int) {
+  check_status(csd_distance_field_compute(field_, nullptr), "distance transform");
+}
+
+void DistanceField::resetOccupancy() {
+  check_status(csd_distance_field_reset_occupancy(field_, nullptr), "reset occupancy");
+}
+
+void DistanceField::setOccupancy(const MatrixXf &P) {
+  require(P.cols == 0 || P.rows == 3, "points must be a 3 x N matrix");
+  if (P.cols == 0) return;
+  DevBuf d;
+  void *dp = d.get(P.data.size() * 4);
+  check_status(csd_memcpy_h2d(dp, P.data.data(), P.data.size() * 4, nullptr), "upload");
+  check_status(csd_distance_field_set_occupancy(field_, (const float *)dp, P.cols, nullptr),
+               "set occupancy");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+}
+
+std::vector<float> DistanceField::getDistance(const MatrixXf &P) {
+  require(P.cols == 0 || P.rows == 3, "points must be a 3 x N matrix");
+  std::vector<float> out(P.cols);
+  if (P.cols == 0) return out;
+  DevBuf dpts, dout;
+  void *dp = dpts.get(P.data.size() * 4);
+  void *dd = dout.get(out.size() * 4);
+  check_status(csd_memcpy_h2d(dp, P.data.data(), P.data.size() * 4, nullptr), "upload");
+  check_status(csd_distance_field_get_distance(field_, (const float *)dp, P.cols, (float *)dd,
+                                               nullptr),
+               "distance lookup");
+  check_status(csd_memcpy_d2h(out.data(), dd, out.size() * 4, nullptr), "download");
+  check_status(csd_stream_synchronize(nullptr), "synchronize");
+  return out;
+}
+
+// ---------------------------------------------------------------------------
 // region editing
 // ---------------------------------------------------------------------------
 std::pair<std::vector<HPolyhedron>, std::pair<MatrixXf, MatrixXf>> EditRegionsCuda(

@@ -349,6 +349,31 @@ MinimalHPolyhedron EizoCuda(const std::vector<float> &line_start, const std::vec
                             const EizoOptions &options, EizoInfo &eizo_info);
 
 // ---------------------------------------------------------------------------
+// Distance field (geometry/cuda_distance_field.h:133-172): exact distance
+// from each voxel of a gridSize^3 grid to the nearest occupied voxel.
+// Points are (3, N) matrices; a point maps to voxel round((p - origin) /
+// voxelSize) per axis.
+// ---------------------------------------------------------------------------
+class DistanceField {
+ public:
+  DistanceField(const Vector3f &origin, size_t gridSize, float voxelSize);
+  ~DistanceField();
+  DistanceField(const DistanceField &) = delete;
+  DistanceField &operator=(const DistanceField &) = delete;
+  // the PBA band parameters of the reference are accepted and unused
+  void computeEDT(int phase1Band = 1, int phase2Band = 1, int phase3Band = 2);
+  void setOccupancy(const MatrixXf &P_VG);
+  void resetOccupancy();
+  // 0 outside the grid, +inf if no voxel is occupied
+  std::vector<float> getDistance(const MatrixXf &P_VG);
+  const int gridSize;
+  const size_t numVoxels;
+
+ private:
+  csd_distance_field *field_ = nullptr;
+};
+
+// ---------------------------------------------------------------------------
 // Region editing (planning/cuda_edit_regions.h)
 // ---------------------------------------------------------------------------
 struct EditRegionsOptions {

@@ -310,6 +310,34 @@ int csd_edit_regions(const csd_plant *plant, const float *collisions,
                      float margin, int32_t bisection_steps, float *scol,
                      float *proj, int32_t *status, void *stream);
 
+/* ---- distance field ----
+ * Replaces DistanceField (geometry/cuda_distance_field.h:133-172): exact
+ * Euclidean distance from each voxel of a G^3 grid (voxel i at origin +
+ * i * voxel_size) to the nearest occupied voxel. A point maps to voxel
+ * round((p - origin) / voxel_size) per axis; points outside the grid are
+ * ignored by set_occupancy, and get_distance returns 0 for them (as the
+ * reference does) and +inf inside a field with no occupied voxel.
+ * grid_size 1..1024 (the reference requires a multiple of 32). Memory: G^3
+ * bytes plus three G^3 int32 grids. Points are device arrays [M][3]. */
+typedef struct csd_distance_field csd_distance_field;
+int csd_distance_field_create(const float *origin /* host [3] */, int32_t grid_size,
+                              float voxel_size, csd_distance_field **out);
+int csd_distance_field_destroy(csd_distance_field *field);
+/* resetOccupancy / setOccupancy (cuda_distance_field.cu:315-330) */
+int csd_distance_field_reset_occupancy(csd_distance_field *field, void *stream);
+int csd_distance_field_set_occupancy(csd_distance_field *field,
+                                     const float *points, int64_t num_points,
+                                     void *stream);
+/* computeEDT (cuda_distance_field.cu:304-313) */
+int csd_distance_field_compute(csd_distance_field *field, void *stream);
+/* getDistance (cuda_distance_field.cu:332-349): out (device) [M] */
+int csd_distance_field_get_distance(const csd_distance_field *field,
+                                    const float *points, int64_t num_points,
+                                    float *out, void *stream);
+/* Device pointer to the squared distances in voxel units, [z][y][x] int32
+ * (0x3fffffff where no voxel is occupied); valid until the next compute. */
+int csd_distance_field_squared(const csd_distance_field *field, const int32_t **d2);
+
 /* ---- edge validation and occupancy ----
  * Configurations along each edge are generated in-kernel, never stored.
  * rule 0 -- checkEdgeCollisionFree (plant/collision_checker.cpp:41-53):

@@ -1040,6 +1040,53 @@ int32_t orc_edit_region(float *A, float *b, int32_t *num_faces, int32_t cap,
   return added;
 }
 
+/* distance field (geometry/cuda_distance_field.h:76-131): occupancy by
+ * voxel round((p - origin) / voxel) per axis; the squared distance in voxels
+ * from every voxel to the nearest occupied one, by definition (a scan over
+ * all occupied voxels), 0x3fffffff when none is occupied. */
+void orc_voxelize(const float *points, int64_t m, const float *origin,
+                  float voxel, int32_t G, uint8_t *occ) {
+  for (int64_t i = 0; i < m; ++i) {
+    int c[3];
+    int inside = 1;
+    for (int k = 0; k < 3; ++k) {
+      c[k] = (int)roundf((points[3 * i + k] - origin[k]) / voxel);
+      inside &= c[k] >= 0 && c[k] < G;
+    }
+    if (inside) occ[((int64_t)c[2] * G + c[1]) * G + c[0]] = 1;
+  }
+}
+
+void orc_edt_brute(const uint8_t *occ, int32_t G, int32_t *d2, int threads) {
+  const int64_t n = (int64_t)G * G * G;
+  int64_t ns = 0;
+  for (int64_t i = 0; i < n; ++i) ns += occ[i] != 0;
+  int32_t *sites = (int32_t *)malloc(sizeof(int32_t) * 3 * (ns > 0 ? ns : 1));
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (occ[i]) {
+      sites[3 * k] = (int32_t)(i % G);
+      sites[3 * k + 1] = (int32_t)((i / G) % G);
+      sites[3 * k + 2] = (int32_t)(i / ((int64_t)G * G));
+      ++k;
+    }
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = (int)(i % G), y = (int)((i / G) % G), z = (int)(i / ((int64_t)G * G));
+    int32_t best = 0x3fffffff;
+    for (int64_t s = 0; s < ns; ++s) {
+      const int dx = x - sites[3 * s], dy = y - sites[3 * s + 1], dz = z - sites[3 * s + 2];
+      const int32_t v = dx * dx + dy * dy + dz * dz;
+      if (v < best) best = v;
+    }
+    d2[i] = best;
+  }
+  free(sites);
+}
+
 /* one EI-ZO region (cuda_edge_inflation_zero_order.cu:109-306), with the
  * build's deterministic choices: colliding samples are taken in sample-index
  * order, the sampler draws round-dependent Philox streams, failed

@@ -124,6 +124,13 @@ int32_t orc_edit_region(float *A, float *b, int32_t *num_faces, int32_t cap,
                         int32_t u, const float *ls, const float *le,
                         float margin);
 
+/* distance field (geometry/cuda_distance_field.*): voxelization of points
+ * [m][3] into a G^3 uint8 grid ([z][y][x]) and the squared distance in
+ * voxels to the nearest occupied voxel by definition (0x3fffffff if none). */
+void orc_voxelize(const float *points, int64_t m, const float *origin,
+                  float voxel, int32_t G, uint8_t *occ);
+void orc_edt_brute(const uint8_t *occ, int32_t G, int32_t *d2, int threads);
+
 typedef struct {
   float configuration_margin, tau, delta, epsilon;
   uint32_t bisection_steps, num_particles, max_hyperplanes_per_iteration;

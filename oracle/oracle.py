@@ -77,6 +77,9 @@ def _load():
                               C.c_float, C.c_void_p, C.c_int32, C.c_int, _f32p]),
         "orc_place_faces": (C.c_int32, [_f32p, _f32p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                         _f32p, _f32p, C.c_int32, _f32p, _f32p, C.c_float, C.c_int32]),
+        "orc_voxelize": (None, [_f32p, C.c_int64, _f32p, C.c_float, C.c_int32, _u8p]),
+        "orc_edt_brute": (None, [_u8p, C.c_int32, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                 C.c_int]),
         "orc_edit_region": (C.c_int32, [_f32p, _f32p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                         _f32p, _f32p, C.c_int32, _f32p, _f32p, C.c_float]),
         "orc_eizo": (C.c_int32, [P, _i32p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, _f32p,
@@ -366,3 +369,25 @@ def eizo(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, stream=0, rule
     if rc != 0:
         raise RuntimeError("oracle EI-ZO face capacity exceeded")
     return A[:nf.value], b[:nf.value], info
+
+
+FAR = 0x3fffffff
+
+
+def voxelize(points, origin, voxel, G):
+    """uint8 occupancy [z][y][x] of points (3, M) (orc_voxelize)."""
+    p = np.ascontiguousarray(np.asarray(points, np.float32).T)
+    occ = np.zeros(G * G * G, np.uint8)
+    lib().orc_voxelize(p.reshape(-1), p.shape[0], np.ascontiguousarray(origin, np.float32),
+                       voxel, G, occ)
+    return occ.reshape(G, G, G)
+
+
+def edt_brute(occ, threads=0):
+    """Squared distance in voxels to the nearest occupied voxel, [z][y][x]
+    int32, FAR where nothing is occupied (orc_edt_brute)."""
+    occ = np.ascontiguousarray(occ, np.uint8)
+    G = occ.shape[0]
+    d2 = np.zeros(occ.size, np.int32)
+    lib().orc_edt_brute(occ.reshape(-1), G, d2, threads)
+    return d2.reshape(occ.shape)
