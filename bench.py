@@ -332,6 +332,46 @@ def distance_field_benchmark(grid=512, sites=20000, reps=10):
             "pass_io_GBps": round(21 * voxels / (ms * 1e-3) / 1e9, 1)}
 
 
+def voxel_maps_benchmark(nodes=20000, edges=2000, step=0.02):
+    """DRM offline maps on the 7-DOF arm (48 boxes + 16 spheres, 32 robot
+    spheres) over a 64^3 grid of 3.125 cm voxels (2 m cube, resolution
+    1/64 m): the node map of `nodes` uniform configurations and the edge map
+    of `edges` edges of length U[0.1, 0.4] rad at `step`, each one
+    csd_voxel_collision_map call (count pass, host scan, fill pass)."""
+    import time
+    import torch
+    from csdecomp_amd import capi, scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    dp = capi.DevicePlant(pl.getMinimalPlant())
+    rg = list(pl.getSceneInspector().robot_geometry_ids)
+    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
+    rng = np.random.default_rng(6001)
+    q = rng.uniform(lo, hi, (nodes, 7)).astype(np.float32)
+    d = rng.normal(size=(edges, 7))
+    d *= rng.uniform(0.1, 0.4, (edges, 1)) / np.linalg.norm(d, axis=1, keepdims=True)
+    q2 = np.clip(q[:edges] + d, lo, hi).astype(np.float32)
+    allq = torch.from_numpy(np.concatenate([q, q2])).cuda()
+    e = torch.from_numpy(np.stack([np.arange(edges), nodes + np.arange(edges)], 1)
+                         .astype(np.int32)).cuda()
+    center, size, res = [0.0, 0.0, 0.5], [2.0, 2.0, 2.0], 1.0 / 64
+    V = 64 ** 3
+    off = torch.empty(V + 1, dtype=torch.int64, device="cuda")
+    res_out = {"grid": [64, 64, 64], "nodes": nodes, "edges": edges, "edge_step": step}
+    for name, ed, ne in (("node_map", None, 0), ("edge_map", e, edges)):
+        args = (dp, allq, nodes + edges, ed, ne, step, rg, center, size, res, off)
+        total = capi.voxel_collision_map(*args)
+        items = torch.empty(max(total, 1), dtype=torch.int32, device="cuda")
+        capi.voxel_collision_map(*args, items=items, capacity=total)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        capi.voxel_collision_map(*args, items=items, capacity=total)
+        dt = time.perf_counter() - t0
+        res_out[name] = {"entries": total, "s": round(dt, 4),
+                         "items_per_s": round((nodes if ed is None else edges) / dt)}
+    return res_out
+
+
 def eizo_benchmarks(segments=2048):
     """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
     segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
@@ -365,7 +405,8 @@ def main():
     ap.add_argument("--eizo-segments", type=int, default=2048)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions", "distance_field"],
+    ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions", "distance_field",
+                                      "voxel_maps"],
                     help="only run this secondary leg and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.leg:
@@ -373,7 +414,8 @@ def main():
         capi.init(0)
         fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments), "mlp": mlp_benchmark,
               "roadmap": roadmap_benchmark, "edit_regions": edit_regions_benchmark,
-              "distance_field": distance_field_benchmark}[args.leg]
+              "distance_field": distance_field_benchmark,
+              "voxel_maps": voxel_maps_benchmark}[args.leg]
         print(json.dumps({args.leg: fn()}), flush=True)
         return
     if args.copy_calibration:
@@ -474,6 +516,7 @@ def main():
         line["roadmap"] = roadmap_benchmark()
         line["edit_regions"] = edit_regions_benchmark()
         line["distance_field"] = distance_field_benchmark()
+        line["voxel_maps"] = voxel_maps_benchmark()
     print(json.dumps(line), flush=True)
 
 

@@ -27,6 +27,8 @@ except ImportError as _e:  # pragma: no cover - exercised only on broken install
 from ._pycsdecomp import *  # noqa: F401,F403  (the reference's binding names)
 from ._pycsdecomp import _visibility_graph_edges
 from .urdf import URDFParser  # noqa: F401
+from .roadmap import (DRM, GetCollisionVoxelCenter, GetCollisionVoxelId,  # noqa: F401
+                      RoadmapBuilder, RoadmapOptions)
 
 LIB_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "lib", "libcsdecomp_hip.so")
 

@@ -43,6 +43,8 @@ _SIGS = {
     "csd_validate_edges": (C.c_int, [_VP, _VP, _I32, _VP, _I64, _F, _I32, _VP, _VP]),
     "csd_occupancy": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _VP, _VP, _F, _VP, _VP]),
     "csd_knn": (C.c_int, [_VP, _I32, _I32, _I32, _VP, _VP, _VP]),
+    "csd_voxel_collision_map": (C.c_int, [_VP, _VP, _I32, _VP, _I64, _F, _VP, _I32, _VP, _VP, _F,
+                                          _VP, _VP, _I64, C.POINTER(_I64), _VP]),
     "csd_stream_synchronize": (C.c_int, [_VP]),
     "csd_distance_field_create": (C.c_int, [_VP, _I32, _F, C.POINTER(_VP)]),
     "csd_distance_field_destroy": (C.c_int, [_VP]),
@@ -240,6 +242,22 @@ def occupancy(plant, q, n, stride_n, stride_j, robot_geoms, center, size, resolu
     check(lib().csd_occupancy(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(rg), len(rg),
                               _ptr(c), _ptr(s), resolution, _ptr(grid), _stream(stream)),
           "csd_occupancy")
+
+
+def voxel_collision_map(plant, nodes, num_nodes, edges, num_edges, step, robot_geoms, center,
+                        size, resolution, offsets, items=None, capacity=0, stream=None):
+    """csd_voxel_collision_map; returns the number of entries (total). With
+    items=None only the offsets are written."""
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    c = np.ascontiguousarray(center, np.float32)
+    sz = np.ascontiguousarray(size, np.float32)
+    total = _I64(0)
+    check(lib().csd_voxel_collision_map(plant.handle, _ptr(nodes), num_nodes, _ptr(edges),
+                                        num_edges, step, _ptr(rg), len(rg), _ptr(c), _ptr(sz),
+                                        resolution, _ptr(offsets), _ptr(items), capacity,
+                                        C.byref(total), _stream(stream)),
+          "csd_voxel_collision_map")
+    return int(total.value)
 
 
 def knn(nodes, num_nodes, dim, k, out_idx, out_d2=None, stream=None):

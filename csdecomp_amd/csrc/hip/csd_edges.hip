@@ -5,6 +5,8 @@
 #include <string.h>
 
 #include <utility>
+#include <vector>
+#include <algorithm>
 
 #include "csd_collide.h"
 
@@ -105,6 +107,40 @@ __device__ __forceinline__ float voxel_coord(float c, float size, int i, float r
   return (float)((double)c - (double)size / 2.0 + (double)(float)(i * 2 * res) + (double)res);
 }
 
+// Voxel index box [lo, hi] that can hold a voxel sphere touched by geometry
+// g posed at X (its bounding box widened by the voxel radius, plus a margin
+// of one voxel for rounding).
+__device__ __forceinline__ void geom_voxel_range(const DevGeom &g, const float (&X)[12],
+                                                 const GridArgs &G, int (&lo)[3], int (&hi)[3]) {
+  const float two_res = 2.f * G.res;
+  const int rk = geom_rank(g.type);
+  const int nn[3] = {G.nx, G.ny, G.nz};
+  for (int k = 0; k < 3; ++k) {
+    float ext;
+    if (rk == 0) ext = g.dims[0];
+    else if (rk == 1)
+      ext = fabsf(X[k * 4 + 0]) * g.dims[0] + fabsf(X[k * 4 + 1]) * g.dims[1] +
+            fabsf(X[k * 4 + 2]) * g.dims[2];
+    else ext = g.dims[0] + fabsf(X[k * 4 + 2]) * g.dims[1];
+    ext = ext * 1.0001f + G.radius + 1e-6f;
+    const float origin = G.center[k] - 0.5f * G.size[k] + G.res;
+    const float p = X[k * 4 + 3];
+    lo[k] = max(0, (int)floorf((p - ext - origin) / two_res) - 1);
+    hi[k] = min(nn[k] - 1, (int)ceilf((p + ext - origin) / two_res) + 1);
+  }
+}
+
+__device__ __forceinline__ bool voxel_hit(const DevGeom &g, const float (&X)[12], const GridArgs &G,
+                                          int ix, int iy, int iz) {
+  float Y[12];
+  aff_identity(Y);
+  Y[3] = voxel_coord(G.center[0], G.size[0], ix, G.res);
+  Y[7] = voxel_coord(G.center[1], G.size[1], iy, G.res);
+  Y[11] = voxel_coord(G.center[2], G.size[2], iz, G.res);
+  const float vd[4] = {G.radius, 0.f, 0.f, 0.f};
+  return hit_posed(0, Y, vd, geom_rank(g.type), X, g.dims);
+}
+
 template <int NB>
 __global__ __launch_bounds__(kEdgeBlock) void k_occupancy(PlantImage P, const float *__restrict__ q,
                                                           int64_t n, int64_t sn, int64_t sj,
@@ -149,6 +185,100 @@ __global__ __launch_bounds__(kEdgeBlock) void k_occupancy(PlantImage P, const fl
           if (hit_posed(0, Y, vd, rk, X, g.dims))
             grid[(size_t)ix + (size_t)iy * G.nx + (size_t)iz * G.nx * G.ny] = 1;
         }
+  }
+}
+
+// Voxel collision maps (RoadmapBuilder::BuildCollisionMap and
+// BuildEdgeCollisionMap, planning/roadmap_builder.cpp:326-679): for every
+// item -- a node, or the interior configurations of an edge -- the voxels
+// whose sphere some robot geometry touches. One workgroup per item: for each
+// configuration and geometry, its threads split the geometry's candidate
+// voxels and mark hits in an LDS bitmap of the grid, which removes
+// duplicates within the item. Then every marked voxel counts the item
+// (pass 0) or stores it at the voxel's list cursor (pass 1).
+// Edge configurations follow the roadmap rule (roadmap_builder.cpp:526-541):
+// n = max(0, floor(|q2 - q1| / step) - 2), t = (k + 1) / n for k < n,
+// q = t q2 + (1 - t) q1.
+constexpr int kMapBlock = 256;
+constexpr int kMapMaxWords = 16384;  // 64 KB of LDS: grids up to 524,288 voxels
+
+template <int NB>
+__global__ __launch_bounds__(kMapBlock) void k_voxel_map(
+    PlantImage P, const float *__restrict__ nodes, int num_nodes, int dim,
+    const int32_t *__restrict__ edges, float step, RobotGeomList R, GridArgs G, int pass,
+    int32_t *__restrict__ counts, const int64_t *__restrict__ offsets,
+    int32_t *__restrict__ cursor, int32_t *__restrict__ items) {
+  extern __shared__ uint32_t bits[];
+  const int item = blockIdx.x;
+  const int nvox = G.nx * G.ny * G.nz;
+  const int W = (nvox + 31) / 32;
+  for (int w = threadIdx.x; w < W; w += kMapBlock) bits[w] = 0;
+  __syncthreads();
+  float q1[kEdgeDim], q2[kEdgeDim];
+  int nc = 1;
+  bool ok = true;
+  if (edges) {
+    const int a = edges[2 * item], b = edges[2 * item + 1];
+    ok = a >= 0 && a < num_nodes && b >= 0 && b < num_nodes;
+#pragma unroll
+    for (int d = 0; d < kEdgeDim; ++d) {
+      q1[d] = ok && d < dim ? nodes[(size_t)a * dim + d] : 0.f;
+      q2[d] = ok && d < dim ? nodes[(size_t)b * dim + d] : 0.f;
+    }
+    const float dist = norm_diff_rn(q1, q2, dim);
+    const float steps = floorf(dist / step) - 2.f;
+    nc = ok && steps > 0.f && steps < kMaxEdgePoints ? (int)steps : 0;
+  } else {
+#pragma unroll
+    for (int d = 0; d < kEdgeDim; ++d) q1[d] = d < dim ? nodes[(size_t)item * dim + d] : 0.f;
+  }
+  // configurations and geometries in a uniform loop (every thread computes
+  // the same FK); the candidate voxels of a geometry are split over the
+  // workgroup
+  for (int c = 0; c < nc; ++c) {
+    float q[kEdgeDim];
+    if (edges) {
+      const float t = (float)((double)(c + 1) / (1.0 * (double)nc));
+#pragma unroll
+      for (int d = 0; d < kEdgeDim; ++d)
+        q[d] = __fadd_rn(__fmul_rn(t, q2[d]), __fmul_rn(__fsub_rn(1.f, t), q1[d]));
+    } else {
+#pragma unroll
+      for (int d = 0; d < kEdgeDim; ++d) q[d] = q1[d];
+    }
+    CSD_BODY_FRAMES(B, NB);
+    fk_bodies<NB>(P, QReg<kEdgeDim>{q}, B);
+    for (int r = 0; r < R.n; ++r) {
+      const DevGeom &g = P.geoms[R.ids[r]];
+      float X[12];
+      geom_world<NB>(g, B, X);
+      int lo[3], hi[3];
+      geom_voxel_range(g, X, G, lo, hi);
+      const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+      if (sx <= 0 || sy <= 0 || sz <= 0) continue;
+      const int count = sx * sy * sz;
+      for (int k = threadIdx.x; k < count; k += kMapBlock) {
+        const int ix = lo[0] + k % sx, iy = lo[1] + (k / sx) % sy, iz = lo[2] + k / (sx * sy);
+        if (voxel_hit(g, X, G, ix, iy, iz)) {
+          const int v = ix + iy * G.nx + iz * G.nx * G.ny;
+          atomicOr(&bits[v >> 5], 1u << (v & 31));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < W; w += kMapBlock) {
+    uint32_t m = bits[w];
+    while (m) {
+      const int v = w * 32 + __ffs(m) - 1;
+      m &= m - 1;
+      if (pass == 0) {
+        atomicAdd(&counts[v], 1);
+      } else {
+        const int slot = atomicAdd(&cursor[v], 1);
+        items[offsets[v] + slot] = item;
+      }
+    }
   }
 }
 
@@ -300,6 +430,86 @@ int csd_occupancy(const csd_plant *plant, const float *q, int64_t n, int64_t str
     hipLaunchKernelGGL((k_occupancy<16>), g, dim3(kEdgeBlock), 0, (hipStream_t)stream,
                        plant->img, q, n, stride_n, stride_j, R, G, grid);
   CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+int csd_voxel_collision_map(const csd_plant *plant, const float *nodes, int32_t num_nodes,
+                            const int32_t *edges, int64_t num_edges, float step,
+                            const int32_t *robot_geoms, int32_t num_robot_geoms,
+                            const float *map_center, const float *map_size, float resolution,
+                            int64_t *offsets, int32_t *items, int64_t capacity, int64_t *total,
+                            void *stream) {
+  if (!plant || !map_center || !map_size || !total || !offsets)
+    return fail(CSD_ERR_INVALID_ARGUMENT, "null argument");
+  if (plant->img.num_bodies > 16) return fail(CSD_ERR_CAPACITY, "more than 16 moving joints");
+  const int dim = plant->img.dof;
+  if (dim > kEdgeDim) return fail(CSD_ERR_CAPACITY, "collision maps support up to 16 dims");
+  if (num_nodes < 0 || num_edges < 0 || capacity < 0 || !(resolution > 0.f))
+    return fail(CSD_ERR_INVALID_ARGUMENT, "bad size");
+  if (edges && !(step > 0.f)) return fail(CSD_ERR_INVALID_ARGUMENT, "step must be positive");
+  if (num_nodes > 0 && !nodes) return fail(CSD_ERR_INVALID_ARGUMENT, "null node array");
+  if (edges == nullptr && num_edges > 0) return fail(CSD_ERR_INVALID_ARGUMENT, "null edge array");
+  RobotGeomList R;
+  VoxelArgs V{nullptr, 0, 0.f, 0, 0};
+  int rc = CSD_OK;
+  fill_robot_list(plant, robot_geoms, num_robot_geoms, R, V, &rc);
+  if (rc) return rc;
+  GridArgs G;
+  for (int k = 0; k < 3; ++k) {
+    G.center[k] = map_center[k];
+    G.size[k] = map_size[k];
+  }
+  G.res = resolution;
+  G.radius = (float)(sqrt(3.0) * resolution);  // GetOfflineVoxelRadius, roadmap.h:68
+  G.nx = (int)ceilf(map_size[0] / (2 * resolution));
+  G.ny = (int)ceilf(map_size[1] / (2 * resolution));
+  G.nz = (int)ceilf(map_size[2] / (2 * resolution));
+  const int64_t nvox = (int64_t)G.nx * G.ny * G.nz;
+  if (G.nx < 1 || G.ny < 1 || G.nz < 1 || nvox > (int64_t)kMapMaxWords * 32)
+    return fail(CSD_ERR_CAPACITY, "collision maps support grids of 1..524288 voxels");
+  const int64_t nitems = edges ? num_edges : num_nodes;
+  if (nitems > 0x7fffffff) return fail(CSD_ERR_CAPACITY, "too many items");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<int64_t> off((size_t)nvox + 1, 0);
+  *total = 0;
+  if (nitems == 0 || R.n == 0) {
+    CSD_HIP(hipMemcpyAsync(offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, st));
+    CSD_HIP(hipStreamSynchronize(st));
+    return CSD_OK;
+  }
+  int32_t *counts = nullptr;
+  CSD_HIP(hipMalloc(&counts, (size_t)nvox * 4));
+  std::vector<int32_t> cnt((size_t)nvox, 0);
+  const size_t lds = (size_t)((nvox + 31) / 32) * 4;
+  const dim3 g((unsigned)nitems), b(kMapBlock);
+  hipError_t e = hipMemcpyAsync(counts, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, st);
+  auto launch = [&](int pass) {
+    if (plant->img.num_bodies <= 8)
+      hipLaunchKernelGGL((k_voxel_map<8>), g, b, lds, st, plant->img, nodes, num_nodes, dim, edges,
+                         step, R, G, pass, counts, offsets, counts, items);
+    else
+      hipLaunchKernelGGL((k_voxel_map<16>), g, b, lds, st, plant->img, nodes, num_nodes, dim,
+                         edges, step, R, G, pass, counts, offsets, counts, items);
+    return hipGetLastError();
+  };
+  if (e == hipSuccess) e = launch(0);
+  if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), counts, cnt.size() * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) {
+    for (int64_t v = 0; v < nvox; ++v) off[v + 1] = off[v] + cnt[v];
+    *total = off[nvox];
+    e = hipMemcpyAsync(offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, st);
+  }
+  bool fits = items != nullptr && capacity >= *total;
+  if (e == hipSuccess && fits) {
+    std::fill(cnt.begin(), cnt.end(), 0);  // the counts become the per-voxel cursors
+    e = hipMemcpyAsync(counts, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch(1);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(counts);
+  if (e != hipSuccess) return hip_fail(e, "voxel collision map");
+  if (items != nullptr && !fits) return fail(CSD_ERR_CAPACITY, "item buffer smaller than *total");
   return CSD_OK;
 }
 

@@ -1,0 +1,227 @@
+"""Dynamic-roadmap (DRM) offline maps with csdecomp's names
+(planning/roadmap.h, planning/roadmap_builder.cpp, pybind drm_bindings.h).
+
+The voxel collision maps -- which nodes, and which edges, each workspace
+voxel blocks -- are computed on the GPU by csd_voxel_collision_map (one
+workgroup per node or edge, LDS bitmap of the grid). Host code keeps the
+DRM's dictionaries and applies the reference's post-processing (voxels that
+block every node or every edge are dropped).
+"""
+import math
+
+import numpy as np
+
+from . import _pycsdecomp as _c
+
+
+class RoadmapOptions:
+    """planning/roadmap.h:55-75 (same fields and defaults)."""
+
+    def __init__(self):
+        self.robot_map_size_x = 1.0
+        self.robot_map_size_y = 1.0
+        self.robot_map_size_z = 1.0
+        self.map_center = np.zeros(3, np.float32)
+        self.nodes_processed_before_debug_statement = 1000
+        self.max_task_space_distance_between_nodes = 0.25
+        self.max_configuration_distance_between_nodes = 1.0
+        self.offline_voxel_resolution = 0.05
+        self.edge_step_size = 0.01
+
+    def get_offline_voxel_radius(self):
+        """sqrt(3) * resolution (roadmap.h:68), in float."""
+        return float(np.float32(math.sqrt(3.0) * np.float32(self.offline_voxel_resolution)))
+
+    def grid(self):
+        """(nx, ny, nz): ceil(size / (2 resolution)) per axis."""
+        r = np.float32(self.offline_voxel_resolution)
+        return tuple(int(math.ceil(np.float32(s) / (np.float32(2) * r))) for s in
+                     (self.robot_map_size_x, self.robot_map_size_y, self.robot_map_size_z))
+
+    def __repr__(self):
+        c = np.asarray(self.map_center, np.float32)
+        return (f"RoadmapOptions(robot_map_size_x={self.robot_map_size_x}, "
+                f"robot_map_size_y={self.robot_map_size_y}, robot_map_size_z={self.robot_map_size_z}, "
+                f"map_center=[{c[0]}, {c[1]}, {c[2]}], "
+                f"offline_voxel_resolution={self.offline_voxel_resolution}, "
+                f"edge_step_size={self.edge_step_size})")
+
+
+def GetCollisionVoxelCenter(index, options):
+    """Centre of voxel `index` (roadmap_builder.cpp:709-741): per axis
+    c - size / 2 + i 2 res + res, the products in float, the sum in double,
+    rounded to float."""
+    if index < 0:
+        raise RuntimeError(f"[RoadmapBuilder:GetCollisionVoxelCenter] invalid index {index}\n.")
+    nx, ny, _ = options.grid()
+    iz = index // (nx * ny)
+    iy = (index - nx * ny * iz) // nx
+    ix = index - nx * ny * iz - nx * iy
+    res = np.float32(options.offline_voxel_resolution)
+    c = np.asarray(options.map_center, np.float32)
+    sizes = (options.robot_map_size_x, options.robot_map_size_y, options.robot_map_size_z)
+    out = []
+    for k, i in enumerate((ix, iy, iz)):
+        v = (float(c[k]) - float(np.float32(sizes[k])) / 2.0 + float(np.float32(i * 2) * res)
+             + float(res))
+        out.append(np.float32(v))
+    return np.array(out, np.float32)
+
+
+def GetCollisionVoxelId(robot_p_voxel, options):
+    """Voxel id of a workspace point, -1 outside the map
+    (roadmap_builder.cpp:681-707)."""
+    nx, ny, nz = options.grid()
+    res = np.float32(options.offline_voxel_resolution)
+    p = np.asarray(robot_p_voxel, np.float32) - np.asarray(options.map_center, np.float32)
+    sizes = (options.robot_map_size_x, options.robot_map_size_y, options.robot_map_size_z)
+    idx = []
+    for k, n in enumerate((nx, ny, nz)):
+        l_true = np.float32(np.float32(n * 2) * res)
+        idx.append(math.floor(n * (float(p[k]) + float(np.float32(sizes[k])) / 2.0) / float(l_true)))
+    if min(idx) < 0 or idx[0] > nx - 1 or idx[1] > ny - 1 or idx[2] > nz - 1:
+        return -1
+    return idx[0] + idx[1] * nx + idx[2] * nx * ny
+
+
+class DRM:
+    """planning/roadmap.h DRM: the roadmap and its offline maps."""
+
+    def __init__(self):
+        self.collision_map = {}          # voxel id -> [node ids]
+        self._edge_collision_map = {}    # voxel id -> {(id1, id2), (id2, id1)}
+        self.node_adjacency_map = {}     # node id -> [node ids]
+        self.id_to_node_map = {}         # node id -> configuration
+        self.id_to_pose_map = {}         # node id -> 4x4 pose of the target link
+        self.options = RoadmapOptions()
+        self.target_link_index_ = -1
+        self.target_link_name_ = "TARGET_LINK_IS_UNINITIALIZED"
+
+    @property
+    def edge_collision_map(self):
+        """voxel id -> list of (id1, id2) pairs, both directions."""
+        return {k: sorted(v) for k, v in self._edge_collision_map.items()}
+
+    def Clear(self):
+        self.__init__()
+
+
+def _voxel_map(mplant, robot_geometry_ids, nodes, edges, step, options):
+    """{voxel: sorted item indices} from csd_voxel_collision_map."""
+    import torch
+    from . import capi
+    nx, ny, nz = options.grid()
+    V = nx * ny * nz
+    dev = torch.device("cuda", torch.cuda.current_device())
+    dp = capi.DevicePlant(mplant)
+    dn = torch.from_numpy(np.ascontiguousarray(nodes, np.float32)).to(dev)
+    de = None if edges is None else torch.from_numpy(np.ascontiguousarray(edges, np.int32)).to(dev)
+    off = torch.empty(V + 1, dtype=torch.int64, device=dev)
+    center = np.asarray(options.map_center, np.float32)
+    size = [options.robot_map_size_x, options.robot_map_size_y, options.robot_map_size_z]
+    ne = 0 if edges is None else len(edges)
+    args = (dp, dn, len(nodes), de, ne, float(step), list(robot_geometry_ids), center, size,
+            float(options.offline_voxel_resolution), off)
+    total = capi.voxel_collision_map(*args)
+    items = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    total = capi.voxel_collision_map(*args, items=items, capacity=total)
+    o = off.cpu().numpy()
+    it = items.cpu().numpy()[:total]
+    out = {}
+    for v in np.nonzero(o[1:] - o[:-1])[0]:
+        out[int(v)] = np.sort(it[o[v]:o[v + 1]])
+    return out
+
+
+class RoadmapBuilder:
+    """planning/roadmap_builder.h RoadmapBuilder (the offline maps and node
+    handling; the roadmap's adjacency is set on the DRM)."""
+
+    def __init__(self, plant, target_link_name, options):
+        self._plant = plant
+        self._mplant = plant.getMinimalPlant()
+        self._tree = plant.getKinematicTree()
+        self._robot_ids = list(plant.getSceneInspector().robot_geometry_ids)
+        self._dim = plant.numConfigurationVariables()
+        self._lo = np.asarray(plant.getPositionLowerLimits(), np.float32)
+        self._hi = np.asarray(plant.getPositionUpperLimits(), np.float32)
+        self._raw = {}
+        self._drm = DRM()
+        self._drm.target_link_name_ = target_link_name
+        self._drm.target_link_index_ = self._tree.get_link_index(target_link_name)
+        self._drm.options = options
+
+    def get_drm(self):
+        return self._drm
+
+    def reset(self):
+        self._drm.Clear()
+
+    def add_nodes_manual(self, nodes_to_add):
+        """Adds the collision-free columns of (dim, N) that lie in the joint
+        limits (roadmap_builder.cpp:71-121); returns the number added. Node
+        ids continue from the current count."""
+        q = np.asarray(nodes_to_add, np.float32)
+        if q.ndim != 2 or q.shape[0] != self._dim:
+            raise ValueError("nodes must be a (dim, N) array")
+        inside = np.all((q >= self._lo[:, None] - 1e-9) & (q <= self._hi[:, None] + 1e-9), axis=0)
+        q = q[:, inside]
+        if q.shape[1] == 0:
+            return 0
+        free = np.asarray(_c.CheckCollisionFreeCuda(np.asfortranarray(q), self._mplant), bool)
+        base = len(self._raw)
+        added = 0
+        for k in np.nonzero(free)[0]:
+            added += 1
+            self._raw[base + added] = q[:, k].copy()
+        # without a roadmap build, every added node is a roadmap node
+        for i, v in self._raw.items():
+            self._drm.id_to_node_map.setdefault(i, v)
+        return added
+
+    def build_collision_map(self):
+        """drm.collision_map: voxel -> nodes whose robot geometry touches the
+        voxel sphere, without voxels that touch every node
+        (roadmap_builder.cpp:326-482)."""
+        ids = sorted(self._drm.id_to_node_map)
+        nodes = np.array([self._drm.id_to_node_map[i] for i in ids], np.float32).reshape(-1, self._dim)
+        m = _voxel_map(self._mplant, self._robot_ids, nodes, None, 1.0, self._drm.options)
+        n = len(ids)
+        self._drm.collision_map = {v: [ids[k] for k in lst] for v, lst in m.items() if len(lst) < n}
+
+    def build_edge_collision_map(self, step_size_edge_collision_map):
+        """drm.edge_collision_map: voxel -> edges (both directions) with an
+        interior configuration touching the voxel sphere, without voxels that
+        block every edge (roadmap_builder.cpp:484-679)."""
+        ids = sorted(self._drm.id_to_node_map)
+        pos = {i: k for k, i in enumerate(ids)}
+        nodes = np.array([self._drm.id_to_node_map[i] for i in ids], np.float32).reshape(-1, self._dim)
+        edges = [(pos[a], pos[b]) for a, nbrs in sorted(self._drm.node_adjacency_map.items())
+                 for b in nbrs if a < b]
+        directed = sum(len(v) for v in self._drm.node_adjacency_map.values())
+        out = {}
+        if edges:
+            m = _voxel_map(self._mplant, self._robot_ids, nodes, np.array(edges, np.int32),
+                           step_size_edge_collision_map, self._drm.options)
+            for v, lst in m.items():
+                s = set()
+                for k in lst:
+                    a, b = ids[edges[k][0]], ids[edges[k][1]]
+                    s.add((a, b))
+                    s.add((b, a))
+                if len(s) != directed:
+                    out[v] = s
+        self._drm._edge_collision_map = out
+
+    def build_pose_map(self):
+        """drm.id_to_pose_map: pose of the target link at every node."""
+        ids = sorted(self._drm.id_to_node_map)
+        if not ids:
+            self._drm.id_to_pose_map = {}
+            return
+        q = np.asfortranarray(np.array([self._drm.id_to_node_map[i] for i in ids], np.float32).T)
+        tf = np.asarray(_c.computeForwardKinematicsCuda(q, self._mplant.kin_tree))
+        L = self._tree.num_links()
+        t = self._drm.target_link_index_
+        self._drm.id_to_pose_map = {i: tf[:, 4 * (k * L + t):4 * (k * L + t) + 4].copy()
+                                    for k, i in enumerate(ids)}

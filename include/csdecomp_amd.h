@@ -310,6 +310,32 @@ int csd_edit_regions(const csd_plant *plant, const float *collisions,
                      float margin, int32_t bisection_steps, float *scol,
                      float *proj, int32_t *status, void *stream);
 
+/* Offline voxel collision maps. Replaces BuildCollisionMap and
+ * BuildEdgeCollisionMap (planning/roadmap_builder.cpp:326-679), which check
+ * every (configuration, voxel) pair with
+ * checkCollisionFreeVoxelsWithoutSelfCollisionCuda in chunks.
+ * Grid of RoadmapOptions (planning/roadmap.h:55-75): nx = ceil(size_x /
+ * (2 res)) etc., voxel id ix + iy nx + iz nx ny, centre as
+ * GetCollisionVoxelCenter (roadmap_builder.cpp:709-741), voxel spheres of
+ * radius sqrt(3) res. Items are the nodes (edges == NULL) or the edges
+ * ([E][2] node indices; an edge's configurations are t q2 + (1 - t) q1,
+ * t = (k + 1) / n, k < n = max(0, floor(|q2 - q1| / step) - 2), the
+ * roadmap rule of roadmap_builder.cpp:526-541).
+ * Output, per voxel, the items at which some robot geometry touches the
+ * voxel sphere: offsets (device [V+1] int64, always written) and items
+ * (device [total] int32, item indices in unspecified order within a voxel;
+ * may be NULL to only count). *total (host) receives the number of entries;
+ * CSD_ERR_CAPACITY if items is given with capacity < *total (offsets and
+ * *total are still valid). nodes (device [N][dim]); robot_geoms, map_center,
+ * map_size are host arrays. Grids of at most 524,288 voxels. Synchronous. */
+int csd_voxel_collision_map(const csd_plant *plant, const float *nodes,
+                            int32_t num_nodes, const int32_t *edges,
+                            int64_t num_edges, float step,
+                            const int32_t *robot_geoms, int32_t num_robot_geoms,
+                            const float *map_center, const float *map_size,
+                            float resolution, int64_t *offsets, int32_t *items,
+                            int64_t capacity, int64_t *total, void *stream);
+
 /* ---- distance field ----
  * Replaces DistanceField (geometry/cuda_distance_field.h:133-172): exact
  * Euclidean distance from each voxel of a G^3 grid (voxel i at origin +

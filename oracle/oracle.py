@@ -391,3 +391,61 @@ def edt_brute(occ, threads=0):
     d2 = np.zeros(occ.size, np.int32)
     lib().orc_edt_brute(occ.reshape(-1), G, d2, threads)
     return d2.reshape(occ.shape)
+
+
+def roadmap_grid(center, size, res):
+    """(nx, ny, nz) and the voxel centres (3, V), ids ix + iy nx + iz nx ny:
+    per axis c - size/2 + i 2 res + res with i 2 res in float and the sum in
+    double, rounded to float (roadmap_builder.cpp:709-741)."""
+    res = np.float32(res)
+    n = [int(np.ceil(np.float32(s) / (np.float32(2) * res))) for s in size]
+    axes = []
+    for k in range(3):
+        i = np.arange(n[k])
+        axes.append((np.float64(np.float32(center[k])) - np.float64(np.float32(size[k])) / 2.0
+                     + ((i * 2).astype(np.float32) * res).astype(np.float64)
+                     + np.float64(res)).astype(np.float32))
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return n, np.stack([x.ravel(), y.ravel(), z.ravel()])
+
+
+def roadmap_edge_configs(q1, q2, step):
+    """Interior configurations of an edge by the roadmap rule
+    (roadmap_builder.cpp:526-541): n = max(0, floor(|q2 - q1| / step) - 2),
+    t = (k + 1) / n (double, rounded to float), q = t q2 + (1 - t) q1 in
+    float; |q2 - q1| as a sequential float sum of squares. (dim, n)."""
+    q1 = np.asarray(q1, np.float32)
+    q2 = np.asarray(q2, np.float32)
+    s = np.float32(0)
+    for d in range(len(q1)):
+        e = np.float32(q2[d] - q1[d])
+        s = np.float32(s + np.float32(e * e))
+    dist = np.float32(np.sqrt(s))
+    n = max(0, int(np.floor(np.float32(dist / np.float32(step)))) - 2)
+    out = np.zeros((len(q1), n), np.float32)
+    for k in range(n):
+        t = np.float32((k + 1) / (1.0 * n))
+        out[:, k] = np.float32(t) * q2 + np.float32(np.float32(1) - t) * q1
+    return out
+
+
+def voxel_map(plant, nodes, edges, step, robot_geoms, center, size, res, radius_delta=0.0):
+    """{voxel: sorted items} -- every voxel of the map checked against every
+    configuration of every item with voxel_flags (the reference's
+    BuildCollisionMap / BuildEdgeCollisionMap without their chunking).
+    nodes (N, dim); edges (E, 2) or None (items are then the nodes)."""
+    _, centres = roadmap_grid(center, size, res)
+    radius = float(np.float32(np.sqrt(3.0) * np.float32(res))) + radius_delta
+    nodes = np.asarray(nodes, np.float32)
+    if edges is None:
+        configs = [nodes[i][:, None] for i in range(len(nodes))]
+    else:
+        configs = [roadmap_edge_configs(nodes[a], nodes[b], step) for a, b in edges]
+    out = {}
+    for item, q in enumerate(configs):
+        if q.shape[1] == 0:
+            continue
+        flags = voxel_flags(plant, q, centres, radius, robot_geoms)
+        for v in np.nonzero((flags == 0).any(axis=0))[0]:
+            out.setdefault(int(v), []).append(item)
+    return out

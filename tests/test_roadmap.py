@@ -1,0 +1,78 @@
+"""Host side of the DRM offline maps (planning/roadmap_builder.cpp): grid
+formulas, the roadmap edge discretisation and the maps' post-processing
+(the GPU map itself is replaced by a fixed answer here)."""
+import numpy as np
+import pytest
+
+from conftest import parse
+
+
+def test_options_and_grid(csd):
+    o = csd.RoadmapOptions()
+    assert (o.robot_map_size_x, o.offline_voxel_resolution, o.edge_step_size) == (1.0, 0.05, 0.01)
+    assert o.grid() == (10, 10, 10)
+    assert abs(o.get_offline_voxel_radius() - np.sqrt(3) * 0.05) < 1e-7
+    o.offline_voxel_resolution = 0.25
+    assert o.grid() == (2, 2, 2)
+    # closed form: two voxels per axis centred at -0.25 and +0.25, x fastest
+    np.testing.assert_array_equal(csd.GetCollisionVoxelCenter(0, o), [-0.25, -0.25, -0.25])
+    np.testing.assert_array_equal(csd.GetCollisionVoxelCenter(1, o), [0.25, -0.25, -0.25])
+    np.testing.assert_array_equal(csd.GetCollisionVoxelCenter(6, o), [-0.25, 0.25, 0.25])
+    with pytest.raises(RuntimeError):
+        csd.GetCollisionVoxelCenter(-1, o)
+    assert csd.GetCollisionVoxelId([0.6, 0, 0], o) == -1
+
+
+def test_voxel_id_round_trip(csd, orc):
+    o = csd.RoadmapOptions()
+    o.map_center = np.array([0.1, -0.3, 0.55], np.float32)
+    o.robot_map_size_x, o.robot_map_size_y, o.robot_map_size_z = 1.2, 0.8, 1.0
+    o.offline_voxel_resolution = 0.04
+    nx, ny, nz = o.grid()
+    # 1.2f / (2 * 0.04f) rounds just above 15 in float: 16 voxels, as in the reference
+    assert (nx, ny, nz) == (16, 10, 13)
+    n, centres = orc.roadmap_grid(o.map_center, [1.2, 0.8, 1.0], 0.04)
+    assert n == [nx, ny, nz]
+    for i in range(0, nx * ny * nz, 7):
+        c = csd.GetCollisionVoxelCenter(i, o)
+        np.testing.assert_array_equal(c, centres[:, i])
+        assert csd.GetCollisionVoxelId(c, o) == i
+
+
+def test_edge_configurations(orc):
+    q1, q2 = np.zeros(3, np.float32), np.array([0.1, 0.0, 0.0], np.float32)
+    # floor(0.1 / 0.01) - 2 = 8 points at t = 1/8 .. 1: q2 included, q1 not
+    c = orc.roadmap_edge_configs(q1, q2, 0.01)
+    assert c.shape == (3, 8)
+    np.testing.assert_allclose(c[0], np.arange(1, 9) / 8 * 0.1, atol=1e-7)
+    assert c[0, -1] == np.float32(0.1)
+    assert orc.roadmap_edge_configs(q1, q1 + 0.02, 0.01).shape[1] == 1  # |d| = 0.0346
+    assert orc.roadmap_edge_configs(q1, q1 + 0.005, 0.01).shape[1] == 0
+
+
+def test_map_post_processing(csd, monkeypatch):
+    from csdecomp_amd import roadmap
+    plant = parse("boxes_in_corners.urdf").build_plant()
+    b = csd.RoadmapBuilder(plant, "movable", csd.RoadmapOptions())
+    drm = b.get_drm()
+    assert drm.target_link_index_ == plant.getKinematicTree().get_link_index("movable")
+    drm.id_to_node_map = {5: np.array([0.0, 0.0]), 9: np.array([0.5, 0.0]),
+                          2: np.array([0.0, 0.5])}
+    drm.node_adjacency_map = {5: [9, 2], 9: [5], 2: [5]}
+    # items are indices into the nodes sorted by id: 0 -> 2, 1 -> 5, 2 -> 9
+    calls = []
+
+    def fake(mplant, ids, nodes, edges, step, options):
+        calls.append(edges)
+        if edges is None:
+            return {3: np.array([0, 2]), 4: np.array([0, 1, 2]), 8: np.array([1])}
+        # edges by sorted ids, a < b: (2, 5) -> 0, (5, 9) -> 1
+        return {3: np.array([0]), 4: np.array([0, 1])}
+
+    monkeypatch.setattr(roadmap, "_voxel_map", fake)
+    b.build_collision_map()
+    assert drm.collision_map == {3: [2, 9], 8: [5]}  # voxel 4 blocks every node: dropped
+    b.build_edge_collision_map(0.01)
+    np.testing.assert_array_equal(np.asarray(calls[1]), [[0, 1], [1, 2]])
+    # voxel 4 blocks both edges in both directions (= all 4 adjacency entries): dropped
+    assert drm.edge_collision_map == {3: [(2, 5), (5, 2)]}
