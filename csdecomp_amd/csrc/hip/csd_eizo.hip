@@ -190,9 +190,7 @@ __global__ __launch_bounds__(kRankBlock) void k_eizo_rank(EizoDev D, const float
   if (u <= keep) return;
   const int n = min(u, cand);
   __shared__ float key[kRankMax];
-  __shared__ float s_v[kRankBlock / 64];
-  __shared__ int s_i[kRankBlock / 64];
-  __shared__ int sel[256];
+  __shared__ int idx[kRankMax];
   const float *q1 = D.ls + (size_t)r * dim, *q2 = D.le + (size_t)r * dim;
   for (int i = threadIdx.x; i < n; i += kRankBlock) {
     const float *p = D.part + ((size_t)r * U + i) * dim;
@@ -220,49 +218,39 @@ __global__ __launch_bounds__(kRankBlock) void k_eizo_rank(EizoDev D, const float
     }
     key[i] = frac * sqrtf(len2);
   }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = min(keep, 256);
-  for (int s = 0; s < k; ++s) {
-    float v = INFINITY;
-    int vi = 0x7fffffff;
-    for (int i = threadIdx.x; i < n; i += kRankBlock)
-      if (key[i] < v || (key[i] == v && i < vi)) {
-        v = key[i];
-        vi = i;
-      }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const float ov = __shfl_xor(v, o);
-      const int oi = __shfl_xor(vi, o);
-      if (ov < v || (ov == v && oi < vi)) {
-        v = ov;
-        vi = oi;
-      }
-    }
-    if (lane == 0) {
-      s_v[wave] = v;
-      s_i[wave] = vi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float bv = s_v[0];
-      int bi = s_i[0];
-      for (int w = 1; w < kRankBlock / 64; ++w)
-        if (s_v[w] < bv || (s_v[w] == bv && s_i[w] < bi)) {
-          bv = s_v[w];
-          bi = s_i[w];
-        }
-      sel[s] = bi;
-      key[bi] = INFINITY;
-    }
-    __syncthreads();
+  // bitonic sort of (key, index) over the next power of two >= n, ascending
+  // key, lower index first on ties; padding sorts last
+  int P2 = 1;
+  while (P2 < n) P2 <<= 1;
+  for (int i = threadIdx.x; i < P2; i += kRankBlock) {
+    if (i >= n) key[i] = INFINITY;
+    idx[i] = i < n ? i : 0x7fffffff;
   }
-  // reorder: selected particles to the front (through tmp [slot][256][dim])
-  float *t = tmp + (size_t)slot * 256 * dim;
+  __syncthreads();
+  for (int k2 = 2; k2 <= P2; k2 <<= 1)
+    for (int j = k2 >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P2; i += kRankBlock) {
+        const int l = i ^ j;
+        if (l > i) {
+          const float ki = key[i], kl = key[l];
+          const int ii = idx[i], il = idx[l];
+          const bool after = ki > kl || (ki == kl && ii > il);
+          if (after == ((i & k2) == 0)) {
+            key[i] = kl;
+            key[l] = ki;
+            idx[i] = il;
+            idx[l] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  const int k = min(keep, n);
+  // reorder: selected particles to the front (through tmp [slot][kRankMax][dim])
+  float *t = tmp + (size_t)slot * kRankMax * dim;
   for (int e = threadIdx.x; e < k * dim; e += kRankBlock) {
     const int j = e / dim, d = e - j * dim;
-    t[e] = D.part[((size_t)r * U + sel[j]) * dim + d];
+    t[e] = D.part[((size_t)r * U + idx[j]) * dim + d];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < k * dim; e += kRankBlock)
@@ -419,9 +407,9 @@ int csd_eizo_set_guidance(csd_eizo *w, uint32_t pilot_samples, float pilot_facto
   if (pilot_samples > 262144) return fail(CSD_ERR_CAPACITY, "at most 262144 pilot samples");
   if (pilot_samples && !(pilot_factor > 0.f))
     return fail(CSD_ERR_INVALID_ARGUMENT, "pilot_factor must be positive");
-  if (surrogate && (keep_particles < 1 || keep_particles > 256 || line_points < 1 ||
+  if (surrogate && (keep_particles < 1 || keep_particles > kRankMax || line_points < 1 ||
                     line_points > 64))
-    return fail(CSD_ERR_INVALID_ARGUMENT, "keep_particles must be 1..256 and line_points 1..64");
+    return fail(CSD_ERR_INVALID_ARGUMENT, "keep_particles must be 1..1024 and line_points 1..64");
   w->pilot = pilot_samples;
   w->pilot_factor = pilot_factor;
   w->surrogate = surrogate;
@@ -588,6 +576,15 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
         w->pred_bytes = 0;
         CSD_HIP(hipMalloc(&w->pred, need));
         w->pred_bytes = need;
+      }
+      // the ranking reorders through the staging buffer (dead at this point)
+      const size_t rank_need = (size_t)na * kRankMax * dim * sizeof(float);
+      if (rank_need > w->stage_bytes) {
+        if (D.stage) hipFree(D.stage);
+        D.stage = nullptr;
+        w->stage_bytes = 0;
+        CSD_HIP(hipMalloc(&D.stage, rank_need));
+        w->stage_bytes = rank_need;
       }
       SegmentPoints sp{D.part, D.ls, D.le, D.active, D.u, U, dim, L, cand};
       rc = mlp_predict_segment_points(w->surrogate, sp, na, w->pred, 0);

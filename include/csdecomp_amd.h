@@ -241,7 +241,7 @@ struct csd_mlp;
  *    samples, so certification keeps the reference's guarantee.
  *  surrogate != NULL: particles are ranked by the surrogate's predicted
  *    boundary distance (signed distance at line_points points between each
- *    particle's projection and the particle) and only the keep_particles best
+ *    particle's projection and the particle) and only the keep_particles (1..1024) best
  *    of the first 1024 are bisected; the exact checker still judges every
  *    bisection step and every sample. */
 int csd_eizo_set_guidance(csd_eizo *ws, uint32_t pilot_samples, float pilot_factor,

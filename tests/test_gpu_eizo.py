@@ -377,13 +377,14 @@ def test_guided_eizo_certifies(gpu, csd, orc):
     rg = list(pl.getRobotGeometryIds())
     model = _train_surrogate(gpu, pl)
     results = {}
-    for mode in ("plain", "pilot", "guided"):
+    for mode in ("plain", "pilot", "guided", "guided_wide"):
         inf = csd.CudaEdgeInflator(mp, rg, opt, dom, R, 10)
         if mode == "pilot":
             inf.set_guidance(pilot_samples=2048, pilot_factor=10.0)
-        elif mode == "guided":
+        elif mode.startswith("guided"):
+            # 32 of the ranked particles, or all 1,024 ranked (the bitonic order)
             inf.set_guidance(pilot_samples=2048, pilot_factor=10.0, surrogate=model.address,
-                             keep_particles=32, line_points=4)
+                             keep_particles=32 if mode == "guided" else 1024, line_points=4)
         regions, infos = inf.inflateEdges(ls, le, None, 0.0, [])
         again, _ = inf.inflateEdges(ls, le, None, 0.0, [])
         results[mode] = infos
@@ -409,3 +410,5 @@ def test_guidance_argument_checks(gpu, csd):
         inf.set_guidance(pilot_samples=1000, pilot_factor=0.0)
     with pytest.raises((ValueError, RuntimeError)):
         inf.set_guidance(surrogate=1, keep_particles=0)
+    with pytest.raises((ValueError, RuntimeError)):
+        inf.set_guidance(surrogate=1, keep_particles=1025)
