@@ -43,6 +43,7 @@ _SIGS = {
     "csd_validate_edges": (C.c_int, [_VP, _VP, _I32, _VP, _I64, _F, _I32, _VP, _VP]),
     "csd_occupancy": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _VP, _VP, _F, _VP, _VP]),
     "csd_knn": (C.c_int, [_VP, _I32, _I32, _I32, _VP, _VP, _VP]),
+    "csd_knn_page": (C.c_int, [_VP, _I32, _I32, _VP, _I32, _I32, _VP, _VP, _VP, _VP, _VP]),
     "csd_voxel_collision_map": (C.c_int, [_VP, _VP, _I32, _VP, _I64, _F, _VP, _I32, _VP, _VP, _F,
                                           _VP, _VP, _I64, C.POINTER(_I64), _VP]),
     "csd_stream_synchronize": (C.c_int, [_VP]),
@@ -258,6 +259,15 @@ def voxel_collision_map(plant, nodes, num_nodes, edges, num_edges, step, robot_g
                                         C.byref(total), _stream(stream)),
           "csd_voxel_collision_map")
     return int(total.value)
+
+
+def knn_page(nodes, num_nodes, dim, queries, num_queries, k, after_d2, after_idx, out_idx,
+             out_d2, stream=None):
+    """csd_knn_page: the next k neighbours of each query node after
+    (after_d2, after_idx) (both None: the first k)."""
+    check(lib().csd_knn_page(_ptr(nodes), num_nodes, dim, _ptr(queries), num_queries, k,
+                             _ptr(after_d2), _ptr(after_idx), _ptr(out_idx), _ptr(out_d2),
+                             _stream(stream)), "csd_knn_page")
 
 
 def knn(nodes, num_nodes, dim, k, out_idx, out_d2=None, stream=None):

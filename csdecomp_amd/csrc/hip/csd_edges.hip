@@ -34,6 +34,8 @@ __device__ __forceinline__ float norm_diff_rn(const float *a, const float *b, in
 //         (plant/collision_checker.cpp:41-53)
 // rule 1: VisibilityGraph, n = (uint)(|q2-q1|/step + 2) points: q1, q2 and
 //         t = (k+1)/(n-1) for k < n-2 (planning/cuda_visibility_graph.cu:58-172)
+// rule 2: RoadmapBuilder, n = max(0, floor(|q2-q1|/step) - 2), t = (k+1)/n,
+//         k < n (q2 checked, q1 not; planning/roadmap_builder.cpp:262-280)
 template <int VAR>
 __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
     PlantImage P, const float *__restrict__ nodes, int num_nodes, int dim,
@@ -68,6 +70,8 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
     float tstep = 0.f;
     if (rule == 0) {
       npts = (int64_t)ceilf(dist / step);
+    } else if (rule == 2) {
+      npts = max((int64_t)0, (int64_t)floorf(dist / step) - 2);
     } else {
       npts = (int64_t)(uint32_t)(dist / step + 2);
       tstep = (float)(1.0 / (double)(npts - 1));
@@ -80,6 +84,8 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
       float t;
       if (rule == 0) {
         t = (float)((double)k / (1.0 * (double)npts));
+      } else if (rule == 2) {
+        t = (float)((double)(k + 1) / (1.0 * (double)npts));
       } else {
         t = k == 0 ? 0.f : (k == 1 ? 1.f : __fmul_rn(tstep, (float)(k - 1)));
       }
@@ -289,16 +295,28 @@ __global__ __launch_bounds__(kMapBlock) void k_voxel_map(
 // a register list with static indices. Node coordinates are staged in LDS in
 // rows zero-padded to DP floats; the padding adds +0 to each sum.
 constexpr int kKnnMax = 16;
+// Paged use: query row i is node queries[i] (all nodes in order when
+// queries is null) and, with after_d2 / after_idx, only candidates after
+// (after_d2[i], after_idx[i]) in (distance, index) order count, so that
+// successive calls return successive pages of the sorted neighbour list.
 template <int K, int DP>
 __global__ __launch_bounds__(256) void k_knn(const float *__restrict__ nodes, int N, int dim,
+                                             const int32_t *__restrict__ queries, int Q,
+                                             const float *__restrict__ after_d2,
+                                             const int32_t *__restrict__ after_idx,
                                              int32_t *__restrict__ out_idx,
                                              float *__restrict__ out_d2) {
   __shared__ __attribute__((aligned(16))) float tile[256 * DP];
   const int i = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = i < N;
+  const bool valid = i < Q;
+  const int qn = valid ? (queries ? queries[i] : i) : 0;
+  const bool paged = after_d2 != nullptr;
+  const float ad = paged && valid ? after_d2[i] : -INFINITY;
+  const int ai = paged && valid ? after_idx[i] : -1;
   float qv[DP];
 #pragma unroll
-  for (int d = 0; d < DP; ++d) qv[d] = (valid && d < dim) ? nodes[(size_t)i * dim + d] : 0.f;
+  for (int d = 0; d < DP; ++d)
+    qv[d] = (valid && d < dim && qn >= 0 && qn < N) ? nodes[(size_t)qn * dim + d] : 0.f;
   float bd[K];
   int bi[K];
 #pragma unroll
@@ -322,7 +340,9 @@ __global__ __launch_bounds__(256) void k_knn(const float *__restrict__ nodes, in
         s = __fadd_rn(s, __fmul_rn(e, e));
       }
       const int jj = t0 + j;
-      if (s > 1.1920928955078125e-07f && (s < bd[K - 1] || (s == bd[K - 1] && jj < bi[K - 1]))) {
+      const bool later = s > ad || (s == ad && jj > ai);
+      if (s > 1.1920928955078125e-07f && later &&
+          (s < bd[K - 1] || (s == bd[K - 1] && jj < bi[K - 1]))) {
         // insertion into the sorted list (static indices)
         float cd = s;
         int ci = jj;
@@ -347,19 +367,31 @@ __global__ __launch_bounds__(256) void k_knn(const float *__restrict__ nodes, in
   }
 }
 
+struct KnnArgs {
+  const float *nodes;
+  int N, dim;
+  const int32_t *queries;
+  int Q;
+  const float *after_d2;
+  const int32_t *after_idx;
+  int32_t *idx;
+  float *d2;
+};
+
 template <int K>
-void launch_knn(const float *nodes, int N, int dim, int32_t *idx, float *d2, hipStream_t st) {
-  const dim3 g((N + 255) / 256), b(256);
-  if (dim <= 8)
-    hipLaunchKernelGGL((k_knn<K, 8>), g, b, 0, st, nodes, N, dim, idx, d2);
+void launch_knn(const KnnArgs &a, hipStream_t st) {
+  const dim3 g((a.Q + 255) / 256), b(256);
+  if (a.dim <= 8)
+    hipLaunchKernelGGL((k_knn<K, 8>), g, b, 0, st, a.nodes, a.N, a.dim, a.queries, a.Q,
+                       a.after_d2, a.after_idx, a.idx, a.d2);
   else
-    hipLaunchKernelGGL((k_knn<K, 16>), g, b, 0, st, nodes, N, dim, idx, d2);
+    hipLaunchKernelGGL((k_knn<K, 16>), g, b, 0, st, a.nodes, a.N, a.dim, a.queries, a.Q,
+                       a.after_d2, a.after_idx, a.idx, a.d2);
 }
 
 template <int... Ks>
-void launch_knn_k(int k, const float *nodes, int N, int dim, int32_t *idx, float *d2,
-                  hipStream_t st, std::integer_sequence<int, Ks...>) {
-  ((k == Ks + 1 ? launch_knn<Ks + 1>(nodes, N, dim, idx, d2, st) : void()), ...);
+void launch_knn_k(int k, const KnnArgs &a, hipStream_t st, std::integer_sequence<int, Ks...>) {
+  ((k == Ks + 1 ? launch_knn<Ks + 1>(a, st) : void()), ...);
 }
 
 namespace {
@@ -384,7 +416,7 @@ int csd_validate_edges(const csd_plant *plant, const float *nodes, int32_t num_n
   if (plant->img.num_bodies > 16) return fail(CSD_ERR_CAPACITY, "more than 16 moving joints");
   if (num_edges < 0 || num_nodes < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative size");
   if (!(step > 0.f)) return fail(CSD_ERR_INVALID_ARGUMENT, "step must be positive");
-  if (rule != 0 && rule != 1) return fail(CSD_ERR_INVALID_ARGUMENT, "rule must be 0 or 1");
+  if (rule < 0 || rule > 2) return fail(CSD_ERR_INVALID_ARGUMENT, "rule must be 0, 1 or 2");
   if (num_edges == 0) return CSD_OK;
   if (!nodes || !edges || !out) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
   const int var = check_variant(plant->img, false);
@@ -520,8 +552,24 @@ int csd_knn(const float *nodes, int32_t num_nodes, int32_t dim, int32_t k, int32
   if (num_nodes < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative size");
   if (num_nodes == 0) return CSD_OK;
   if (!nodes || !out_idx) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
-  launch_knn_k(k, nodes, num_nodes, dim, out_idx, out_d2, (hipStream_t)stream,
-               std::make_integer_sequence<int, kKnnMax>{});
+  const KnnArgs a{nodes, num_nodes, dim, nullptr, num_nodes, nullptr, nullptr, out_idx, out_d2};
+  launch_knn_k(k, a, (hipStream_t)stream, std::make_integer_sequence<int, kKnnMax>{});
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
+
+int csd_knn_page(const float *nodes, int32_t num_nodes, int32_t dim, const int32_t *queries,
+                 int32_t num_queries, int32_t k, const float *after_d2,
+                 const int32_t *after_idx, int32_t *out_idx, float *out_d2, void *stream) {
+  if (dim < 1 || dim > kEdgeDim) return fail(CSD_ERR_CAPACITY, "k-NN supports 1..16 dims");
+  if (k < 1 || k > kKnnMax) return fail(CSD_ERR_CAPACITY, "k must be in 1..16");
+  if (num_nodes < 0 || num_queries < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative size");
+  if (num_queries == 0) return CSD_OK;
+  if (!nodes || !queries || !out_idx || !out_d2 || (after_d2 == nullptr) != (after_idx == nullptr))
+    return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  const KnnArgs a{nodes, num_nodes, dim, queries, num_queries, after_d2, after_idx, out_idx,
+                  out_d2};
+  launch_knn_k(k, a, (hipStream_t)stream, std::make_integer_sequence<int, kKnnMax>{});
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

@@ -134,8 +134,8 @@ def _voxel_map(mplant, robot_geometry_ids, nodes, edges, step, options):
 
 
 class RoadmapBuilder:
-    """planning/roadmap_builder.h RoadmapBuilder (the offline maps and node
-    handling; the roadmap's adjacency is set on the DRM)."""
+    """planning/roadmap_builder.h RoadmapBuilder: nodes, roadmap, pose map
+    and the offline voxel maps."""
 
     def __init__(self, plant, target_link_name, options):
         self._plant = plant
@@ -174,10 +174,91 @@ class RoadmapBuilder:
         for k in np.nonzero(free)[0]:
             added += 1
             self._raw[base + added] = q[:, k].copy()
-        # without a roadmap build, every added node is a roadmap node
-        for i, v in self._raw.items():
-            self._drm.id_to_node_map.setdefault(i, v)
         return added
+
+    def build_roadmap(self, max_neighbors):
+        """The roadmap over the added nodes (roadmap_builder.cpp:137-311).
+        For each node, candidates in (squared distance, index) order with
+        FLT_EPSILON < d^2 < max_configuration_distance_between_nodes; a
+        candidate is accepted unless some link frame moves more than
+        max_task_space_distance_between_nodes, the distance exceeds
+        max_configuration_distance_between_nodes, or an interior point of the
+        edge (roadmap rule: floor(d / edge_step_size) - 2 points, the
+        neighbour included) collides; at most max_neighbors per node. Nodes
+        with a neighbour enter the DRM, and the adjacency is made symmetric.
+        Neighbour lists come page by page from csd_knn_page, edge checks from
+        csd_validate_edges (rule 2), link frames from the GPU FK."""
+        import torch
+        from . import capi
+        ids = sorted(self._raw)
+        N = len(ids)
+        opts = self._drm.options
+        if N == 0 or max_neighbors <= 0:
+            return
+        dim = self._dim
+        X = np.array([self._raw[i] for i in ids], np.float32).reshape(N, dim)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        dX = torch.from_numpy(X).to(dev)
+        dp = capi.DevicePlant(self._mplant)
+        L = self._tree.num_links()
+        tf = np.asarray(_c.computeForwardKinematicsCuda(np.asfortranarray(X.T), self._mplant.kin_tree))
+        T = tf[:3, 3::4].T.reshape(N, L, 3)  # link translations
+        maxd = np.float32(opts.max_configuration_distance_between_nodes)
+        maxt = np.float32(opts.max_task_space_distance_between_nodes)
+        K = 16
+        accepted = [[] for _ in range(N)]
+        pending = np.arange(N, dtype=np.int32)
+        after_d2 = after_idx = None
+        while pending.size:
+            Q = len(pending)
+            out_i = torch.empty(Q * K, dtype=torch.int32, device=dev)
+            out_d = torch.empty(Q * K, dtype=torch.float32, device=dev)
+            capi.knn_page(dX, N, dim, torch.from_numpy(pending).to(dev), Q, K,
+                          None if after_d2 is None else torch.from_numpy(after_d2).to(dev),
+                          None if after_idx is None else torch.from_numpy(after_idx).to(dev),
+                          out_i, out_d)
+            idx = out_i.cpu().numpy().reshape(Q, K)
+            d2 = out_d.cpu().numpy().reshape(Q, K)
+            inside = (idx >= 0) & (d2 < maxd)
+            rows, cols = np.nonzero(inside)
+            a, b = pending[rows], idx[rows, cols]
+            dt = np.sqrt(((T[a] - T[b]) ** 2).sum(-1, dtype=np.float32)).max(-1)
+            e = (X[a] - X[b]).astype(np.float32)
+            dc = np.sqrt((e * e).sum(-1, dtype=np.float32))
+            ok = (dt <= maxt) & (dc <= maxd)
+            free = np.zeros(len(a), bool)
+            if ok.any():
+                edges = np.stack([a[ok], b[ok]], 1).astype(np.int32)
+                flags = torch.empty(len(edges), dtype=torch.uint8, device=dev)
+                capi.validate_edges(dp, dX, N, torch.from_numpy(edges).to(dev), len(edges),
+                                    float(opts.edge_step_size), 2, flags)
+                free[np.nonzero(ok)[0]] = flags.cpu().numpy().astype(bool)
+            good = np.zeros((Q, K), bool)
+            good[rows, cols] = free
+            nxt, nd, ni = [], [], []
+            for r in range(Q):
+                node = pending[r]
+                for c in range(K):
+                    if len(accepted[node]) >= max_neighbors or not inside[r, c]:
+                        break
+                    if good[r, c]:
+                        accepted[node].append(int(idx[r, c]))
+                else:  # a full page inside the radius: the list may go on
+                    if len(accepted[node]) < max_neighbors:
+                        nxt.append(node)
+                        nd.append(d2[r, K - 1])
+                        ni.append(idx[r, K - 1])
+            pending = np.array(nxt, np.int32)
+            after_d2, after_idx = np.array(nd, np.float32), np.array(ni, np.int32)
+        drm = self._drm
+        for k, nb in enumerate(accepted):
+            if nb:
+                drm.id_to_node_map[ids[k]] = X[k].copy()
+                drm.node_adjacency_map[ids[k]] = [ids[j] for j in nb]
+        missing = [(b, a) for a, nb in drm.node_adjacency_map.items() for b in nb
+                   if a not in drm.node_adjacency_map.get(b, [])]
+        for b, a in missing:
+            drm.node_adjacency_map.setdefault(b, []).append(a)
 
     def build_collision_map(self):
         """drm.collision_map: voxel -> nodes whose robot geometry touches the

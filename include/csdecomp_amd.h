@@ -372,6 +372,9 @@ int csd_distance_field_squared(const csd_distance_field *field, const int32_t **
  * rule 1 -- VisibilityGraph (planning/cuda_visibility_graph.cu:58-172):
  *   n = (uint)(|q2-q1| / step + 2) points: q1, q2 and t = (k+1)/(n-1),
  *   k < n-2.
+ * rule 2 -- RoadmapBuilder::BuildRoadmap (planning/roadmap_builder.cpp
+ *   :262-280): n = max(0, floor(|q2-q1| / step) - 2) points t = (k+1)/n,
+ *   k < n (q2 is checked, q1 is not; no points: free).
  * nodes (device) [num_nodes][dim], edges (device) [E][2] node indices,
  * out (device) [E] uint8 (1 = collision free). */
 int csd_validate_edges(const csd_plant *plant, const float *nodes,
@@ -395,6 +398,18 @@ int csd_occupancy(const csd_plant *plant, const float *q, int64_t n,
  * out_idx (device) [N][k] sorted by distance then index. */
 int csd_knn(const float *nodes, int32_t num_nodes, int32_t dim, int32_t k,
             int32_t *out_idx, float *out_d2, void *stream);
+
+/* Paged k nearest neighbours for the nodes queries[0..Q) (device [Q]):
+ * row i of out_idx / out_d2 (device [Q][k]) lists, in (squared distance,
+ * index) order, the k nearest neighbours of node queries[i] that come after
+ * (after_d2[i], after_idx[i]) in that order (both NULL: from the start), so
+ * repeated calls walk the full sorted list of ComputeJointDistancesKernel +
+ * sort (roadmap_builder.cpp:176-203) page by page. -1 / +inf pad short
+ * pages. */
+int csd_knn_page(const float *nodes, int32_t num_nodes, int32_t dim,
+                 const int32_t *queries, int32_t num_queries, int32_t k,
+                 const float *after_d2, const int32_t *after_idx,
+                 int32_t *out_idx, float *out_d2, void *stream);
 
 /* ---- signed-distance surrogate ----
  * No counterpart in the reference (its checkers return booleans only). An MLP

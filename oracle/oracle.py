@@ -449,3 +449,42 @@ def voxel_map(plant, nodes, edges, step, robot_geoms, center, size, res, radius_
         for v in np.nonzero((flags == 0).any(axis=0))[0]:
             out.setdefault(int(v), []).append(item)
     return out
+
+
+def build_roadmap(plant, X, max_neighbors, max_config, max_task, step, rule=RULE_GPU):
+    """RoadmapBuilder::BuildRoadmap restated (roadmap_builder.cpp:137-311):
+    per node, candidates with FLT_EPSILON < d^2 < max_config in (d^2, index)
+    order (d^2 a sequential float sum); skip if any link frame moves more than
+    max_task or |d| > max_config; accept if no interior point of the edge
+    (roadmap_edge_configs) collides; at most max_neighbors; then the
+    symmetric fix-up. X (N, dim); returns {node index: [node indices]}."""
+    X = np.asarray(X, np.float32)
+    N, dim = X.shape
+    T = np.stack([fk(plant, X[k])[:, :3, 3] for k in range(N)])
+    adj = {}
+    for i in range(N):
+        d2 = np.zeros(N, np.float32)
+        for d in range(dim):
+            e = (X[:, d] - X[i, d]).astype(np.float32)
+            d2 = (d2 + (e * e).astype(np.float32)).astype(np.float32)
+        cand = [j for j in np.lexsort((np.arange(N), d2))
+                if d2[j] > np.float32(1.1920928955078125e-07) and d2[j] < np.float32(max_config)]
+        acc = []
+        for j in cand:
+            if len(acc) >= max_neighbors:
+                break
+            dt = np.sqrt(((T[i] - T[j]) ** 2).sum(-1, dtype=np.float32)).max()
+            if dt > np.float32(max_task):
+                continue
+            e = (X[i] - X[j]).astype(np.float32)
+            if np.sqrt((e * e).sum(dtype=np.float32)) > np.float32(max_config):
+                continue
+            q = roadmap_edge_configs(X[i], X[j], step)
+            if q.shape[1] == 0 or check(plant, q, rule).all():
+                acc.append(int(j))
+        if acc:
+            adj[i] = acc
+    missing = [(b, a) for a, nb in adj.items() for b in nb if a not in adj.get(b, [])]
+    for b, a in missing:
+        adj.setdefault(b, []).append(a)
+    return adj

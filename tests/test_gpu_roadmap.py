@@ -85,39 +85,74 @@ def test_edge_map_parity(gpu, orc):
 
 
 def test_roadmap_builder_flow(gpu, orc, csd):
+    """add_nodes_manual, build_roadmap (against the oracle's BuildRoadmap),
+    the two voxel maps with their drop rules, and the pose map."""
     from csdecomp_amd import scenes
     pl, mp, rg = _scene()
     opts = csd.RoadmapOptions()
     opts.map_center = CENTER
     opts.robot_map_size_x = opts.robot_map_size_y = opts.robot_map_size_z = 1.6
     opts.offline_voxel_resolution = RES
+    opts.max_configuration_distance_between_nodes = 4.0
+    opts.max_task_space_distance_between_nodes = 0.6
+    opts.edge_step_size = 0.05
     b = csd.RoadmapBuilder(pl, pl.getKinematicTree().get_link_name(7), opts)
-    q = scenes.uniform_configs(33, 4000, pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
+    q = scenes.uniform_configs(33, 20000, pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
     added = b.add_nodes_manual(q)
-    drm = b.get_drm()
     free = np.asarray(csd.CheckCollisionFreeCuda(q, mp), bool)
-    assert added == free.sum() == len(drm.id_to_node_map) > 10
-    ids = sorted(drm.id_to_node_map)
-    X = np.array([drm.id_to_node_map[i] for i in ids])
-    # a ring roadmap: each node to its successor
-    drm.node_adjacency_map = {i: [] for i in ids}
-    for k in range(len(ids)):
-        a, c = ids[k], ids[(k + 1) % len(ids)]
-        drm.node_adjacency_map[a].append(c)
-        drm.node_adjacency_map[c].append(a)
-    b.build_collision_map()
+    assert added == free.sum() > 100
+    X = np.ascontiguousarray(q[:, free].T)
+    b.build_roadmap(5)
+    drm = b.get_drm()
     op = orc.Plant(mp)
-    ref = orc.voxel_map(op, X, None, 1.0, rg, CENTER, SIZE, RES)
-    expect = {v: [ids[i] for i in lst] for v, lst in ref.items() if len(lst) < len(ids)}
+    ref = orc.build_roadmap(op, X, 5, 4.0, 0.6, 0.05)
+    ids = sorted(drm.id_to_node_map)
+    got = {k - 1: sorted(j - 1 for j in v) for k, v in drm.node_adjacency_map.items()}
+    assert {k: sorted(v) for k, v in ref.items()} == got
+    assert len(got) > 20 and all(len(v) >= 1 for v in got.values())
+    b.build_collision_map()
+    nodes = np.array([drm.id_to_node_map[i] for i in ids])
+    cm = orc.voxel_map(op, nodes, None, 1.0, rg, CENTER, SIZE, RES)
+    expect = {v: [ids[i] for i in lst] for v, lst in cm.items() if len(lst) < len(ids)}
     assert len(set(drm.collision_map) ^ set(expect)) <= 3
-    agree = sum(drm.collision_map.get(v) == expect.get(v) for v in expect)
-    assert agree >= len(expect) - 3
+    assert sum(drm.collision_map.get(v) == expect.get(v) for v in expect) >= len(expect) - 3
     b.build_edge_collision_map(0.05)
     ecm = drm.edge_collision_map
     assert ecm and all((b_, a_) in v for v in ecm.values() for a_, b_ in v)
     b.build_pose_map()
     t = drm.target_link_index_
     for k in (0, len(ids) - 1):
-        tf = csd.computeLinkFrameToWorldTransformsMinimal(X[k], mp.kin_tree)
+        tf = csd.computeLinkFrameToWorldTransformsMinimal(nodes[k], mp.kin_tree)
         np.testing.assert_allclose(drm.id_to_pose_map[ids[k]], np.asarray(tf)[4 * t:4 * t + 4],
                                    atol=1e-5)
+
+
+def test_knn_pages(gpu, orc):
+    """Successive csd_knn_page calls walk each node's full sorted list."""
+    import torch
+    from csdecomp_amd import capi
+    rng = np.random.default_rng(41)
+    X = rng.uniform(-1, 1, (500, 7)).astype(np.float32)
+    X[7] = X[3]  # an exact duplicate is skipped (d^2 <= FLT_EPSILON)
+    dX = torch.from_numpy(X).to(gpu)
+    queries = np.array([0, 3, 250, 499], np.int32)
+    got = {int(qn): [] for qn in queries}
+    ad = ai = None
+    for _ in range(4):
+        oi = torch.empty(len(queries) * 16, dtype=torch.int32, device=gpu)
+        od = torch.empty(len(queries) * 16, dtype=torch.float32, device=gpu)
+        capi.knn_page(dX, 500, 7, torch.from_numpy(queries).to(gpu), len(queries), 16,
+                      None if ad is None else torch.from_numpy(ad).to(gpu),
+                      None if ai is None else torch.from_numpy(ai).to(gpu), oi, od)
+        idx = oi.cpu().numpy().reshape(-1, 16)
+        d2 = od.cpu().numpy().reshape(-1, 16)
+        for r, qn in enumerate(queries):
+            got[int(qn)] += idx[r].tolist()
+        ad, ai = d2[:, -1].copy(), idx[:, -1].copy()
+    for qn in queries:
+        d2 = np.zeros(500, np.float32)
+        for d in range(7):
+            e = (X[:, d] - X[qn, d]).astype(np.float32)
+            d2 = (d2 + e * e).astype(np.float32)
+        order = [j for j in np.lexsort((np.arange(500), d2)) if d2[j] > np.float32(1.1920928955078125e-07)]
+        assert got[int(qn)] == order[:64], qn
