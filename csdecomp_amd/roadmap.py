@@ -105,6 +105,123 @@ class DRM:
     def Clear(self):
         self.__init__()
 
+    def GetWorkspaceCorners(self):
+        """The 8 corners of the map box (roadmap.cpp:40-60), in the
+        reference's order (-x -y -z, -x -y +z, -x +y -z, ...)."""
+        o = self.options
+        h = [np.float32(o.robot_map_size_x / 2.0), np.float32(o.robot_map_size_y / 2.0),
+             np.float32(o.robot_map_size_z / 2.0)]
+        c = np.asarray(o.map_center, np.float32)
+        return [c + np.array([sx * h[0], sy * h[1], sz * h[2]], np.float32)
+                for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
+
+    # -- cereal binary archive of DRM::serialize (roadmap.h:103-110) --------
+    # Layout of cereal's BinaryOutputArchive: arithmetic values as raw
+    # little-endian bytes; container sizes as uint64 size tags; a map as its
+    # size then key, value pairs; a vector of int32 as its size then the raw
+    # block; a string as its size then its bytes; Eigen matrices as rows and
+    # cols (size_t each) then the column-major data (roadmap.h:32-53).
+    def Write(self, file_name):
+        import struct
+        out = bytearray()
+        u64 = lambda n: out.extend(struct.pack("<Q", n))  # noqa: E731
+        i32s = lambda v: out.extend(np.asarray(v, "<i4").tobytes())  # noqa: E731
+
+        def mat(m, rows, cols):
+            out.extend(struct.pack("<QQ", rows, cols))
+            out.extend(np.asarray(m, "<f4").reshape(rows, cols).T.tobytes())  # column-major
+
+        def vec_map(m):
+            u64(len(m))
+            for k in sorted(m):
+                i32s([k])
+                u64(len(m[k]))
+                i32s(m[k])
+
+        vec_map(self.collision_map)
+        u64(len(self._edge_collision_map))
+        for k in sorted(self._edge_collision_map):
+            i32s([k])
+            es = sorted(self._edge_collision_map[k])
+            u64(len(es))
+            for e in es:
+                u64(2)
+                i32s(e)
+        vec_map(self.node_adjacency_map)
+        u64(len(self.id_to_node_map))
+        for k in sorted(self.id_to_node_map):
+            v = np.asarray(self.id_to_node_map[k], np.float32).reshape(-1)
+            i32s([k])
+            mat(v, len(v), 1)
+        u64(len(self.id_to_pose_map))
+        for k in sorted(self.id_to_pose_map):
+            i32s([k])
+            mat(self.id_to_pose_map[k], 4, 4)
+        o = self.options
+        mat(np.asarray(o.map_center, np.float32), 3, 1)
+        i32s([o.nodes_processed_before_debug_statement])
+        out.extend(np.array([o.max_task_space_distance_between_nodes,
+                             o.max_configuration_distance_between_nodes,
+                             o.offline_voxel_resolution, o.edge_step_size, o.robot_map_size_x,
+                             o.robot_map_size_y, o.robot_map_size_z], "<f4").tobytes())
+        name = self.target_link_name_.encode()
+        u64(len(name))
+        out.extend(name)
+        i32s([self.target_link_index_])
+        with open(file_name, "wb") as f:
+            f.write(bytes(out))
+
+    def Read(self, file_name):
+        import struct
+        self.Clear()
+        with open(file_name, "rb") as f:
+            data = f.read()
+        pos = [0]
+
+        def take(n):
+            if pos[0] + n > len(data):
+                raise RuntimeError(f"truncated DRM archive: {file_name}")
+            b = data[pos[0]:pos[0] + n]
+            pos[0] += n
+            return b
+
+        u64 = lambda: struct.unpack("<Q", take(8))[0]  # noqa: E731
+        i32 = lambda: struct.unpack("<i", take(4))[0]  # noqa: E731
+        i32s = lambda n: np.frombuffer(take(4 * n), "<i4").astype(np.int32)  # noqa: E731
+
+        def mat():
+            r, c = struct.unpack("<QQ", take(16))
+            return np.frombuffer(take(4 * r * c), "<f4").reshape(c, r).T.astype(np.float32)
+
+        for _ in range(u64()):
+            k = i32()
+            self.collision_map[k] = i32s(u64()).tolist()
+        for _ in range(u64()):
+            k = i32()
+            s = set()
+            for _ in range(u64()):
+                s.add(tuple(i32s(u64()).tolist()))
+            self._edge_collision_map[k] = s
+        for _ in range(u64()):
+            k = i32()
+            self.node_adjacency_map[k] = i32s(u64()).tolist()
+        for _ in range(u64()):
+            k = i32()
+            self.id_to_node_map[k] = mat().reshape(-1)
+        for _ in range(u64()):
+            k = i32()
+            self.id_to_pose_map[k] = mat()
+        o = RoadmapOptions()
+        o.map_center = mat().reshape(-1)
+        o.nodes_processed_before_debug_statement = i32()
+        f = np.frombuffer(take(28), "<f4")
+        (o.max_task_space_distance_between_nodes, o.max_configuration_distance_between_nodes,
+         o.offline_voxel_resolution, o.edge_step_size, o.robot_map_size_x, o.robot_map_size_y,
+         o.robot_map_size_z) = [float(x) for x in f]
+        self.options = o
+        self.target_link_name_ = take(u64()).decode()
+        self.target_link_index_ = i32()
+
 
 def _voxel_map(mplant, robot_geometry_ids, nodes, edges, step, options):
     """{voxel: sorted item indices} from csd_voxel_collision_map."""
@@ -155,7 +272,28 @@ class RoadmapBuilder:
         return self._drm
 
     def reset(self):
+        """Clears the DRM and restores default options (roadmap_builder.cpp:127-130)."""
         self._drm.Clear()
+
+    def write(self, file_path):
+        self._drm.Write(file_path)
+
+    def read(self, file_path):
+        self._drm.Read(file_path)
+
+    def get_random_samples(self, num_samples):
+        """num_samples hit-and-run chains of 50 steps from the centre of the
+        joint-limit box, on the GPU sampler; the collision-free ones become
+        nodes 1, 2, ... where those ids are free (roadmap_builder.cpp:53-69)."""
+        dom = _c.HPolyhedron()
+        dom.MakeBox(self._lo, self._hi)
+        c = ((self._lo + self._hi) / 2).astype(np.float32)
+        q = np.asarray(_c.UniformSampleInHPolyhedraCuda([dom], c.reshape(-1, 1), num_samples, 50)[0])
+        free = np.asarray(_c.CheckCollisionFreeCuda(np.asfortranarray(q), self._mplant), bool)
+        k = 1
+        for j in np.nonzero(free)[0]:
+            self._raw.setdefault(k, q[:, j].astype(np.float32).copy())
+            k += 1
 
     def add_nodes_manual(self, nodes_to_add):
         """Adds the collision-free columns of (dim, N) that lie in the joint

@@ -156,3 +156,42 @@ def test_knn_pages(gpu, orc):
             d2 = (d2 + e * e).astype(np.float32)
         order = [j for j in np.lexsort((np.arange(500), d2)) if d2[j] > np.float32(1.1920928955078125e-07)]
         assert got[int(qn)] == order[:64], qn
+
+
+def test_random_samples_and_archive(gpu, orc, csd, tmp_path):
+    """get_random_samples (roadmap_builder.cpp:53-69): nodes 1..n are the
+    collision-free hit-and-run samples, inside the joint limits and free per
+    the oracle's exact check; the DRM then survives a Write/Read round trip."""
+    pl, mp, rg = _scene()
+    opts = csd.RoadmapOptions()
+    opts.max_configuration_distance_between_nodes = 4.0
+    opts.max_task_space_distance_between_nodes = 0.6
+    b = csd.RoadmapBuilder(pl, pl.getKinematicTree().get_link_name(7), opts)
+    b.get_random_samples(8000)
+    raw = dict(b._raw)
+    ids = sorted(raw)
+    assert 40 < len(ids) < 8000 and ids == list(range(1, len(ids) + 1))
+    X = np.array([raw[i] for i in ids], np.float32)
+    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
+    assert np.all((X >= lo - 1e-6) & (X <= hi + 1e-6))
+    op = orc.Plant(mp)
+    q = np.asfortranarray(X.T)
+    ref = orc.check(op, q, orc.RULE_GPU)
+    near = np.abs(orc.min_sd(op, q)) < 1e-5
+    assert np.all(ref.astype(bool) | near)
+    b.build_roadmap(4)
+    b.build_pose_map()
+    drm = b.get_drm()
+    ids = sorted(drm.id_to_node_map)  # only nodes the greedy connected (roadmap_builder.cpp:278)
+    assert ids and set(ids) <= set(raw)
+    assert all(np.array_equal(drm.id_to_node_map[i], raw[i]) for i in ids)
+    p = tmp_path / "r.drm"
+    b.write(str(p))
+    e = csd.DRM()
+    e.Read(str(p))
+    assert e.node_adjacency_map == {k: list(v) for k, v in drm.node_adjacency_map.items()}
+    for i in (ids[0], ids[-1]):
+        np.testing.assert_array_equal(e.id_to_node_map[i], drm.id_to_node_map[i])
+        np.testing.assert_array_equal(e.id_to_pose_map[i], drm.id_to_pose_map[i])
+    assert e.target_link_index_ == drm.target_link_index_

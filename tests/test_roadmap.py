@@ -76,3 +76,58 @@ def test_map_post_processing(csd, monkeypatch):
     np.testing.assert_array_equal(np.asarray(calls[1]), [[0, 1], [1, 2]])
     # voxel 4 blocks both edges in both directions (= all 4 adjacency entries): dropped
     assert drm.edge_collision_map == {3: [(2, 5), (5, 2)]}
+
+
+def test_drm_archive_layout(tmp_path, csd):
+    """An empty DRM in cereal's binary layout: five empty maps (uint64 0
+    each), options (map centre as rows, cols, 3 floats; one int32; seven
+    floats), the link name (uint64 length + bytes) and the link index."""
+    import struct
+    d = csd.DRM()
+    p = tmp_path / "empty.drm"
+    d.Write(str(p))
+    b = p.read_bytes()
+    name = b"TARGET_LINK_IS_UNINITIALIZED"
+    assert len(b) == 5 * 8 + 16 + 12 + 4 + 7 * 4 + 8 + len(name) + 4 == 140
+    assert b[:40] == bytes(40)
+    assert struct.unpack("<QQ3f", b[40:68]) == (3, 1, 0.0, 0.0, 0.0)
+    assert struct.unpack("<i", b[68:72]) == (1000,)
+    f = struct.unpack("<7f", b[72:100])
+    assert np.allclose(f, [0.25, 1.0, 0.05, 0.01, 1.0, 1.0, 1.0])
+    assert struct.unpack("<Q", b[100:108]) == (len(name),) and b[108:136] == name
+    assert struct.unpack("<i", b[136:140]) == (-1,)
+
+
+def test_drm_round_trip(tmp_path, csd):
+    d = csd.DRM()
+    d.collision_map = {7: [1, 2, 3], 2: [5]}
+    d._edge_collision_map = {4: {(1, 2), (2, 1)}}
+    d.node_adjacency_map = {1: [2], 2: [1]}
+    d.id_to_node_map = {1: np.array([0.1, 0.2], np.float32), 2: np.array([0.3, -0.4], np.float32)}
+    d.id_to_pose_map = {1: np.arange(16, dtype=np.float32).reshape(4, 4)}
+    d.options.map_center = np.array([0.5, -0.5, 1.0], np.float32)
+    d.options.offline_voxel_resolution = 0.02
+    d.target_link_name_, d.target_link_index_ = "tool", 9
+    p = tmp_path / "d.drm"
+    d.Write(str(p))
+    b = p.read_bytes()
+    # the first map: 2 entries, key 2 -> [5], key 7 -> [1, 2, 3] (keys in order)
+    assert b[:8] == (2).to_bytes(8, "little")
+    assert b[8:12] == (2).to_bytes(4, "little") and b[12:20] == (1).to_bytes(8, "little")
+    e = csd.DRM()
+    e.Read(str(p))
+    assert e.collision_map == {2: [5], 7: [1, 2, 3]}
+    assert e.edge_collision_map == {4: [(1, 2), (2, 1)]}
+    assert e.node_adjacency_map == d.node_adjacency_map
+    np.testing.assert_array_equal(e.id_to_node_map[2], d.id_to_node_map[2])
+    np.testing.assert_array_equal(e.id_to_pose_map[1], d.id_to_pose_map[1])
+    np.testing.assert_array_equal(e.options.map_center, d.options.map_center)
+    assert abs(e.options.offline_voxel_resolution - 0.02) < 1e-9
+    assert (e.target_link_name_, e.target_link_index_) == ("tool", 9)
+    c = e.GetWorkspaceCorners()
+    assert len(c) == 8
+    np.testing.assert_allclose(c[0], [0.0, -1.0, 0.5], atol=1e-7)
+    np.testing.assert_allclose(c[7], [1.0, 0.0, 1.5], atol=1e-7)
+    p.write_bytes(b[:-3])
+    with pytest.raises(RuntimeError, match="truncated"):
+        e.Read(str(p))
