@@ -393,6 +393,36 @@ def eizo_benchmarks(segments=2048):
     return res
 
 
+def large_batch_benchmark(plant, dp, stream, sizes=(262144, 1048576, 4194304), steps=20):
+    """The headline check on larger batches of the same workload (configs
+    seed 2000), each autotuned first: from a few hundred thousand
+    configurations on, the tuner picks the screened checker."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    res = {}
+    for n in sizes:
+        q = scenes.uniform_configs(2000, n, plant.getPositionLowerLimits(),
+                                   plant.getPositionUpperLimits())
+        dq = torch.from_numpy(np.ascontiguousarray(q)).to("cuda:0")
+        out = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+        shape = capi.tune_check(dp, dq, n, 1, n, out, stream)
+        for _ in range(3):
+            capi.check_collision_free(dp, dq, n, 1, n, out, stream)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(steps):
+            capi.check_collision_free(dp, dq, n, 1, n, out, stream)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / steps
+        res[str(n)] = {"configs_per_s": n / (ms * 1e-3), "ms_per_step": ms,
+                       "checker_shape": shape}
+        del dq, out
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -501,13 +531,17 @@ def main():
                    "num_configs": n, "num_collision_pairs": mp.num_collision_pairs,
                    "layout": "SoA [7][N] f32 in HBM", "collision_free_fraction": round(free_frac, 4),
                    "kernel": ("k_check<fast 8x4>" if split < 10 else
-                              "k_check2<fast 8x4> (self pairs, then obstacles for survivors)"),
+                              "k_check2<fast 8x4> (self pairs, then obstacles for survivors)"
+                              if split < 20 else
+                              "k_screen + k_finish <fast 8x4> (screened pairs, then the full "
+                              "check of the undecided)"),
                    "checker_shape": split,
                    "waves_per_64_configs": split if split < 10 else split - 10},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_VECTOR_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP32_VECTOR_PEAK_TFLOPS,
                      "traffic": traffic, "flops_per_config": f_cfg, "kernel_ms": kern_ms},
     }
+    line["large_batches"] = large_batch_benchmark(plant, dp, stream)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mp, q, args.cpu_seconds)
     if not args.no_eizo:

@@ -282,8 +282,8 @@ def synchronize(stream=None):
 def tune_check(plant, q, n, stride_n, stride_j, out, stream=None, force=0):
     """Autotunes the checker on this batch (see csd_tune_check) and returns
     the chosen shape (1, 2, 4: one pass with that many waves per 64
-    configurations; 11, 12, 14: two passes). force: record that shape
-    instead of timing."""
+    configurations; 11, 12, 14: two passes; 20: screened). force: record
+    that shape instead of timing."""
     s = _I32(force)
     check(lib().csd_tune_check(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
                                _stream(stream), C.byref(s)), "csd_tune_check")

@@ -8,7 +8,11 @@
 // (config, pair): plant/cuda_collision_checker.cu:16-85).
 #include <string.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "csd_collide.h"
+#include "csd_screen.h"
 
 namespace csd {
 
@@ -33,7 +37,7 @@ __global__ __launch_bounds__(kCheckBlock) void k_check(PlantImage P, const float
   extern __shared__ __attribute__((aligned(16))) float csd_smem[];
   const float *tab = stage_tables<VAR>(P, csd_smem);
   // idle lanes start "done" so they never hold the wave back
-  const bool col = collides_variant<VAR>(P, qa, V, R, !valid, tab, sub, S);
+  const bool col = collides_at<VAR>(P, qa, V, R, !valid, tab, sub, S);
   if constexpr (S == 1) {
     if (valid) out[gi] = col ? 0 : 1;
   } else {
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(kCheckBlock) void k_check2(PlantImage P, const floa
   const int64_t gi = base + loc;
   const bool valid = gi < n;
   const int64_t i = valid ? gi : n - 1;
-  bool col = collides_variant<VAR, false, 1>(P, QMem{q, i * sn, sj}, V, R, !valid, tab, sub, S1);
+  bool col = collides_at<VAR, false, 1>(P, QMem{q, i * sn, sj}, V, R, !valid, tab, sub, S1);
   if constexpr (S1 > 1) {
     part[threadIdx.x] = col ? 1 : 0;
     __syncthreads();
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(kCheckBlock) void k_check2(PlantImage P, const floa
     const bool ok = k < total;
     const int64_t j = base + surv[ok ? k : total - 1];
     const bool c =
-        collides_variant<VAR, false, 2>(P, QMem{q, j * sn, sj}, V, R, !ok, tab, wave, kWaves);
+        collides_at<VAR, false, 2>(P, QMem{q, j * sn, sj}, V, R, !ok, tab, wave, kWaves);
     part[threadIdx.x] = c ? 1 : 0;
     __syncthreads();
     if (wave == 0 && ok) {
@@ -297,7 +301,11 @@ int launch_check(const csd_plant *pl, const float *q, int64_t n, int64_t sn, int
   bool all_slots = fill_robot_list(pl, robot, nvox > 0 ? nr : 0, R, V, &rc);
   if (rc) return rc;
   const int var = check_variant(pl->img, nvox > 0 && !all_slots);
-  launch_k_check(var, check_split(pl, n, var), pl, q, n, sn, sj, V, R, out, st);
+  const int32_t range = pl->check_range[size_class(n)];
+  if (nvox == 0 && range > 0 && screen_supported(var))
+    launch_screened(var, pl, q, n, sn, sj, range, out, st);
+  else
+    launch_k_check(var, check_split(pl, n, var), pl, q, n, sn, sj, V, R, out, st);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }
@@ -333,15 +341,39 @@ int csd_tune_check(const csd_plant *plant, const float *q, int64_t n, int64_t st
   memset(&R, 0, sizeof(R));
   const VoxelArgs V{nullptr, 0, 0.f, 0, 0};
   const int var = check_variant(plant->img, false);
+  csd_plant *pl = const_cast<csd_plant *>(plant);
+  const int cls = size_class(n);
   int best = 1;
   const int forced = split ? *split : 0;
+  // survivor-workgroup ranges worth timing for a screen that leaves a
+  // fraction u undecided: about 32, 64 or 128 undecided per workgroup
+  auto ranges_for = [&](double u) {
+    std::vector<int32_t> r;
+    for (double t : {32.0, 64.0, 128.0}) {
+      int64_t v = (int64_t)(t / std::max(u, 1e-4));
+      v = std::min<int64_t>(std::max<int64_t>((v + 63) / 64 * 64, 512), 8192);
+      if (std::find(r.begin(), r.end(), (int32_t)v) == r.end()) r.push_back((int32_t)v);
+    }
+    return r;
+  };
   if (forced != 0) {  // record a given shape instead of timing
     if (!(forced == 1 || forced == 2 || forced == 4 || forced == 11 || forced == 12 ||
-          forced == 14))
-      return fail(CSD_ERR_INVALID_ARGUMENT, "split must be 1, 2, 4, 11, 12 or 14");
+          forced == 14 || forced == 20))
+      return fail(CSD_ERR_INVALID_ARGUMENT, "split must be 1, 2, 4, 11, 12, 14 or 20");
     best = var < kGeneric8 ? forced : 1;
-    const_cast<csd_plant *>(plant)->check_split[size_class(n)] = (int8_t)best;
-    launch_k_check(var, best, plant, q, n, stride_n, stride_j, V, R, out, st);
+    if (best == 20 && !screen_supported(var)) best = 1;
+    pl->check_range[cls] = 0;
+    if (best == 20) {
+      double u = 1.0;
+      rc = build_screen(var, pl, q, n, stride_n, stride_j, st, &u);
+      if (rc) return rc;
+      const std::vector<int32_t> rs = ranges_for(u);
+      pl->check_range[cls] = rs[rs.size() / 2];
+      launch_screened(var, pl, q, n, stride_n, stride_j, pl->check_range[cls], out, st);
+    } else {
+      pl->check_split[cls] = (int8_t)best;
+      launch_k_check(var, best, plant, q, n, stride_n, stride_j, V, R, out, st);
+    }
   } else if (var < kGeneric8) {
     hipEvent_t e0, e1;
     CSD_HIP(hipEventCreate(&e0));
@@ -351,26 +383,56 @@ int csd_tune_check(const csd_plant *plant, const float *q, int64_t n, int64_t st
       return hip_fail(e, "hipEventCreate");
     }
     float best_ms = INFINITY;
-    for (int s : {1, 2, 4, 11, 12, 14}) {
-      launch_k_check(var, s, plant, q, n, stride_n, stride_j, V, R, out, st);  // warm-up
+    // one timing of `launch` (a warm-up, then 5 launches between events)
+    auto time5 = [&](auto launch) {
+      launch();
       e = hipEventRecord(e0, st);
-      for (int k = 0; k < 5 && e == hipSuccess; ++k)
-        launch_k_check(var, s, plant, q, n, stride_n, stride_j, V, R, out, st);
+      for (int k = 0; k < 5 && e == hipSuccess; ++k) launch();
       if (e == hipSuccess) e = hipEventRecord(e1, st);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.f;
+      float ms = INFINITY;
       if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      return ms;
+    };
+    for (int s : {1, 2, 4, 11, 12, 14}) {
+      const float ms =
+          time5([&] { launch_k_check(var, s, plant, q, n, stride_n, stride_j, V, R, out, st); });
       if (e != hipSuccess) break;
       if (ms < best_ms) {
         best_ms = ms;
         best = s;
       }
     }
+    int32_t best_range = 0;
+    if (e == hipSuccess && screen_supported(var)) {
+      double u = 1.0;
+      rc = build_screen(var, pl, q, n, stride_n, stride_j, st, &u);
+      if (rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+      }
+      for (int32_t r : ranges_for(u)) {
+        const float ms = time5(
+            [&] { launch_screened(var, pl, q, n, stride_n, stride_j, r, out, st); });
+        if (e != hipSuccess) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_range = r;
+        }
+      }
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (e != hipSuccess) return hip_fail(e, "check autotuning");
-    const_cast<csd_plant *>(plant)->check_split[size_class(n)] = (int8_t)best;
-    launch_k_check(var, best, plant, q, n, stride_n, stride_j, V, R, out, st);
+    pl->check_range[cls] = best_range;
+    if (best_range > 0) {
+      best = 20;
+      launch_screened(var, pl, q, n, stride_n, stride_j, best_range, out, st);
+    } else {
+      pl->check_split[cls] = (int8_t)best;
+      launch_k_check(var, best, plant, q, n, stride_n, stride_j, V, R, out, st);
+    }
   }
   else {
     rc = launch_check(plant, q, n, stride_n, stride_j, nullptr, 0, 0.f, nullptr, 0, out, st);

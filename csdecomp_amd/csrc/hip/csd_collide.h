@@ -85,8 +85,8 @@ __device__ __forceinline__ void fast_spheres(const PlantImage &P, const Q &qa,
 // e.g. an idle tail lane) and is returned OR-ed with the result.
 // `tab`: for 64 sphere slots, the plant's contact-distance tables staged in
 // LDS by the kernel (stage_tables); unused otherwise. (sub, nsub): this call
-// tests only part `sub` of `nsub` of the work -- self-collision blocks of
-// bodies ba = sub mod nsub, obstacles and voxels o = sub mod nsub -- so that
+// tests only part `sub` of `nsub` of the work -- the body-pair blocks and
+// obstacles dealt to the parts in turn, voxels v = sub mod nsub -- so that
 // nsub waves can share one configuration (the caller ORs the parts).
 // PH selects the pair classes tested: bit 0 the moving x moving (self)
 // pairs, bit 1 the obstacles and voxels; a two-pass checker runs the cheap
@@ -115,14 +115,21 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // (Loops over bodies keep a constant trip count with uniform guards
   // instead of `break`, so they unroll fully for 16 bodies too and every
   // register index stays static.)
+  // Parts: the active body-pair blocks, then the obstacles, are dealt to the
+  // nsub parts in turn (rr: the part that takes the next one), so that every
+  // part gets a similar number of tests.
+  int rr = 0;
 #pragma unroll
   for (int ba = 0; ba < NB; ++ba) {
     if constexpr (!(PH & 1)) continue;
     const uint64_t act = P.self_active[ba];
-    if (act == 0 || ba % nsub != sub || wave_done<ANY>(col)) continue;
+    if (act == 0 || wave_done<ANY>(col)) continue;
 #pragma unroll
     for (int bb = ba + 1; bb < NB; ++bb) {
       if ((act >> bb) & 1) {
+        const bool mine = rr == sub;
+        if (++rr == nsub) rr = 0;
+        if (!mine) continue;
         if constexpr (kTab || kLds) {
           const float *r2 = (kLds ? tab : P.self_r2) + ((size_t)ba * NB + bb) * (KM * KM);
 #pragma unroll
@@ -149,12 +156,15 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
     }
   }
 
+  // the part that takes obstacle 0: obstacle o goes to part (rr + o) mod nsub
+  const int o0 = sub >= rr ? sub - rr : sub - rr + nsub;
+
   // moving sphere x world-fixed obstacle, tables: the record and its row of
   // contact distances obstacle_r2[o][s] (box r^2, sphere/capsule
   // (r + r_o)^2, -1 if the slot is not paired with o) are read once per
   // obstacle with a few wide scalar loads; the slot loop has no branch.
   if constexpr (kTab && (PH & 2)) {
-    for (int o = sub; o < P.num_obstacles; o += nsub) {
+    for (int o = o0; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
       float ov[16], r2[NS];
@@ -202,7 +212,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // its tests (one 16-byte read per 4 slots).
   if constexpr (kLds && (PH & 2)) {
     const float *orow = tab + NB * NB * KM * KM;
-    for (int o = sub; o < P.num_obstacles; o += nsub) {
+    for (int o = o0; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
       float ov[16];
@@ -245,13 +255,13 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // r * r (box) and (r + r_o) * (r + r_o) (sphere, capsule); a slot not
   // paired with the obstacle gets -1, which never reports a hit, so there is
   // no per-slot branch. Slots of absent bodies are skipped per body.
-  if (kMask && (PH & 2) && sub < P.num_obstacles) {
-    int tn = P.obstacle_type[sub];
+  if (kMask && (PH & 2) && o0 < P.num_obstacles) {
+    int tn = P.obstacle_type[o0];
     float ovn[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) ovn[k] = P.obstacles[sub].v[k];
-    uint64_t m0n = P.obstacle_mask[2 * sub], m1n = P.obstacle_mask[2 * sub + 1];
-    for (int o = sub; o < P.num_obstacles; o += nsub) {
+    for (int k = 0; k < 16; ++k) ovn[k] = P.obstacles[o0].v[k];
+    uint64_t m0n = P.obstacle_mask[2 * o0], m1n = P.obstacle_mask[2 * o0 + 1];
+    for (int o = o0; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = tn;
       float ov[16];
@@ -499,6 +509,21 @@ __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &q
   else if (sub != 0) return col;
   else if constexpr (VAR == kGeneric8) return collides_generic<8, ANY>(P, qa, V, R, col);
   else return collides_generic<16, ANY>(P, qa, V, R, col);
+}
+
+// collides_variant on configuration i of a strided array; fast shapes load
+// the joint values into registers first (one memory wait before FK)
+template <int VAR, bool ANY = false, int PH = 3>
+__device__ __forceinline__ bool collides_at(const PlantImage &P, const QMem &qm,
+                                            const VoxelArgs &V, const RobotGeomList &R,
+                                            bool col, const float *tab, int sub = 0,
+                                            int nsub = 1) {
+  if constexpr (VAR == kFast8x4 || VAR == kFast8x8)
+    return collides_variant<VAR, ANY, PH>(P, preload_q<8>(qm, P.dof), V, R, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast16x4 || VAR == kFast16x8)
+    return collides_variant<VAR, ANY, PH>(P, preload_q<16>(qm, P.dof), V, R, col, tab, sub, nsub);
+  else
+    return collides_variant<VAR, ANY, PH>(P, qm, V, R, col, tab, sub, nsub);
 }
 
 // host: launch KERNEL<VAR> for the runtime variant

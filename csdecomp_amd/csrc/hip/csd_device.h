@@ -118,6 +118,17 @@ struct QReg {
   __device__ __forceinline__ float operator()(int j) const { return v[j]; }
 };
 
+// The first min(dof, N) joint values of a configuration in registers, all
+// loads issued before FK needs any of them (FK then waits for memory once,
+// not once per joint). Fast-path plants have dof <= N bodies.
+template <int N>
+__device__ __forceinline__ QReg<N> preload_q(const QMem &qm, int dof) {
+  float a[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) a[j] = j < dof ? qm(j) : 0.f;
+  return QReg<N>(a);
+}
+
 // Forward kinematics of all bodies of one configuration.
 template <int NB, class Q>
 __device__ __forceinline__ void fk_bodies(const PlantImage &P, const Q &qa, BodyFrames<NB> &B) {

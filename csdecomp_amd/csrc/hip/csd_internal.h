@@ -101,6 +101,30 @@ struct PlantImage {
   const int32_t *geom_slot;       // [G] slot of moving sphere geometry, -1 else
 };
 
+// Screen of the fast checker: a subset of the plant's pair tests that finds
+// most collisions of a workload early. Configurations the screen leaves
+// undecided get the full check. Pair tests are the full check's own (same
+// contact distances, same arithmetic), so the screened result is the
+// unscreened result. The screen is a program in device memory that each
+// workgroup copies to LDS (16-byte words, layout for NB bodies of KM slots,
+// NS = NB * KM):
+//   uint64 self_active[NB]       bit bb: body pair (ba, bb) has screened pairs
+//   uint64 self_rows[NB][NB]     bit ka: sphere ka of body ba has screened
+//                                pairs with body bb
+//   float  self_r2[NB][NB][KM][KM]  contact distances of screened pairs
+//                                (the plant's), -1 for the others
+//   then num_obst records of {float ov[16]; float r2[NS]; uint64 bodies;
+//   int32 type; int32 pad}: the obstacle, its contact distances with the
+//   screened slots (-1 for the others) and the bodies owning them
+constexpr int kScreenMaxObst = 32;
+struct ScreenImage {
+  int32_t num_obst;    // obstacles with screened slots
+  int32_t num_tests;   // screened pair tests in total
+  int32_t prog_words;  // program size in 16-byte words
+  int32_t pad;
+  const uint4 *prog;
+};
+
 }  // namespace csd
 
 struct csd_plant {
@@ -114,6 +138,12 @@ struct csd_plant {
   // waves sharing one configuration in the checker, tuned per batch-size
   // class (index: bit length of n); 0 = not tuned
   int8_t check_split[64] = {0};
+  // screened checking, chosen by the autotuner per batch-size class:
+  // check_range[c] > 0 is the configuration range of one survivor workgroup
+  // (0 = unscreened); the screen itself lives in screen_arena
+  int32_t check_range[64] = {0};
+  csd::ScreenImage screen{};
+  void *screen_arena = nullptr;
 };
 
 namespace csd {

@@ -409,6 +409,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   if (kmax == 0) fast = false;
   int nbt = NB <= 8 ? 8 : 16;
   if (fast && nbt * kmax > kFastMaxSlots) fast = false;
+  if (fast && dof > nbt) fast = false;  // the fast kernels hold q in nbt registers
   const int nslots = fast ? nbt * kmax : 0;
   std::vector<float4> slot_sphere(std::max(nslots, 1), make_float4(0, 0, 0, -1));
   std::vector<int32_t> geom_slot(std::max(G, 1), -1);
@@ -598,6 +599,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
 int csd_plant_destroy(csd_plant *plant) {
   if (!plant) return CSD_OK;
   if (plant->arena) hipFree(plant->arena);
+  if (plant->screen_arena) hipFree(plant->screen_arena);
   delete plant;
   return CSD_OK;
 }

@@ -127,10 +127,19 @@ int csd_check_collision_free(const csd_plant *plant, const float *q, int64_t n,
  *   11, 12, 14 -- two passes: FK + robot self pairs with 1, 2 or 4 waves per
  *                 64 configurations, then obstacles and voxels only for the
  *                 configurations still free, compacted per workgroup (pays
- *                 when most configurations collide with the robot itself).
+ *                 when most configurations collide with the robot itself);
+ *   20         -- screened (plants of at most 4 sphere slots per body): the
+ *                 pair tests that hit most often in an even sample of up to
+ *                 4,096 of the caller's configurations (greedy set cover)
+ *                 run first for every configuration; the configurations
+ *                 they leave undecided get the full check, compacted per
+ *                 workgroup with 8 waves sharing each 64. The flags are the
+ *                 unscreened ones bit for bit; only the speed depends on
+ *                 the sample. Checks with voxels stay unscreened.
  * out receives the flags. Synchronizes `stream`. *split (may be NULL)
- * reports the choice; if it holds one of the six shapes on entry, that shape
- * is recorded without timing. Generic-path plants always use shape 1. */
+ * reports the choice; if it holds one of the seven shapes on entry, that
+ * shape is recorded without timing (20 still samples q for its screen).
+ * Generic-path plants always use shape 1. */
 int csd_tune_check(const csd_plant *plant, const float *q, int64_t n,
                    int64_t stride_n, int64_t stride_j, uint8_t *out, void *stream,
                    int32_t *split);

@@ -261,7 +261,7 @@ def test_split_checker_and_tuner(gpu, orc):
         ref = orc.check(op, q, orc.RULE_GPU)
         assert_flags_match(got, ref, orc.min_sd(op, q), f"n={n}")
         s = capi.tune_check(dp, dq, n, 1, n, out)
-        assert s in (1, 2, 4, 11, 12, 14)
+        assert s in (1, 2, 4, 11, 12, 14, 20)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(out.cpu().numpy(), got)
         capi.check_collision_free(dp, dq, n, 1, n, out)  # uses the tuned split
@@ -320,3 +320,74 @@ def test_two_pass_checker(gpu, orc, scene):
         assert_flags_match(out.cpu().numpy(), vref, vsd, f"{scene} voxels shape {shape}")
     with pytest.raises(capi.CsdError):
         capi.tune_check(dp, dq, n, 1, n, out, force=3)
+
+
+@pytest.mark.parametrize("scene", ["arm16", "arm64", "dual"])
+def test_screened_checker(gpu, orc, scene):
+    """Screened checking (tuner shape 20): FK and a greedy subset of the pair
+    tests for every configuration, then the full check of the undecided
+    ones. Flags equal the one-pass checker's bit for bit (and the oracle's
+    outside the contact band) for ragged, tiny and strided batches; voxel
+    checks keep the unscreened path."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    from conftest import assert_flags_match
+    pl = {"arm16": lambda: scenes.arm_scene(1000, 16, 0),
+          "arm64": lambda: scenes.arm_scene(1001, 48, 16),
+          "dual": lambda: scenes.dual_arm_scene(1002, 72, 24)}[scene]()
+    mp = pl.getMinimalPlant()
+    dp = capi.DevicePlant(mp)
+    op = orc.Plant(mp)
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    n = 20011
+    q = scenes.uniform_configs(6, n, lo, hi)
+    dq = torch.from_numpy(np.ascontiguousarray(q)).to(gpu)
+    out = torch.empty(n, dtype=torch.uint8, device=gpu)
+    assert capi.tune_check(dp, dq, n, 1, n, out, force=1) == 1
+    torch.cuda.synchronize()
+    base = out.cpu().numpy()
+    out.fill_(7)
+    shape = capi.tune_check(dp, dq, n, 1, n, out, force=20)
+    assert shape in (1, 20)
+    if scene == "arm16":
+        assert shape == 20  # 8 bodies x 4 sphere slots: screened
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert set(np.unique(got)) <= {0, 1}
+    np.testing.assert_array_equal(got, base)
+    assert_flags_match(got, orc.check(op, q, orc.RULE_GPU), orc.min_sd(op, q), f"{scene} screened")
+    out.fill_(7)
+    capi.check_collision_free(dp, dq, n, 1, n, out)  # the recorded shape
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), base)
+    # another batch of the same size class, and configurations as rows
+    # (stride_n = dof, stride_j = 1)
+    m = 16411
+    qa = torch.from_numpy(np.ascontiguousarray(q[:, :m].T)).to(gpu)
+    o2 = torch.full((m,), 7, dtype=torch.uint8, device=gpu)
+    capi.check_collision_free(dp, qa, m, q.shape[0], 1, o2)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(o2.cpu().numpy(), base[:m])
+    # batches smaller than one survivor workgroup's range
+    for k in (1, 63, 700):
+        ok = torch.full((k,), 7, dtype=torch.uint8, device=gpu)
+        dk = torch.from_numpy(np.ascontiguousarray(q[:, :k])).to(gpu)
+        capi.tune_check(dp, dk, k, 1, k, ok, force=20)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(ok.cpu().numpy(), base[:k])
+    # the timed tuner may keep the screen or not; the flags stay the same
+    out.fill_(7)
+    assert capi.tune_check(dp, dq, n, 1, n, out) in (1, 2, 4, 11, 12, 14, 20)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), base)
+    if scene == "arm16":
+        rng = np.random.default_rng(9)
+        vox = torch.from_numpy(rng.uniform(-0.8, 0.8, (40, 3)).astype(np.float32)).to(gpu)
+        rg = list(pl.getRobotGeometryIds())
+        capi.tune_check(dp, dq, n, 1, n, out, force=20)
+        out.fill_(7)
+        capi.check_collision_free_voxels(dp, dq, n, 1, n, vox, 40, 0.03, rg, out)
+        torch.cuda.synchronize()
+        vnp = vox.cpu().numpy().T
+        assert_flags_match(out.cpu().numpy(), orc.check_voxels(op, q, vnp, 0.03, rg),
+                           orc.min_sd(op, q, vnp, 0.03, rg), "arm16 voxels after screening")
