@@ -423,6 +423,14 @@ def large_batch_benchmark(plant, dp, stream, sizes=(262144, 1048576, 4194304), s
     return res
 
 
+def _large_batches_leg():
+    import torch
+    from csdecomp_amd import capi, scenes
+    plant = scenes.arm_scene(1000, 16, 0)
+    dp = capi.DevicePlant(plant.getMinimalPlant())
+    return large_batch_benchmark(plant, dp, torch.cuda.current_stream())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,7 +444,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions", "distance_field",
-                                      "voxel_maps"],
+                                      "voxel_maps", "large_batches"],
                     help="only run this secondary leg and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.leg:
@@ -445,7 +453,8 @@ def main():
         fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments), "mlp": mlp_benchmark,
               "roadmap": roadmap_benchmark, "edit_regions": edit_regions_benchmark,
               "distance_field": distance_field_benchmark,
-              "voxel_maps": voxel_maps_benchmark}[args.leg]
+              "voxel_maps": voxel_maps_benchmark,
+              "large_batches": _large_batches_leg}[args.leg]
         print(json.dumps({args.leg: fn()}), flush=True)
         return
     if args.copy_calibration:

@@ -346,12 +346,13 @@ int csd_tune_check(const csd_plant *plant, const float *q, int64_t n, int64_t st
   int best = 1;
   const int forced = split ? *split : 0;
   // survivor-workgroup ranges worth timing for a screen that leaves a
-  // fraction u undecided: about 32, 64 or 128 undecided per workgroup
+  // fraction u undecided: about 32 to 512 undecided per workgroup (1 batch
+  // of 64 with 8 waves sharing its pair tests up to 8 batches of 1 wave)
   auto ranges_for = [&](double u) {
     std::vector<int32_t> r;
-    for (double t : {32.0, 64.0, 128.0}) {
+    for (double t : {32.0, 64.0, 128.0, 256.0, 512.0}) {
       int64_t v = (int64_t)(t / std::max(u, 1e-4));
-      v = std::min<int64_t>(std::max<int64_t>((v + 63) / 64 * 64, 512), 8192);
+      v = std::min<int64_t>(std::max<int64_t>((v + 63) / 64 * 64, 512), 16384);
       if (std::find(r.begin(), r.end(), (int32_t)v) == r.end()) r.push_back((int32_t)v);
     }
     return r;
@@ -368,7 +369,7 @@ int csd_tune_check(const csd_plant *plant, const float *q, int64_t n, int64_t st
       rc = build_screen(var, pl, q, n, stride_n, stride_j, st, &u);
       if (rc) return rc;
       const std::vector<int32_t> rs = ranges_for(u);
-      pl->check_range[cls] = rs[rs.size() / 2];
+      pl->check_range[cls] = rs[std::min<size_t>(1, rs.size() - 1)];
       launch_screened(var, pl, q, n, stride_n, stride_j, pl->check_range[cls], out, st);
     } else {
       pl->check_split[cls] = (int8_t)best;

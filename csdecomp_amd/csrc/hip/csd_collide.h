@@ -162,7 +162,10 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // moving sphere x world-fixed obstacle, tables: the record and its row of
   // contact distances obstacle_r2[o][s] (box r^2, sphere/capsule
   // (r + r_o)^2, -1 if the slot is not paired with o) are read once per
-  // obstacle with a few wide scalar loads; the slot loop has no branch.
+  // obstacle with a few wide loads (scalar loads for one part; with parts
+  // derived from the wave index the compiler cannot prove o uniform and
+  // issues vector loads, which all lanes share through the L1 -- forcing
+  // scalar loads there measured slower); the slot loop has no branch.
   if constexpr (kTab && (PH & 2)) {
     for (int o = o0; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;

@@ -5,7 +5,7 @@
 namespace csd {
 // FK + screened pair tests for every configuration, then the full check of
 // the undecided ones, `range` configurations per survivor workgroup
-// (a multiple of 64, at most 8192).
+// (a multiple of 64, at most 16384).
 void launch_screened(int var, const csd_plant *pl, const float *q, int64_t n, int64_t sn,
                      int64_t sj, int32_t range, uint8_t *out, hipStream_t st);
 int screen_range_ok(int32_t range);

@@ -27,7 +27,7 @@ namespace csd {
 constexpr int kScreenBlock = 256;
 constexpr int kFinishBlock = 512;    // 8 waves, 2 per SIMD (<= 256 VGPRs)
 constexpr int kFinishWaves = kFinishBlock / 64;
-constexpr int kMaxRange = 8192;      // configurations per survivor workgroup
+constexpr int kMaxRange = 16384;     // configurations per survivor workgroup
 
 template <int VAR>
 struct FastShape {
