@@ -440,6 +440,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eizo", action="store_true", help="skip the EI-ZO regions/s legs")
+    ap.add_argument("--no-large", action="store_true", help="skip the large-batch checker leg")
     ap.add_argument("--eizo-segments", type=int, default=2048)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
@@ -550,7 +551,8 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP32_VECTOR_PEAK_TFLOPS,
                      "traffic": traffic, "flops_per_config": f_cfg, "kernel_ms": kern_ms},
     }
-    line["large_batches"] = large_batch_benchmark(plant, dp, stream)
+    if not args.no_large:
+        line["large_batches"] = large_batch_benchmark(plant, dp, stream)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mp, q, args.cpu_seconds)
     if not args.no_eizo:
