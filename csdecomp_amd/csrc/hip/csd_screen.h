@@ -8,7 +8,6 @@ namespace csd {
 // (a multiple of 64, at most 16384).
 void launch_screened(int var, const csd_plant *pl, const float *q, int64_t n, int64_t sn,
                      int64_t sj, int32_t range, uint8_t *out, hipStream_t st);
-int screen_range_ok(int32_t range);
 // screening applies to fast-path plants of 4 sphere slots per body
 bool screen_supported(int var);
 // Chooses the plant's screen from the pair-test hit bits of up to 4,096

@@ -267,6 +267,7 @@ int sfail(int code, const char *msg) {
 template <int VAR>
 void launch_finish(const csd_plant *pl, const float *q, int64_t n, int64_t sn, int64_t sj,
                    int32_t range, uint8_t *out, hipStream_t st) {
+  range = std::min(std::max(range, 64), kMaxRange);  // the LDS list holds kMaxRange entries
   const size_t lds = check_lds_bytes(pl->img, VAR);
   const dim3 g((unsigned)((n + range - 1) / range));
   hipLaunchKernelGGL((k_screen<VAR>), dim3((unsigned)((n + kScreenBlock - 1) / kScreenBlock)),
@@ -276,10 +277,6 @@ void launch_finish(const csd_plant *pl, const float *q, int64_t n, int64_t sn, i
                      range, out);
 }
 }  // namespace
-
-int screen_range_ok(int32_t range) {
-  return range > 0 && range <= kMaxRange && range % 64 == 0;
-}
 
 // the 8-slot-per-body shapes hold 64 or 128 sphere centres per lane and
 // would spill in 512-thread survivor workgroups
