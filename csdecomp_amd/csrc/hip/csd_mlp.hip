@@ -40,8 +40,21 @@ __host__ __device__ __forceinline__ uint16_t f2bf(float x) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// float -> bf16 with the gfx950 conversion instruction (round to nearest
+// even, as f2bf, for the finite activations of the inference kernel)
+__device__ __forceinline__ uint16_t f2bf_hw(float x) {
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
+}
 
 __device__ __forceinline__ float silu(float z) { return z / (1.f + expf(-z)); }
+// SiLU for activations that are rounded to bf16 right after (8-bit
+// mantissa): hardware exp2 and reciprocal (about 1 ulp in fp32) instead of
+// the accurate expf and IEEE division, 5 instructions instead of ~25. For
+// z -> -inf, exp2 overflows to inf and the reciprocal to 0 (SiLU -> -0).
+__device__ __forceinline__ float silu_bf(float z) {
+  const float e = __builtin_amdgcn_exp2f(-1.44269504088896341f * z);
+  return z * __builtin_amdgcn_rcpf(1.f + e);
+}
 __device__ __forceinline__ float dsilu(float z) {
   const float s = 1.f / (1.f + expf(-z));
   return s * (1.f + z * (1.f - s));
@@ -85,23 +98,25 @@ struct MlpDev {
 // Fused inference: one wave computes 32 rows through all layers. Layer
 // outputs stay in registers; each 32x32 output tile goes through a small
 // per-wave LDS tile to become the next layer's A fragments. Weight chunks
-// (256 rows x 32 k) are staged in LDS once per block for its 4 waves.
+// (256 rows x 64 k; layer 1: its 32) are staged in LDS once per block for
+// its 4 waves; two blocks share a CU (2 waves per SIMD).
 // ---------------------------------------------------------------------------
-constexpr int kChunk = 32;
-constexpr int kChunkPad = kChunk + 8;  // 80-byte rows: 16-byte aligned, fewer conflicts
+constexpr int kChunk = 64;             // k per weight chunk (layer 1: its 32)
+constexpr int kChunkPad = kChunk + 8;  // 144-byte rows: 16-byte aligned, fewer conflicts
 constexpr int kTilePad = 32 + 8;
-
-__device__ __forceinline__ void load_chunk(uint16_t *sw, const uint16_t *W, int K, int k0) {
-  // 256 rows x 32 bf16 = 1024 16-byte pieces, 4 per thread
+// 256 rows x cw bf16 of W (row stride K) from column k0 into LDS rows of
+// kChunkPad: 32 * cw 16-byte pieces, cw / 8 per thread
+__device__ __forceinline__ void load_chunk(uint16_t *sw, const uint16_t *W, int K, int k0,
+                                           int cw) {
+  const int per_row = cw / 8;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < cw / 8; ++r) {
     const int piece = threadIdx.x + r * kBlock;
-    const int row = piece >> 2, part = piece & 3;
+    const int row = piece / per_row, part = piece % per_row;
     *reinterpret_cast<uint4 *>(sw + row * kChunkPad + part * 8) =
         *reinterpret_cast<const uint4 *>(W + (size_t)row * K + k0 + part * 8);
   }
 }
-
 // raw inputs of row r: q[r * sn + k * sj]
 struct StridedInput {
   const float *q;
@@ -147,7 +162,7 @@ struct SegmentInput {
 };
 
 template <class In>
-__global__ __launch_bounds__(kBlock) void k_mlp_infer(MlpDev M, In in, int64_t n,
+__global__ __launch_bounds__(kBlock, 2) void k_mlp_infer(MlpDev M, In in, int64_t n,
                                                       float *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) uint16_t sw[kW * kChunkPad];
   __shared__ __attribute__((aligned(16))) uint16_t st[4][32 * kTilePad];
@@ -182,13 +197,14 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(MlpDev M, In in, int64_t n
     f32x16 acc[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) acc[t] = f32x16{};
+    const int cw = K < kChunk ? K : kChunk;
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += kChunk) {
+    for (int k0 = 0; k0 < K; k0 += cw) {
       __syncthreads();
-      load_chunk(sw, M.wb[l], K, k0);
+      load_chunk(sw, M.wb[l], K, k0, cw);
       __syncthreads();
 #pragma unroll
-      for (int s = 0; s < kChunk / 16; ++s) {
+      for (int s = 0; s < cw / 16; ++s) {
         const bf16x8 a = af[(k0 >> 4) + s];
 #pragma unroll
         for (int t = 0; t < 8; ++t)
@@ -205,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(MlpDev M, In in, int64_t n
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const int r = 8 * (i >> 2) + 4 * hr + (i & 3);
-          tile[r * kTilePad + c] = f2bf(silu(acc[t][i] + bc));
+          tile[r * kTilePad + c] = f2bf_hw(silu_bf(acc[t][i] + bc));
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): tile written (single wave)
         __builtin_amdgcn_wave_barrier();
@@ -224,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(MlpDev M, In in, int64_t n
         const float wc = bf2f(M.w5b[t * 32 + c]);
 #pragma unroll
         for (int i = 0; i < 16; ++i)
-          part[i] = fmaf(bf2f(f2bf(silu(acc[t][i] + bc))), wc, part[i]);
+          part[i] = fmaf(bf2f(f2bf_hw(silu_bf(acc[t][i] + bc))), wc, part[i]);
       }
     }
   }
