@@ -40,6 +40,10 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
   constexpr bool LDS = SamplerShape<DPL>::LDS;
   extern __shared__ float csd_smem[];
   const int slot = blockIdx.y;
+  if (D.ctl) {  // device-resident round: slots past the active count idle
+    if (slot >= (int)D.ctl[1]) return;
+    round = D.ctl[2];
+  }
   const int r = D.active[slot];
   const uint32_t N = D.nsamp[r];
   if (blockIdx.x * kSampleBlock >= N) return;  // whole block past this region's samples
@@ -79,6 +83,10 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
                                                              RobotGeomList R, EizoDev D, int dim,
                                                              int nblk, uint32_t M) {
   const int slot = blockIdx.y;
+  if (D.ctl) {
+    if (slot >= (int)D.ctl[1]) return;
+    M = D.ctl[3];
+  }
   const int r = D.active[slot];
   const uint32_t N = D.nsamp[r];
   if (blockIdx.x * kSampleBlock >= N) {
@@ -133,6 +141,11 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
 __global__ __launch_bounds__(256) void k_eizo_compact(EizoDev D, int dim, int nblk, uint32_t M,
                                                       float thresh, int U, int test) {
   const int slot = blockIdx.x;
+  if (D.ctl) {
+    if (slot >= (int)D.ctl[1]) return;
+    M = D.ctl[3];
+    thresh = __uint_as_float(D.ctl[4]);
+  }
   const int r = D.active[slot];
   const uint32_t N = D.nsamp[r];
   __shared__ uint32_t s_off[1024];
@@ -264,6 +277,7 @@ template <int VAR, int DP>
 __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, VoxelArgs V,
                                                               RobotGeomList R, EizoDev D,
                                                               int dim, int U, int steps) {
+  if (D.ctl && blockIdx.y >= D.ctl[1]) return;
   const int r = D.active[blockIdx.y];
   if (D.status[r] != 0) return;
   const int u = D.u[r];
@@ -295,6 +309,53 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
   D.dist[(size_t)r * U + i] = sqrt(dd);
 }
 
+// Device-resident round start (one workgroup): the regions still running
+// (status 0), in region order, become this round's slots, with this round's
+// sample count max(M_k, budget) and round count; ctl takes the round number,
+// M_k, the test threshold and the slot count for the round's other stages.
+// Past the last round the slot count is 0 and every stage idles.
+constexpr int kBeginBlock = 1024;
+
+__global__ __launch_bounds__(kBeginBlock) void k_eizo_round_begin(EizoDev D, int Rn,
+                                                                  uint32_t rounds) {
+  __shared__ uint32_t s_w[kBeginBlock / 64];
+  __shared__ uint32_t s_base;
+  const uint32_t it = D.ctl[0];
+  const bool live = it < rounds;
+  const uint32_t M = live ? D.mtab[it] : 0u;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  for (int r0 = 0; r0 < Rn; r0 += kBeginBlock) {
+    const int r = r0 + (int)threadIdx.x;
+    const bool act = live && r < Rn && D.status[r] == 0;
+    const uint64_t m = __ballot(act);
+    if (lane == 0) s_w[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = s_base;
+    for (int w = 0; w < wave; ++w) off += s_w[w];
+    if (act) {
+      D.active[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = r;
+      D.nsamp[r] = max(M, D.budget[r]);
+      D.iters[r] += 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < kBeginBlock / 64; ++w) t += s_w[w];
+      s_base += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    D.ctl[1] = s_base;
+    D.ctl[2] = it;
+    D.ctl[3] = M;
+    D.ctl[4] = live ? __float_as_uint(D.ttab[it]) : 0u;
+    if (live) D.ctl[0] = it + 1;
+  }
+}
+
 }  // namespace csd
 
 // ---------------------------------------------------------------------------
@@ -317,6 +378,12 @@ struct csd_eizo {
   int keep = 64, line_points = 4;
   float *pred = nullptr;  // [slots][cand][L] surrogate outputs
   size_t pred_bytes = 0;
+  // device-resident rounds (csd_eizo_set_graph): host status read every
+  // graph_sync rounds, 0 for host-driven rounds
+  int graph_sync = 0;
+  hipStream_t gs = nullptr;
+  uint32_t *ctl = nullptr, *mtab = nullptr;
+  float *ttab = nullptr;
 };
 
 using namespace csd;
@@ -328,6 +395,38 @@ int dalloc(csd_eizo *w, T **p, size_t count) {
   CSD_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
   w->owned.push_back(q);
   *p = (T *)q;
+  return CSD_OK;
+}
+
+// Samples of the unadaptive test in round `it`
+// (cuda_edge_inflation_zero_order.cu:113-119, float arithmetic).
+uint32_t unadaptive_samples(const csd_eizo_options &o, uint32_t it) {
+  const float delta_k = (float)(6 * o.delta / (M_PI * M_PI * (it + 1) * (it + 1)));
+  return (uint32_t)ceilf(2 * logf(1 / delta_k) / (o.epsilon * o.tau * o.tau));
+}
+
+// Grows the staging segments and block counters to na regions x nblk blocks.
+int grow_stage(csd_eizo *w, int na, int nblk) {
+  EizoDev &D = w->D;
+  const size_t stage_need =
+      std::max((size_t)na * nblk * kSampleBlock, (size_t)na * 256) * w->dim * sizeof(float);
+  if (stage_need > w->stage_bytes) {
+    if (D.stage) hipFree(D.stage);
+    D.stage = nullptr;
+    w->stage_bytes = 0;
+    CSD_HIP(hipMalloc(&D.stage, stage_need));
+    w->stage_bytes = stage_need;
+  }
+  const size_t blk_need = (size_t)na * nblk;
+  if (blk_need > w->blk_entries) {
+    if (D.blk_cnt) hipFree(D.blk_cnt);
+    if (D.blk_first) hipFree(D.blk_first);
+    D.blk_cnt = D.blk_first = nullptr;
+    w->blk_entries = 0;
+    CSD_HIP(hipMalloc(&D.blk_cnt, blk_need * 4));
+    CSD_HIP(hipMalloc(&D.blk_first, blk_need * 4));
+    w->blk_entries = blk_need;
+  }
   return CSD_OK;
 }
 }  // namespace
@@ -382,6 +481,9 @@ int csd_eizo_create(const csd_plant *plant, const int32_t *robot_geoms, int32_t 
   if (!rc) rc = dalloc(w, &D.samples, R);
   if (!rc) rc = dalloc(w, &D.collisions, R);
   if (!rc) rc = dalloc(w, &w->vox, std::max(max_voxels, 1) * 3);
+  if (!rc) rc = dalloc(w, &w->ctl, 8);
+  if (!rc) rc = dalloc(w, &w->mtab, std::max<uint32_t>(options->max_iterations, 1));
+  if (!rc) rc = dalloc(w, &w->ttab, std::max<uint32_t>(options->max_iterations, 1));
   if (rc) {
     csd_eizo_destroy(w);
     return rc;
@@ -397,6 +499,7 @@ int csd_eizo_destroy(csd_eizo *w) {
   if (w->D.blk_cnt) hipFree(w->D.blk_cnt);
   if (w->D.blk_first) hipFree(w->D.blk_first);
   if (w->pred) hipFree(w->pred);
+  if (w->gs) hipStreamDestroy(w->gs);
   delete w;
   return CSD_OK;
 }
@@ -417,6 +520,15 @@ int csd_eizo_set_guidance(csd_eizo *w, uint32_t pilot_samples, float pilot_facto
     w->keep = keep_particles;
     w->line_points = line_points;
   }
+  return CSD_OK;
+}
+
+int csd_eizo_set_graph(csd_eizo *w, int32_t rounds_per_sync) {
+  if (!w) return fail(CSD_ERR_INVALID_ARGUMENT, "null workspace");
+  if (rounds_per_sync < 0 || rounds_per_sync > 64)
+    return fail(CSD_ERR_INVALID_ARGUMENT, "rounds_per_sync must be 0..64");
+  if (rounds_per_sync && !w->gs) CSD_HIP(hipStreamCreateWithFlags(&w->gs, hipStreamNonBlocking));
+  w->graph_sync = rounds_per_sync;
   return CSD_OK;
 }
 
@@ -494,25 +606,8 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     for (int r : slots) nmax = std::max(nmax, nsamp[r]);
     const int nblk = (int)((nmax + kSampleBlock - 1) / kSampleBlock);
     if (nblk > 1024) return fail(CSD_ERR_CAPACITY, "more than 262144 samples in one round");
-    const size_t stage_need =
-        std::max((size_t)na * nblk * kSampleBlock, (size_t)na * 256) * dim * sizeof(float);
-    if (stage_need > w->stage_bytes) {
-      if (D.stage) hipFree(D.stage);
-      D.stage = nullptr;
-      w->stage_bytes = 0;
-      CSD_HIP(hipMalloc(&D.stage, stage_need));
-      w->stage_bytes = stage_need;
-    }
-    const size_t blk_need = (size_t)na * nblk;
-    if (blk_need > w->blk_entries) {
-      if (D.blk_cnt) hipFree(D.blk_cnt);
-      if (D.blk_first) hipFree(D.blk_first);
-      D.blk_cnt = D.blk_first = nullptr;
-      w->blk_entries = 0;
-      CSD_HIP(hipMalloc(&D.blk_cnt, blk_need * 4));
-      CSD_HIP(hipMalloc(&D.blk_first, blk_need * 4));
-      w->blk_entries = blk_need;
-    }
+    const int grc = grow_stage(w, na, nblk);
+    if (grc) return grc;
     CSD_HIP(hipMemcpy(D.active, slots.data(), na * 4, hipMemcpyHostToDevice));
     CSD_HIP(hipMemcpy(D.nsamp, nsamp.data(), Rn * 4, hipMemcpyHostToDevice));
     const float thresh = (1 - o.tau) * o.epsilon * M;
@@ -541,12 +636,99 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     return CSD_OK;
   };
 
+  if (w->graph_sync) {
+    // Device-resident rounds: one round (start, sampling, check, test and
+    // gather, bisection, faces) is captured as a graph and replayed; the
+    // host reads the region status every graph_sync rounds only. Rounds are
+    // the host-driven ones launched with every region as a slot.
+    if (w->pilot || w->surrogate)
+      return fail(CSD_ERR_INVALID_ARGUMENT,
+                  "device-resident rounds run plain EI-ZO: clear the guidance first");
+    const uint32_t T = o.max_iterations;
+    std::vector<uint32_t> mt(std::max<uint32_t>(T, 1), 0u);
+    std::vector<float> tt(mt.size(), 0.f);
+    for (uint32_t it = 0; it < T; ++it) {
+      mt[it] = unadaptive_samples(o, it);
+      tt[it] = (1 - o.tau) * o.epsilon * mt[it];
+    }
+    const uint32_t zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    CSD_HIP(hipMemcpy(w->ctl, zero8, sizeof(zero8), hipMemcpyHostToDevice));
+    CSD_HIP(hipMemcpy(w->mtab, mt.data(), mt.size() * 4, hipMemcpyHostToDevice));
+    CSD_HIP(hipMemcpy(w->ttab, tt.data(), tt.size() * 4, hipMemcpyHostToDevice));
+    const hipStream_t s = w->gs;
+    const int ublk = (int)((max_budget + kSampleBlock - 1) / kSampleBlock);
+    // the sampler stages the largest polytope the workspace holds
+    const int svar = dim <= 8 && polytope_lds_bytes<8>(cap) <= kLdsLimit    ? 17
+                     : polytope_lds_bytes<16>(cap) <= kLdsLimit ? 33
+                                                                : 32;
+    const size_t slds = svar == 17 ? polytope_lds_bytes<8>(cap)
+                        : svar == 33 ? polytope_lds_bytes<16>(cap)
+                                     : 0;
+    bool done = false;
+    uint32_t it = 0;
+    while (it < T && !done) {
+      // rounds [it, end) share one graph, sized for the largest M_k among them
+      const uint32_t end = std::min<uint32_t>(T, std::max<uint32_t>(16u, it * 4u));
+      const int nblk = (int)((std::max(mt[end - 1], max_budget) + kSampleBlock - 1) / kSampleBlock);
+      if (nblk > 1024) return fail(CSD_ERR_CAPACITY, "more than 262144 samples in one round");
+      rc = grow_stage(w, Rn, nblk);
+      if (rc) return rc;
+      EizoDev G = D;
+      G.ctl = w->ctl;
+      G.mtab = w->mtab;
+      G.ttab = w->ttab;
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      CSD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_eizo_round_begin, dim3(1), dim3(kBeginBlock), 0, s, G, Rn, T);
+      const dim3 sg(nblk, Rn), sb(kSampleBlock);
+      if (svar == 17)
+        hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, slds, s, G, dim, cap, nblk,
+                           (int)o.mixing_steps, k0, k1, 0u, 0u);
+      else if (svar == 33)
+        hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, slds, s, G, dim, cap, nblk,
+                           (int)o.mixing_steps, k0, k1, 0u, 0u);
+      else
+        hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, s, G, dim, cap, nblk,
+                           (int)o.mixing_steps, k0, k1, 0u, 0u);
+      if (dp == 8)
+        CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 8, sg, sb, tab_bytes, s, P, V, RL, G, dim, nblk,
+                               0u);
+      else
+        CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, s, P, V, RL, G, dim,
+                               nblk, 0u);
+      hipLaunchKernelGGL(k_eizo_compact, dim3(Rn), dim3(256), 0, s, G, dim, nblk, 0u, 0.f, U, 1);
+      if (dp == 8)
+        CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 8, dim3(ublk, Rn), dim3(kSampleBlock),
+                               tab_bytes, s, P, V, RL, G, dim, U, (int)o.bisection_steps);
+      else
+        CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 16, dim3(ublk, Rn), dim3(kSampleBlock),
+                               tab_bytes, s, P, V, RL, G, dim, U, (int)o.bisection_steps);
+      hipLaunchKernelGGL(k_eizo_faces, dim3(Rn), dim3(kFaceBlock), 0, s, G, dim, U, cap,
+                         o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1);
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (graph) hipGraphDestroy(graph);
+      while (e == hipSuccess && it < end && !done) {
+        const uint32_t n = std::min<uint32_t>((uint32_t)w->graph_sync, end - it);
+        for (uint32_t k = 0; k < n && e == hipSuccess; ++k) e = hipGraphLaunch(exec, s);
+        it += n;
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(status.data(), D.status, Rn * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        done = std::none_of(status.begin(), status.end(), [](int32_t v) { return v == 0; });
+      }
+      if (exec) hipGraphExecDestroy(exec);
+      if (e != hipSuccess) return hip_fail(e, "EI-ZO round graph");
+    }
+    CSD_HIP(hipMemcpy(rounds.data(), D.iters, Rn * 4, hipMemcpyDeviceToHost));
+    active.clear();  // the host-driven loop below has nothing left to do
+  }
+
   for (uint32_t it = 0; it < o.max_iterations && !active.empty(); ++it) {
     const int na = (int)active.size();
     for (int r : active) rounds[r] += 1;
-    // cuda_edge_inflation_zero_order.cu:113-119 (float arithmetic)
-    const float delta_k = (float)(6 * o.delta / (M_PI * M_PI * (it + 1) * (it + 1)));
-    const uint32_t M = (uint32_t)ceilf(2 * logf(1 / delta_k) / (o.epsilon * o.tau * o.tau));
+    const uint32_t M = unadaptive_samples(o, it);
     // pilot: regions whose pilot collision fraction is far above epsilon skip
     // this round's test and take their particles from the pilot
     std::vector<int32_t> full = active;

@@ -164,6 +164,12 @@ struct EizoDev {
   int32_t *iters;          // [R]
   int64_t *samples;        // [R]
   int64_t *collisions;     // [R]
+  // Device-resident rounds (graph-captured): null when the host drives the
+  // rounds. ctl = {next round, active regions, current round, M_k, test
+  // threshold bits}; mtab/ttab hold M_k and the threshold of every round.
+  uint32_t *ctl;
+  const uint32_t *mtab;
+  const float *ttab;
 };
 
 // Projection parameter of p on the segment q1-q2 and the projected point
@@ -250,6 +256,7 @@ __device__ __forceinline__ void argmin_pair(double &v, int &i, double ov, int oi
 static __global__ __launch_bounds__(kFaceBlock) void k_eizo_faces(EizoDev D, int dim, int U, int cap,
                                                            float margin, int max_faces,
                                                            int by_slot) {
+  if (by_slot && D.ctl && blockIdx.x >= D.ctl[1]) return;  // slot past this round's regions
   const int r = by_slot ? D.active[blockIdx.x] : blockIdx.x;
   if (D.status[r] != 0) return;
   const int u = D.u[r];

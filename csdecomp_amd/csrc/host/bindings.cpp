@@ -460,6 +460,9 @@ PYBIND11_MODULE(_pycsdecomp, m) {
           py::arg("pilot_samples") = 0, py::arg("pilot_factor") = 10.0f,
           py::arg("surrogate") = 0, py::arg("keep_particles") = 256, py::arg("line_points") = 4,
           "surrogate: the address of a csd_mlp (csdecomp_amd.capi.Mlp.address), 0 for none")
+      .def("set_graph", &CudaEdgeInflator::setGraph, py::arg("rounds_per_sync") = 1,
+           "replay one graph-captured round per launch; the host reads the region status "
+           "every rounds_per_sync rounds (0: host-driven rounds)")
       .def("last_info", &CudaEdgeInflator::lastInfo)
       .def("face_capacity", &CudaEdgeInflator::faceCapacity);
 

@@ -948,6 +948,10 @@ void CudaEdgeInflator::setGuidance(uint32_t pilot_samples, float pilot_factor,
                "EI-ZO guidance");
 }
 
+void CudaEdgeInflator::setGraph(int rounds_per_sync) {
+  check_status(csd_eizo_set_graph(ws_, rounds_per_sync), "EI-ZO device-resident rounds");
+}
+
 CudaEdgeInflator::~CudaEdgeInflator() {
   if (ws_) csd_eizo_destroy(ws_);
 }

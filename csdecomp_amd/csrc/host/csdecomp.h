@@ -329,6 +329,10 @@ class CudaEdgeInflator {
   // particles to bisect. surrogate: a csd_mlp* (or null).
   void setGuidance(uint32_t pilot_samples, float pilot_factor, const void *surrogate,
                    int keep_particles = 256, int line_points = 4);
+  // Device-resident, graph-captured rounds (csd_eizo_set_graph): the host
+  // reads the region status every rounds_per_sync rounds; 0 for host-driven
+  // rounds. Same polytopes either way.
+  void setGraph(int rounds_per_sync);
   const EizoInfo &lastInfo() const { return last_info_; }
   int faceCapacity() const { return face_capacity_; }
 

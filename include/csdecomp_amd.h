@@ -257,6 +257,16 @@ int csd_eizo_set_guidance(csd_eizo *ws, uint32_t pilot_samples, float pilot_fact
                           const struct csd_mlp *surrogate, int32_t keep_particles,
                           int32_t line_points);
 
+/* Device-resident rounds for later csd_eizo_run calls (no counterpart in the
+ * reference, whose executeEizo launches every stage from the host and reads
+ * counters back between them, cuda_edge_inflation_zero_order.cu:109-306).
+ * rounds_per_sync 1..64: a kernel builds each round's region list on the
+ * device, one round is captured as a hipGraph and replayed, and the host
+ * reads the region status every rounds_per_sync rounds. Results are those of
+ * host-driven rounds bit for bit. 0 (the default) restores host-driven
+ * rounds. Plain EI-ZO only: csd_eizo_run fails while guidance is set. */
+int csd_eizo_set_graph(csd_eizo *ws, int32_t rounds_per_sync);
+
 /* EI-ZO stage kernels exposed for stage-wise parity tests (device arrays).
  * Replaces executeProjectSamplesOntoLineSegmentsKernel and the bisection
  * kernels (planning/cuda_set_builder_utils.h:25-131), fused: per particle,

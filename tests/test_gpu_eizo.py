@@ -412,3 +412,67 @@ def test_guidance_argument_checks(gpu, csd):
         inf.set_guidance(surrogate=1, keep_particles=0)
     with pytest.raises((ValueError, RuntimeError)):
         inf.set_guidance(surrogate=1, keep_particles=1025)
+
+
+@pytest.mark.parametrize("scene", ["arm7", "dual14"])
+def test_graph_rounds_match_host_rounds(gpu, csd, scene):
+    """Device-resident rounds (one graph-captured round per launch, region
+    list built on the GPU) give the host-driven rounds' polytopes, round
+    counts and sample counts bit for bit: uneven particle budgets, several
+    status-read intervals, replays of the captured graph with host-driven
+    runs in between, and a second call with fewer regions."""
+    from csdecomp_amd import scenes
+    if scene == "arm7":
+        pl, R = scenes.arm_scene(1001, 48, 16), 6
+    else:
+        pl, R = scenes.dual_arm_scene(1002, 72, 24), 4
+    mp = pl.getMinimalPlant()
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    ls, le = scenes.free_segments(pl, R, 3100, (0.1, 0.6))
+    dom = csd.HPolyhedron()
+    dom.MakeBox(lo, hi)
+    opt = csd.EizoOptions()
+    opt.epsilon = 0.01
+    opt.num_particles = 2000
+    opt.bisection_steps = 9
+    opt.max_iterations = 30
+    opt.verbose = False
+    budgets = [500, 2000, 1000, 1800, 800, 1500][:R]  # <= num_particles (the capacity)
+    inf = csd.CudaEdgeInflator(mp, list(pl.getRobotGeometryIds()), opt, dom, R, 10)
+
+    def run(sync, n):
+        inf.set_graph(sync)
+        regions, infos = inf.inflateEdges(np.asfortranarray(ls[:, :n]), np.asfortranarray(le[:, :n]),
+                                          None, 0.0, budgets[:n])
+        return regions, infos
+
+    runs = [run(0, R), run(1, R), run(3, R), run(0, R), run(1, R), run(2, R - 2), run(0, R - 2)]
+    for k, (regions, infos) in enumerate(runs):
+        ref_regions, ref_infos = runs[0] if len(regions) == R else runs[-1]
+        for r in range(len(regions)):
+            np.testing.assert_array_equal(regions[r].A(), ref_regions[r].A(), err_msg=f"run {k} region {r}")
+            np.testing.assert_array_equal(regions[r].b(), ref_regions[r].b(), err_msg=f"run {k} region {r}")
+            got, ref = infos[r], ref_infos[r]
+            assert (got.iterations, got.converged, got.samples_drawn, got.faces_added,
+                    got.face_capacity_reached) == \
+                (ref.iterations, ref.converged, ref.samples_drawn, ref.faces_added,
+                 ref.face_capacity_reached), (k, r)
+    assert sum(i.converged for i in runs[0][1]) >= 1
+    assert max(i.iterations for i in runs[0][1]) >= 2  # several rounds ran
+
+
+def test_graph_rounds_argument_checks(gpu, csd):
+    from csdecomp_amd import scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    dom = csd.HPolyhedron()
+    dom.MakeBox(pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
+    inf = csd.CudaEdgeInflator(pl.getMinimalPlant(), list(pl.getRobotGeometryIds()), csd.EizoOptions(), dom)
+    with pytest.raises((ValueError, RuntimeError)):
+        inf.set_graph(-1)
+    with pytest.raises((ValueError, RuntimeError)):
+        inf.set_graph(65)
+    ls, le = scenes.free_segments(pl, 1, 3101, (0.1, 0.3))
+    inf.set_graph(1)
+    inf.set_guidance(pilot_samples=1000, pilot_factor=10.0)
+    with pytest.raises((ValueError, RuntimeError)):
+        inf.inflateEdges(ls, le, None, 0.0, [])
