@@ -94,15 +94,19 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
         u[4 * c] = u[4 * c + 1] = u[4 * c + 2] = u[4 * c + 3] = 0.5f;
       }
     }
+    // Box-Muller with the hardware transcendentals: r = sqrt(-2 ln u) via
+    // log2 (v_log_f32), and sin/cos of 2*pi*u straight from v_sin_f32 /
+    // v_cos_f32, whose argument is in revolutions (u in (0, 1]). Their
+    // ~1e-6 relative error only perturbs the direction, which any
+    // rotation-invariant law serves.
     float dir[DP];
 #pragma unroll
     for (int k = 0; k < DP / 2; ++k) {
       if (k < npairs) {
-        const float r = sqrtf(-2.0f * logf(u[2 * k]));
-        float sn, cs;
-        sincosf(6.2831853071795864769f * u[2 * k + 1], &sn, &cs);
-        dir[2 * k] = r * cs;
-        dir[2 * k + 1] = r * sn;
+        const float r =
+            __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u[2 * k]));
+        dir[2 * k] = r * __builtin_amdgcn_cosf(u[2 * k + 1]);
+        dir[2 * k + 1] = r * __builtin_amdgcn_sinf(u[2 * k + 1]);
       } else {
         dir[2 * k] = dir[2 * k + 1] = 0.f;
       }
@@ -119,8 +123,10 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
           ad = fmaf(a[d], dir[d], ad);
         }
       }
-      // one (IEEE) division per face, selected into the bound it tightens
-      const float t = (b[f] - ax) / ad;
+      // one division per face, by the hardware reciprocal (1 ulp), selected
+      // into the bound it tightens; ad = 0 gives +-inf or NaN and no bound,
+      // as the IEEE quotient does
+      const float t = (b[f] - ax) * __builtin_amdgcn_rcpf(ad);
       tmin = ad < 0.f ? fmaxf(tmin, t) : tmin;
       tmax = ad > 0.f ? fminf(tmax, t) : tmax;
     }
