@@ -211,6 +211,15 @@ __device__ __forceinline__ float seg_point(float t, float q1, float q2) {
 // verdicts are OR-ed through `part` (one byte per thread) at every step, so
 // all S waves follow the same bisection. Every thread of the workgroup must
 // call this with the same `steps`.
+// A wave-uniform pointer whose value the compiler cannot see through (an
+// empty asm keeps it in scalar registers), so loads through it are not
+// hoisted out of the loop that re-reads them.
+template <class T>
+__device__ __forceinline__ const T *opaque_uniform(const T *p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 template <int VAR, int DP, int S = 1>
 __device__ __forceinline__ void bisect_particle(const PlantImage &P, const VoxelArgs &V,
                                                 const RobotGeomList &R, const float (&p)[DP],
@@ -230,7 +239,16 @@ __device__ __forceinline__ void bisect_particle(const PlantImage &P, const Voxel
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
     for (int d = 0; d < DP; ++d) mid[d] = 0.5f * (hi[d] + lo[d]);
-    bool col = collides_variant<VAR>(P, QReg<DP>{mid}, V, R, idle, tab, sub, S);
+    // 64 and 128 sphere slots: the plant's body and sphere records are read
+    // again in every step; hoisted out of the loop they stay live across all
+    // steps and spill (to scratch for 128 slots)
+    PlantImage Ps = P;
+    if constexpr (VAR == kFast8x8 || VAR == kFast16x4 || VAR == kFast16x8) {
+      Ps.bodies = opaque_uniform(P.bodies);
+      Ps.slot_sphere = opaque_uniform(P.slot_sphere);
+      Ps.self_active = opaque_uniform(P.self_active);
+    }
+    bool col = collides_variant<VAR>(Ps, QReg<DP>{mid}, V, R, idle, tab, sub, S);
     if constexpr (S > 1) {
       part[threadIdx.x] = col ? 1 : 0;
       __syncthreads();
