@@ -119,9 +119,10 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         inf = csd.CudaEdgeInflator(mp, list(plant.getRobotGeometryIds()), opt, dom, R, 10, nd + face_budget)
         if mode == "plain_graph":
             inf.set_graph(1)  # one graph-captured round per launch, status read each round
-        if mode == "guided":
+        if mode in ("guided", "guided_fallback"):
             inf.set_guidance(pilot_samples=4096, pilot_factor=10.0, surrogate=model.address,
-                             keep_particles=256, line_points=4)
+                             keep_particles=256, line_points=4,
+                             plain_fallback=mode == "guided_fallback")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         regions, infos = inf.inflateEdges(starts, ends, None, 0.0, budgets)
@@ -135,6 +136,8 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
                      "rounds_mean": round(float(np.mean([i.iterations for i in infos])), 2),
                      "faces_mean": round(float(np.mean([r.A().shape[0] - nd for r in regions])), 1),
                      "samples": int(sum(i.samples_drawn for i in infos))}
+        if mode == "guided_fallback":
+            out[mode]["fallback_regions"] = int(sum(i.plain_fallback for i in infos))
     if "plain" in faces and "plain_graph" in faces:
         out["plain_graph_identical"] = all(
             np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1])
@@ -390,13 +393,13 @@ def eizo_benchmarks(segments=2048):
     arm = scenes.arm_scene(1001, 48, 16)
     s1, e1 = scenes.free_segments(arm, 256, 3001, (0.2, 1.5))
     m7, tr7 = train_surrogate(arm)
-    res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, ("plain", "plain_graph", "guided"), m7)
+    res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, ("plain", "plain_graph", "guided", "guided_fallback"), m7)
     res["single_edge_arm7"]["surrogate"] = tr7
     dual = scenes.dual_arm_scene(1002, 72, 24)
     s2, e2 = scenes.free_segments(dual, segments, 3002, (0.05, 2.0), log_uniform=True)
     b2 = scenes.log_uniform_budgets(3003, segments)
     m14, tr14 = train_surrogate(dual)
-    res["batched_dualarm14"] = eizo_leg("batched", dual, s2, e2, b2, 16384, ("plain", "plain_graph", "guided"), m14)
+    res["batched_dualarm14"] = eizo_leg("batched", dual, s2, e2, b2, 16384, ("plain", "plain_graph", "guided", "guided_fallback"), m14)
     res["batched_dualarm14"]["surrogate"] = tr14
     return res
 

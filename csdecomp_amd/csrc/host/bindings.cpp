@@ -421,6 +421,7 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def_readonly("iterations", &EizoInfo::iterations)
       .def_readonly("converged", &EizoInfo::converged)
       .def_readonly("face_capacity_reached", &EizoInfo::face_capacity_reached)
+      .def_readonly("plain_fallback", &EizoInfo::plain_fallback)
       .def_readonly("samples_drawn", &EizoInfo::samples_drawn)
       .def_readonly("faces_added", &EizoInfo::faces_added);
 
@@ -452,13 +453,14 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def(
           "set_guidance",
           [](CudaEdgeInflator &self, uint32_t pilot_samples, float pilot_factor,
-             uintptr_t surrogate, int keep_particles, int line_points) {
+             uintptr_t surrogate, int keep_particles, int line_points, bool plain_fallback) {
             self.setGuidance(pilot_samples, pilot_factor,
                              reinterpret_cast<const void *>(surrogate), keep_particles,
-                             line_points);
+                             line_points, plain_fallback);
           },
           py::arg("pilot_samples") = 0, py::arg("pilot_factor") = 10.0f,
           py::arg("surrogate") = 0, py::arg("keep_particles") = 256, py::arg("line_points") = 4,
+          py::arg("plain_fallback") = false,
           "surrogate: the address of a csd_mlp (csdecomp_amd.capi.Mlp.address), 0 for none")
       .def("set_graph", &CudaEdgeInflator::setGraph, py::arg("rounds_per_sync") = 1,
            "replay one graph-captured round per launch; the host reads the region status "

@@ -941,11 +941,18 @@ CudaEdgeInflator::CudaEdgeInflator(const MinimalPlant &plant,
 }
 
 void CudaEdgeInflator::setGuidance(uint32_t pilot_samples, float pilot_factor,
-                                   const void *surrogate, int keep_particles, int line_points) {
+                                   const void *surrogate, int keep_particles, int line_points,
+                                   bool plain_fallback) {
   check_status(csd_eizo_set_guidance(ws_, pilot_samples, pilot_factor,
                                      static_cast<const csd_mlp *>(surrogate), keep_particles,
                                      line_points),
                "EI-ZO guidance");
+  pilot_samples_ = pilot_samples;
+  pilot_factor_ = pilot_factor;
+  surrogate_ = surrogate;
+  keep_particles_ = keep_particles;
+  line_points_ = line_points;
+  plain_fallback_ = plain_fallback;
 }
 
 void CudaEdgeInflator::setGraph(int rounds_per_sync) {
@@ -984,6 +991,54 @@ std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, cons
                             voxels.data.data(), voxels.cols, voxel_radius, A.data(), b.data(),
                             inf.data()),
                "EI-ZO");
+  std::vector<char> fell_back(R, 0);
+  if (plain_fallback_ && (pilot_samples_ || surrogate_)) {
+    // plain EI-ZO for the regions the guided rounds left uncertified, in the
+    // same workspace with the guidance off (restored afterwards)
+    std::vector<int> redo;
+    for (int r = 0; r < R; ++r)
+      if (!inf[r].converged) redo.push_back(r);
+    if (!redo.empty()) {
+      const int n = (int)redo.size();
+      MatrixXf ls2(dim, n), le2(dim, n);
+      std::vector<uint32_t> bud2;
+      for (int k = 0; k < n; ++k) {
+        for (int d = 0; d < dim; ++d) {
+          ls2(d, k) = ls(d, redo[k]);
+          le2(d, k) = le(d, redo[k]);
+        }
+        if (!budgets.empty()) bud2.push_back(budgets[redo[k]]);
+      }
+      std::vector<float> A2((size_t)n * face_capacity_ * dim), b2((size_t)n * face_capacity_);
+      std::vector<csd_eizo_region_info> inf2(n);
+      int rc = csd_eizo_set_guidance(ws_, 0, 1.f, nullptr, 1, 1);
+      if (rc == CSD_OK)
+        rc = csd_eizo_run(ws_, dom.A.data(), dom.b.data(), dom.num_faces, ls2.data.data(),
+                          le2.data.data(), n, bud2.empty() ? nullptr : bud2.data(),
+                          voxels.data.data(), voxels.cols, voxel_radius, A2.data(), b2.data(),
+                          inf2.data());
+      const std::string err = rc == CSD_OK ? std::string() : std::string(csd_last_error());
+      check_status(csd_eizo_set_guidance(ws_, pilot_samples_, pilot_factor_,
+                                         static_cast<const csd_mlp *>(surrogate_),
+                                         keep_particles_, line_points_),
+                   "EI-ZO guidance");
+      if (rc != CSD_OK) throw std::runtime_error("[csdecomp_amd] EI-ZO plain fallback: " + err);
+      for (int k = 0; k < n; ++k) {
+        const int r = redo[k];
+        std::copy(A2.begin() + (size_t)k * face_capacity_ * dim,
+                  A2.begin() + (size_t)(k + 1) * face_capacity_ * dim,
+                  A.begin() + (size_t)r * face_capacity_ * dim);
+        std::copy(b2.begin() + (size_t)k * face_capacity_, b2.begin() + (size_t)(k + 1) * face_capacity_,
+                  b.begin() + (size_t)r * face_capacity_);
+        csd_eizo_region_info g = inf[r];
+        inf[r] = inf2[k];
+        inf[r].iterations += g.iterations;
+        inf[r].samples_drawn += g.samples_drawn;
+        inf[r].collisions_found += g.collisions_found;
+        fell_back[r] = 1;
+      }
+    }
+  }
   std::vector<HPolyhedron> out;
   if (infos) infos->assign(R, EizoInfo());
   for (int r = 0; r < R; ++r) {
@@ -1002,6 +1057,7 @@ std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, cons
       e.face_capacity_reached = inf[r].capacity_reached != 0;
       e.samples_drawn = inf[r].samples_drawn;
       e.faces_added = inf[r].faces_added;
+      e.plain_fallback = fell_back[r] != 0;
     }
   }
   return out;

@@ -305,6 +305,10 @@ struct EizoInfo {
   bool face_capacity_reached = false;
   int64_t samples_drawn = 0;
   int faces_added = 0;
+  // guided rounds left the region uncertified and plain EI-ZO inflated it
+  // again (setGuidance plain_fallback); iterations and samples_drawn count
+  // both runs
+  bool plain_fallback = false;
 };
 
 uint32_t numSamplesUnadaptive(float epsilon, float delta, float tau);
@@ -327,8 +331,10 @@ class CudaEdgeInflator {
   // EI-ZO guidance (csd_eizo_set_guidance): an exact pilot that lets clearly
   // colliding regions skip a round's test, and surrogate ranking of the
   // particles to bisect. surrogate: a csd_mlp* (or null).
+  // plain_fallback: regions that guided rounds leave uncertified are
+  // inflated again by plain EI-ZO, whose result replaces the guided one.
   void setGuidance(uint32_t pilot_samples, float pilot_factor, const void *surrogate,
-                   int keep_particles = 256, int line_points = 4);
+                   int keep_particles = 256, int line_points = 4, bool plain_fallback = false);
   // Device-resident, graph-captured rounds (csd_eizo_set_graph): the host
   // reads the region status every rounds_per_sync rounds; 0 for host-driven
   // rounds. Same polytopes either way.
@@ -342,6 +348,12 @@ class CudaEdgeInflator {
   EizoOptions options_;
   HPolyhedron domain_;
   int max_regions_, max_voxels_, face_capacity_;
+  // guidance as last set (restored after a plain fallback run)
+  uint32_t pilot_samples_ = 0;
+  float pilot_factor_ = 10.f;
+  const void *surrogate_ = nullptr;
+  int keep_particles_ = 256, line_points_ = 4;
+  bool plain_fallback_ = false;
   csd_eizo *ws_ = nullptr;
   EizoInfo last_info_;
 };

@@ -476,3 +476,52 @@ def test_graph_rounds_argument_checks(gpu, csd):
     inf.set_guidance(pilot_samples=1000, pilot_factor=10.0)
     with pytest.raises((ValueError, RuntimeError)):
         inf.inflateEdges(ls, le, None, 0.0, [])
+
+
+def test_guided_plain_fallback(gpu, csd):
+    """plain_fallback: regions the guided rounds certify keep the guided
+    polytope; the others are exactly what plain EI-ZO gives for that subset
+    of edges (same order, same budgets), and their info counts both runs."""
+    from csdecomp_amd import scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    ls, le = scenes.free_segments(pl, 8, 3102, (0.4, 1.0))
+    R = ls.shape[1]
+    dom = csd.HPolyhedron()
+    dom.MakeBox(lo, hi)
+    opt = csd.EizoOptions()
+    opt.epsilon = 0.01
+    opt.num_particles = 1500
+    opt.bisection_steps = 9
+    opt.max_iterations = 3
+    opt.verbose = False
+    rg = list(pl.getRobotGeometryIds())
+    budgets = [600, 1500, 900, 1200, 700, 1500, 1000, 800]
+
+    def inflator(fallback=None):
+        inf = csd.CudaEdgeInflator(mp, rg, opt, dom, R, 10)
+        if fallback is not None:
+            inf.set_guidance(pilot_samples=2048, pilot_factor=10.0, plain_fallback=fallback)
+        return inf
+
+    guided, ginfo = inflator(False).inflateEdges(ls, le, None, 0.0, budgets)
+    both, binfo = inflator(True).inflateEdges(ls, le, None, 0.0, budgets)
+    redo = [r for r in range(R) if not ginfo[r].converged]
+    assert redo, "every edge certified in the guided rounds; the fallback went unused"
+    sub = np.asfortranarray(ls[:, redo]), np.asfortranarray(le[:, redo])
+    plain, pinfo = inflator().inflateEdges(*sub, None, 0.0, [budgets[r] for r in redo])
+    for r in range(R):
+        if r in redo:
+            k = redo.index(r)
+            ref, refinfo = plain[k], pinfo[k]
+            assert binfo[r].plain_fallback
+            assert binfo[r].converged == refinfo.converged
+            assert binfo[r].iterations == ginfo[r].iterations + refinfo.iterations
+            assert binfo[r].samples_drawn == ginfo[r].samples_drawn + refinfo.samples_drawn
+        else:
+            ref = guided[r]
+            assert not binfo[r].plain_fallback and binfo[r].converged
+        np.testing.assert_array_equal(both[r].A(), ref.A())
+        np.testing.assert_array_equal(both[r].b(), ref.b())
+        assert both[r].PointInSet(ls[:, r], 1e-5) and both[r].PointInSet(le[:, r], 1e-5)
