@@ -404,7 +404,42 @@ def eizo_benchmarks(segments=2048):
     return res
 
 
-def large_batch_benchmark(plant, dp, stream, sizes=(262144, 1048576, 4194304), steps=20):
+def graph_replay_benchmark(dp, dq, n, ref_flags, launches=50, replays=5):
+    """The headline check captured as one hipGraph of `launches` launches
+    (torch.cuda.graph over the C-ABI call, which only launches kernels) and
+    replayed `replays` times, the output cleared eagerly between replays;
+    every replay's flags must equal the eager flags."""
+    import torch
+    from csdecomp_amd import capi
+    out = torch.empty(n, dtype=torch.uint8, device=dq.device)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        capi.check_collision_free(dp, dq, n, 1, n, out, s)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for _ in range(launches):
+            capi.check_collision_free(dp, dq, n, 1, n, out, s)
+    torch.cuda.synchronize()
+    times, identical = [], True
+    for _ in range(replays):
+        out.zero_()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b))
+        identical = identical and bool(torch.equal(out, ref_flags))
+    ms = float(np.median(times)) / launches
+    del g
+    return {"launches_per_graph": launches, "replays": replays, "ms_per_launch": ms,
+            "configs_per_s": n / (ms * 1e-3), "flags_identical_to_eager": identical}
+
+
+def large_batch_benchmark(plant, dp, stream, sizes=(262144, 1048576, 4194304, 16777216), steps=20):
     """The headline check on larger batches of the same workload (configs
     seed 2000), each autotuned first: from a few hundred thousand
     configurations on, the tuner picks the screened checker."""
@@ -562,6 +597,7 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP32_VECTOR_PEAK_TFLOPS,
                      "traffic": traffic, "flops_per_config": f_cfg, "kernel_ms": kern_ms},
     }
+    line["graph_replay"] = graph_replay_benchmark(dp, dq, n, out.clone())
     if not args.no_large:
         line["large_batches"] = large_batch_benchmark(plant, dp, stream)
     if not args.no_cpu:
