@@ -69,6 +69,42 @@ def cpu_baseline(mp, q, seconds):
                       f"CPU rule of collision_checker.cpp with early exit"}
 
 
+def eizo_cpu_baseline(plant, starts, ends, seconds=12.0, face_budget=100):
+    """CPU restatement of single-edge EI-ZO (the oracle's orc_eizo: the
+    rounds of cuda_edge_inflation_zero_order.cu:94-339 with the GPU leg's
+    options) over the leg's first segments, one region after another until
+    `seconds` have passed. The collision checks of the samples use OpenMP
+    over the host cores; sampling and bisection run on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import csdecomp_amd as csd
+    op = oracle.Plant(plant.getMinimalPlant())
+    dom = csd.HPolyhedron()
+    dom.MakeBox(plant.getPositionLowerLimits(), plant.getPositionUpperLimits())
+    A, b = np.asarray(dom.A(), np.float32), np.asarray(dom.b(), np.float32)
+    # margin, tau, delta, epsilon, bisection steps, particles, faces per round,
+    # rounds, mixing steps, seed
+    opts = (0.01, 0.5, 0.05, 1e-3, 10, 4096, 10, 500, 40, 1337)
+    rg = list(plant.getRobotGeometryIds())
+    done = certified = 0
+    t0 = time.perf_counter()
+    while done < starts.shape[1]:
+        try:
+            info = oracle.eizo(op, rg, A, b, starts[:, done], ends[:, done], opts,
+                               cap=A.shape[0] + face_budget)[2]
+            certified += int(info.converged)
+        except RuntimeError:
+            pass  # face capacity reached: the region ends uncertified, as on the GPU
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": done / el, "unit": "regions/s", "cores": os.cpu_count() or 1, "kind": "port",
+            "certified_fraction": certified / done,
+            "sample": f"{done} regions (the first segments of the leg), {el:.1f} s; OpenMP collision "
+                      f"checks, sampling and bisection on one thread"}
+
+
 def train_surrogate(plant, n=2_000_000, batch=8192, epochs=3, seed=7):
     """Surrogate for the plant: n exact FP32 signed-distance labels (clipped to
     +-0.2 m, scaled to +-1) at uniform configurations, Adam lr 1e-3."""
@@ -383,7 +419,7 @@ def voxel_maps_benchmark(nodes=20000, edges=2000, step=0.02):
     return res_out
 
 
-def eizo_benchmarks(segments=2048):
+def eizo_benchmarks(segments=2048, cpu=True):
     """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
     segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
     dual arm, 96 obstacles, 2048 segments, lengths log-U[0.05, 2] rad, budgets
@@ -395,6 +431,8 @@ def eizo_benchmarks(segments=2048):
     m7, tr7 = train_surrogate(arm)
     res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, ("plain", "plain_graph", "guided", "guided_fallback"), m7)
     res["single_edge_arm7"]["surrogate"] = tr7
+    if cpu:
+        res["single_edge_arm7"]["cpu_baseline"] = eizo_cpu_baseline(arm, s1, e1)
     dual = scenes.dual_arm_scene(1002, 72, 24)
     s2, e2 = scenes.free_segments(dual, segments, 3002, (0.05, 2.0), log_uniform=True)
     b2 = scenes.log_uniform_budgets(3003, segments)
@@ -497,7 +535,7 @@ def main():
     if args.leg:
         from csdecomp_amd import capi
         capi.init(0)
-        fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments), "mlp": mlp_benchmark,
+        fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments, not args.no_cpu), "mlp": mlp_benchmark,
               "roadmap": roadmap_benchmark, "edit_regions": edit_regions_benchmark,
               "distance_field": distance_field_benchmark,
               "voxel_maps": voxel_maps_benchmark,
@@ -603,7 +641,7 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(mp, q, args.cpu_seconds)
     if not args.no_eizo:
-        line["eizo"] = eizo_benchmarks(args.eizo_segments)
+        line["eizo"] = eizo_benchmarks(args.eizo_segments, not args.no_cpu)
         line["surrogate_mlp"] = mlp_benchmark()
         line["roadmap"] = roadmap_benchmark()
         line["edit_regions"] = edit_regions_benchmark()
