@@ -31,7 +31,7 @@ namespace csd {
 // unchecked into the block's staging segment as [dim][256] (coalesced).
 // Kept apart from the collision check so that the chains run at the high
 // occupancy their small register footprint allows.
-template <int DPL>
+template <int DPL, bool BROW = false>
 __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim, int cap,
                                                               int nblk, int mixing, uint32_t k0,
                                                               uint32_t k1, uint32_t round,
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
   const int F = D.nf[r];
   const float *A = D.A + (size_t)r * cap * dim, *b = D.b + (size_t)r * cap;
   if constexpr (LDS) {
-    stage_polytope<DP>(csd_smem, csd_smem + (size_t)F * DP, A, b, F, dim);
+    stage_polytope<DP, BROW>(csd_smem, csd_smem + (size_t)F * DP, A, b, F, dim);
     __syncthreads();
     A = csd_smem;
     b = csd_smem + (size_t)F * DP;
@@ -67,7 +67,8 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
     const float e = d < dim ? D.le[(size_t)r * dim + d] : 0.f;
     x[d] = fmaf(f, e - a, a);
   }
-  hit_and_run<DP, LDS>(A, b, F, dim, x, i, (uint32_t)r + stream_off, round, mixing, k0, k1);
+  hit_and_run<DP, LDS, BROW>(A, b, F, dim, x, i, (uint32_t)r + stream_off, round, mixing, k0,
+                             k1);
   float *seg = D.stage + ((size_t)slot * nblk + blockIdx.x) * kSampleBlock * dim;
 #pragma unroll
   for (int d = 0; d < DP; ++d)
@@ -615,9 +616,15 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     int nfmax = 0;
     for (int r : slots) nfmax = std::max(nfmax, nfh[r]);
     const dim3 sg(nblk, na), sb(kSampleBlock);
-    if (dim <= 8 && polytope_lds_bytes<8>(nfmax) <= kLdsLimit)
+    if (dim <= 7 && polytope_lds_bytes<8>(nfmax) <= kLdsLimit)
+      hipLaunchKernelGGL((k_eizo_sample<17, true>), sg, sb, polytope_lds_bytes<8>(nfmax), 0, D,
+                         dim, cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
+    else if (dim <= 8 && polytope_lds_bytes<8>(nfmax) <= kLdsLimit)
       hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, polytope_lds_bytes<8>(nfmax), 0, D, dim,
                          cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
+    else if (dim <= 15 && polytope_lds_bytes<16>(nfmax) <= kLdsLimit)
+      hipLaunchKernelGGL((k_eizo_sample<33, true>), sg, sb, polytope_lds_bytes<16>(nfmax), 0, D,
+                         dim, cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
     else if (polytope_lds_bytes<16>(nfmax) <= kLdsLimit)
       hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, polytope_lds_bytes<16>(nfmax), 0, D, dim,
                          cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
@@ -682,8 +689,14 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       CSD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       hipLaunchKernelGGL(k_eizo_round_begin, dim3(1), dim3(kBeginBlock), 0, s, G, Rn, T);
       const dim3 sg(nblk, Rn), sb(kSampleBlock);
-      if (svar == 17)
+      if (svar == 17 && dim <= 7)
+        hipLaunchKernelGGL((k_eizo_sample<17, true>), sg, sb, slds, s, G, dim, cap, nblk,
+                           (int)o.mixing_steps, k0, k1, 0u, 0u);
+      else if (svar == 17)
         hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, slds, s, G, dim, cap, nblk,
+                           (int)o.mixing_steps, k0, k1, 0u, 0u);
+      else if (svar == 33 && dim <= 15)
+        hipLaunchKernelGGL((k_eizo_sample<33, true>), sg, sb, slds, s, G, dim, cap, nblk,
                            (int)o.mixing_steps, k0, k1, 0u, 0u);
       else if (svar == 33)
         hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, slds, s, G, dim, cap, nblk,

@@ -46,14 +46,17 @@ __host__ __device__ constexpr size_t polytope_lds_bytes(int faces) {
   return (size_t)faces * (DP + 1) * sizeof(float);
 }
 
-template <int DP>
+// BROW (dim < DP only): b_f is stored in the row's last padding slot, so a
+// face is read with the row alone.
+template <int DP, bool BROW = false>
 __device__ __forceinline__ void stage_polytope(float *sA, float *sb, const float *A,
                                                const float *b, int F, int dim) {
   for (int e = threadIdx.x; e < F * DP; e += blockDim.x) {
     const int f = e / DP, d = e - f * DP;
-    sA[e] = d < dim ? A[(size_t)f * dim + d] : 0.f;
+    sA[e] = d < dim ? A[(size_t)f * dim + d] : (BROW && d == DP - 1) ? b[f] : 0.f;
   }
-  for (int f = threadIdx.x; f < F; f += blockDim.x) sb[f] = b[f];
+  if constexpr (!BROW)
+    for (int f = threadIdx.x; f < F; f += blockDim.x) sb[f] = b[f];
 }
 
 // One hit-and-run chain (cuda_hit_and_run_sampling.cu:74-112) of `mixing`
@@ -61,7 +64,7 @@ __device__ __forceinline__ void stage_polytope(float *sA, float *sb, const float
 // wave-uniform: rows are LDS broadcasts (LDS = true, stride DP) or scalar
 // loads (stride dim). A failed step (empty chord) leaves the chain where it
 // is.
-template <int DP, bool LDS>
+template <int DP, bool LDS, bool BROW = false>
 __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
                                             const float *__restrict__ b, int F, int dim,
                                             float (&x)[DP], uint32_t chain, uint32_t stream,
@@ -116,17 +119,20 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
     for (int f = 0; f < F; ++f) {
       const float *a = A + (size_t)f * stride;
       float ax = 0.f, ad = 0.f;
+      // with b in the last slot that slot is skipped: x and dir are 0 there,
+      // so the skipped terms were exact zeros
 #pragma unroll
-      for (int d = 0; d < DP; ++d) {
+      for (int d = 0; d < (BROW ? DP - 1 : DP); ++d) {
         if (LDS || d < dim) {
           ax = fmaf(a[d], x[d], ax);
           ad = fmaf(a[d], dir[d], ad);
         }
       }
+      const float bf = BROW ? a[DP - 1] : b[f];
       // one division per face, by the hardware reciprocal (1 ulp), selected
       // into the bound it tightens; ad = 0 gives +-inf or NaN and no bound,
       // as the IEEE quotient does
-      const float t = (b[f] - ax) * __builtin_amdgcn_rcpf(ad);
+      const float t = (bf - ax) * __builtin_amdgcn_rcpf(ad);
       tmin = ad < 0.f ? fmaxf(tmin, t) : tmin;
       tmax = ad > 0.f ? fminf(tmax, t) : tmax;
     }
