@@ -245,11 +245,13 @@ __device__ __forceinline__ void bisect_particle(const PlantImage &P, const Voxel
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
     for (int d = 0; d < DP; ++d) mid[d] = 0.5f * (hi[d] + lo[d]);
-    // 64 and 128 sphere slots: the plant's body and sphere records are read
-    // again in every step; hoisted out of the loop they stay live across all
-    // steps and spill (to scratch for 128 slots)
+    // fast plants (32 slots with one wave per particle; 64 and 128 slots):
+    // the plant's body and sphere records are read again in every step;
+    // hoisted out of the loop they stay live across all steps and spill (to
+    // scratch for 128 slots)
     PlantImage Ps = P;
-    if constexpr (VAR == kFast8x8 || VAR == kFast16x4 || VAR == kFast16x8) {
+    if constexpr ((VAR == kFast8x4 && S == 1) || VAR == kFast8x8 || VAR == kFast16x4 ||
+                  VAR == kFast16x8) {
       Ps.bodies = opaque_uniform(P.bodies);
       Ps.slot_sphere = opaque_uniform(P.slot_sphere);
       Ps.self_active = opaque_uniform(P.self_active);
