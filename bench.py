@@ -117,7 +117,8 @@ def train_surrogate(plant, n=2_000_000, batch=8192, epochs=3, seed=7):
     t0 = time.perf_counter()
     q = torch.from_numpy(np.random.default_rng(seed).uniform(lo, hi, (n, dof)).astype(np.float32)).cuda()
     sd = torch.empty(n, device="cuda")
-    capi.signed_distance(capi.DevicePlant(mp), q, n, dof, 1, sd)
+    dplant = capi.DevicePlant(mp)  # alive until the labels are computed
+    capi.signed_distance(dplant, q, n, dof, 1, sd)
     target = (sd.clamp(-0.2, 0.2) / 0.2).contiguous()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -156,7 +157,7 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         if mode == "plain_graph":
             inf.set_graph(1)  # one graph-captured round per launch, status read each round
         if mode in ("guided", "guided_fallback"):
-            inf.set_guidance(pilot_samples=4096, pilot_factor=10.0, surrogate=model.address,
+            inf.set_guidance(pilot_samples=4096, pilot_factor=10.0, surrogate=model,
                              keep_particles=256, line_points=4,
                              plain_fallback=mode == "guided_fallback")
         torch.cuda.synchronize()

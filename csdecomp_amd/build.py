@@ -22,7 +22,8 @@ LIB_DIR = os.path.join(PKG, "lib")
 OBJ_DIR = os.path.join(ROOT, "_build", "obj")
 
 HIP_SOURCES = ["csd_plant.hip", "csd_check.hip", "csd_eizo.hip", "csd_stages.hip", "csd_edges.hip",
-               "csd_mlp.hip", "csd_field.hip", "csd_screen.hip"]
+               "csd_mlp.hip", "csd_field.hip", "csd_screen.hip", "csd_bisect8.hip",
+               "csd_bisect16.hip"]
 HOST_SOURCES = ["csdecomp.cpp", "bindings.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

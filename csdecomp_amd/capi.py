@@ -101,13 +101,55 @@ def init(device=0):
 
 
 def _ptr(a):
+    """Pointer to a HOST array (numpy), or None."""
     if a is None:
         return None
-    if hasattr(a, "data_ptr"):  # torch tensor
-        return C.c_void_p(a.data_ptr())
     if isinstance(a, np.ndarray):
         return a.ctypes.data_as(C.c_void_p)
+    raise TypeError(f"expected a host numpy array, got {type(a).__name__}")
+
+
+def _dptr(a, what="device array"):
+    """Pointer to DEVICE memory: a torch tensor on the GPU, or a raw address
+    (int) the caller vouches for. Host memory is refused: a kernel given a
+    host pointer would fault."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        raise TypeError(f"{what}: a host numpy array was passed where device memory is required")
+    if hasattr(a, "data_ptr"):  # torch tensor
+        if not a.is_cuda:
+            raise TypeError(f"{what}: the tensor is on {a.device}, not on the GPU")
+        return C.c_void_p(a.data_ptr())
     return C.c_void_p(int(a))
+
+
+def _dtensor(a, what, dtype, min_numel):
+    """Checks a torch tensor handed to a kernel that reads it densely:
+    on the GPU, contiguous, of `dtype`, with at least `min_numel` elements."""
+    import torch
+    if not isinstance(a, torch.Tensor):
+        raise TypeError(f"{what}: expected a torch tensor on the GPU")
+    if not a.is_cuda:
+        raise TypeError(f"{what}: the tensor is on {a.device}, not on the GPU")
+    if a.dtype != dtype:
+        raise TypeError(f"{what}: dtype {a.dtype}, expected {dtype}")
+    if not a.is_contiguous():
+        raise ValueError(f"{what}: the tensor must be contiguous")
+    if a.numel() < min_numel:
+        raise ValueError(f"{what}: {a.numel()} elements, at least {min_numel} needed")
+    return C.c_void_p(a.data_ptr())
+
+
+def _span(a, n, stride_n, stride_j, dim, what="configurations"):
+    """For a contiguous torch tensor, checks that every element the two
+    strides address lies inside it."""
+    if n <= 0 or not hasattr(a, "numel") or not a.is_contiguous():
+        return
+    last = (n - 1) * stride_n + (dim - 1) * stride_j
+    if stride_n < 0 or stride_j < 0 or last >= a.numel():
+        raise ValueError(f"{what}: strides ({stride_n}, {stride_j}) over {n} x {dim} address "
+                         f"element {last}, past the tensor's {a.numel()}")
 
 
 def _stream(s):
@@ -169,50 +211,54 @@ class DevicePlant:
 
 
 def check_collision_free(plant, q, n, stride_n, stride_j, out, stream=None):
-    check(lib().csd_check_collision_free(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+    _span(q, n, stride_n, stride_j, plant.dof)
+    check(lib().csd_check_collision_free(plant.handle, _dptr(q), n, stride_n, stride_j, _dptr(out),
                                          _stream(stream)), "csd_check_collision_free")
 
 
 def check_collision_free_voxels(plant, q, n, stride_n, stride_j, voxels, num_voxels, radius,
                                 robot_geoms, out, stream=None):
     rg = np.ascontiguousarray(robot_geoms, np.int32)
-    check(lib().csd_check_collision_free_voxels(plant.handle, _ptr(q), n, stride_n, stride_j,
-                                                _ptr(voxels), num_voxels, radius, _ptr(rg), len(rg),
-                                                _ptr(out), _stream(stream)),
+    _span(q, n, stride_n, stride_j, plant.dof)
+    check(lib().csd_check_collision_free_voxels(plant.handle, _dptr(q), n, stride_n, stride_j,
+                                                _dptr(voxels), num_voxels, radius, _ptr(rg), len(rg),
+                                                _dptr(out), _stream(stream)),
           "csd_check_collision_free_voxels")
 
 
 def voxel_flags(plant, q, n, stride_n, stride_j, voxels, num_voxels, radius, robot_geoms, out,
                 stream=None):
     rg = np.ascontiguousarray(robot_geoms, np.int32)
-    check(lib().csd_voxel_flags(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(voxels),
-                                num_voxels, radius, _ptr(rg), len(rg), _ptr(out), _stream(stream)),
+    _span(q, n, stride_n, stride_j, plant.dof)
+    check(lib().csd_voxel_flags(plant.handle, _dptr(q), n, stride_n, stride_j, _dptr(voxels),
+                                num_voxels, radius, _ptr(rg), len(rg), _dptr(out), _stream(stream)),
           "csd_voxel_flags")
 
 
 def forward_kinematics(plant, q, n, stride_n, stride_j, out, stream=None):
-    check(lib().csd_forward_kinematics(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+    _span(q, n, stride_n, stride_j, plant.dof)
+    check(lib().csd_forward_kinematics(plant.handle, _dptr(q), n, stride_n, stride_j, _dptr(out),
                                        _stream(stream)), "csd_forward_kinematics")
 
 
 def hit_and_run(A, b, nf, face_stride, dim, num_polytopes, starts, n, mixing, seed, round_, out,
                 stream=None):
-    check(lib().csd_hit_and_run(_ptr(A), _ptr(b), _ptr(nf), face_stride, dim, num_polytopes,
-                                _ptr(starts), n, mixing, seed, round_, _ptr(out), _stream(stream)),
+    check(lib().csd_hit_and_run(_dptr(A), _dptr(b), _dptr(nf), face_stride, dim, num_polytopes,
+                                _dptr(starts), n, mixing, seed, round_, _dptr(out), _stream(stream)),
           "csd_hit_and_run")
 
 
 def bisect(plant, particles, ls, le, seg_idx, n, steps, scol, proj, voxels=None, num_voxels=0,
            radius=0.0, robot_geoms=(), stream=None):
     rg = np.ascontiguousarray(robot_geoms, np.int32)
-    check(lib().csd_bisect(plant.handle, _ptr(particles), _ptr(ls), _ptr(le), _ptr(seg_idx), n,
-                           steps, _ptr(voxels), num_voxels, radius, _ptr(rg), len(rg), _ptr(scol),
-                           _ptr(proj), _stream(stream)), "csd_bisect")
+    check(lib().csd_bisect(plant.handle, _dptr(particles), _dptr(ls), _dptr(le), _dptr(seg_idx), n,
+                           steps, _dptr(voxels), num_voxels, radius, _ptr(rg), len(rg), _dptr(scol),
+                           _dptr(proj), _stream(stream)), "csd_bisect")
 
 
 def place_faces(A, b, nf, cap, dim, scol, proj, n, ls, le, margin, max_faces, added, stream=None):
-    check(lib().csd_place_faces(_ptr(A), _ptr(b), _ptr(nf), cap, dim, _ptr(scol), _ptr(proj), n,
-                                _ptr(ls), _ptr(le), margin, max_faces, _ptr(added), _stream(stream)),
+    check(lib().csd_place_faces(_dptr(A), _dptr(b), _dptr(nf), cap, dim, _dptr(scol), _dptr(proj), n,
+                                _dptr(ls), _dptr(le), margin, max_faces, _dptr(added), _stream(stream)),
           "csd_place_faces")
 
 
@@ -222,17 +268,17 @@ def edit_regions(plant, collisions, seg_idx, offsets, num_collisions, ls, le, nu
     """csd_edit_regions on device arrays: collisions grouped by region
     (offsets [R+1]); A [R][cap][dim], b [R][cap], nf [R] edited in place."""
     rg = np.ascontiguousarray(robot_geoms, np.int32)
-    check(lib().csd_edit_regions(plant.handle, _ptr(collisions), _ptr(seg_idx), _ptr(offsets),
-                                 num_collisions, _ptr(ls), _ptr(le), num_regions, _ptr(A), _ptr(b),
-                                 _ptr(nf), cap, _ptr(voxels), num_voxels, radius, _ptr(rg), len(rg),
-                                 margin, steps, _ptr(scol), _ptr(proj), _ptr(status),
+    check(lib().csd_edit_regions(plant.handle, _dptr(collisions), _dptr(seg_idx), _dptr(offsets),
+                                 num_collisions, _dptr(ls), _dptr(le), num_regions, _dptr(A), _dptr(b),
+                                 _dptr(nf), cap, _dptr(voxels), num_voxels, radius, _ptr(rg), len(rg),
+                                 margin, steps, _dptr(scol), _dptr(proj), _dptr(status),
                                  _stream(stream)),
           "csd_edit_regions")
 
 
 def validate_edges(plant, nodes, num_nodes, edges, num_edges, step, rule, out, stream=None):
-    check(lib().csd_validate_edges(plant.handle, _ptr(nodes), num_nodes, _ptr(edges), num_edges,
-                                   step, rule, _ptr(out), _stream(stream)), "csd_validate_edges")
+    check(lib().csd_validate_edges(plant.handle, _dptr(nodes), num_nodes, _dptr(edges), num_edges,
+                                   step, rule, _dptr(out), _stream(stream)), "csd_validate_edges")
 
 
 def occupancy(plant, q, n, stride_n, stride_j, robot_geoms, center, size, resolution, grid,
@@ -240,8 +286,9 @@ def occupancy(plant, q, n, stride_n, stride_j, robot_geoms, center, size, resolu
     rg = np.ascontiguousarray(robot_geoms, np.int32)
     c = np.ascontiguousarray(center, np.float32)
     s = np.ascontiguousarray(size, np.float32)
-    check(lib().csd_occupancy(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(rg), len(rg),
-                              _ptr(c), _ptr(s), resolution, _ptr(grid), _stream(stream)),
+    _span(q, n, stride_n, stride_j, plant.dof)
+    check(lib().csd_occupancy(plant.handle, _dptr(q), n, stride_n, stride_j, _ptr(rg), len(rg),
+                              _ptr(c), _ptr(s), resolution, _dptr(grid), _stream(stream)),
           "csd_occupancy")
 
 
@@ -253,9 +300,9 @@ def voxel_collision_map(plant, nodes, num_nodes, edges, num_edges, step, robot_g
     c = np.ascontiguousarray(center, np.float32)
     sz = np.ascontiguousarray(size, np.float32)
     total = _I64(0)
-    check(lib().csd_voxel_collision_map(plant.handle, _ptr(nodes), num_nodes, _ptr(edges),
+    check(lib().csd_voxel_collision_map(plant.handle, _dptr(nodes), num_nodes, _dptr(edges),
                                         num_edges, step, _ptr(rg), len(rg), _ptr(c), _ptr(sz),
-                                        resolution, _ptr(offsets), _ptr(items), capacity,
+                                        resolution, _dptr(offsets), _dptr(items), capacity,
                                         C.byref(total), _stream(stream)),
           "csd_voxel_collision_map")
     return int(total.value)
@@ -265,13 +312,13 @@ def knn_page(nodes, num_nodes, dim, queries, num_queries, k, after_d2, after_idx
              out_d2, stream=None):
     """csd_knn_page: the next k neighbours of each query node after
     (after_d2, after_idx) (both None: the first k)."""
-    check(lib().csd_knn_page(_ptr(nodes), num_nodes, dim, _ptr(queries), num_queries, k,
-                             _ptr(after_d2), _ptr(after_idx), _ptr(out_idx), _ptr(out_d2),
+    check(lib().csd_knn_page(_dptr(nodes), num_nodes, dim, _dptr(queries), num_queries, k,
+                             _dptr(after_d2), _dptr(after_idx), _dptr(out_idx), _dptr(out_d2),
                              _stream(stream)), "csd_knn_page")
 
 
 def knn(nodes, num_nodes, dim, k, out_idx, out_d2=None, stream=None):
-    check(lib().csd_knn(_ptr(nodes), num_nodes, dim, k, _ptr(out_idx), _ptr(out_d2),
+    check(lib().csd_knn(_dptr(nodes), num_nodes, dim, k, _dptr(out_idx), _dptr(out_d2),
                         _stream(stream)), "csd_knn")
 
 
@@ -285,14 +332,16 @@ def tune_check(plant, q, n, stride_n, stride_j, out, stream=None, force=0):
     configurations; 11, 12, 14: two passes; 20: screened). force: record
     that shape instead of timing."""
     s = _I32(force)
-    check(lib().csd_tune_check(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+    _span(q, n, stride_n, stride_j, plant.dof)
+    check(lib().csd_tune_check(plant.handle, _dptr(q), n, stride_n, stride_j, _dptr(out),
                                _stream(stream), C.byref(s)), "csd_tune_check")
     return int(s.value)
 
 
 def signed_distance(plant, q, n, stride_n, stride_j, out, stream=None):
     """Smallest FP32 signed distance over the collision pairs (labels)."""
-    check(lib().csd_signed_distance(plant.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
+    _span(q, n, stride_n, stride_j, plant.dof)
+    check(lib().csd_signed_distance(plant.handle, _dptr(q), n, stride_n, stride_j, _dptr(out),
                                     _stream(stream)), "csd_signed_distance")
 
 
@@ -326,25 +375,41 @@ class Mlp:
         check(lib().csd_mlp_set_params(self.handle, _ptr(p)), "csd_mlp_set_params")
 
     def predict(self, q, n, stride_n, stride_j, out, stream=None):
-        check(lib().csd_mlp_predict(self.handle, _ptr(q), n, stride_n, stride_j, _ptr(out),
-                                    _stream(stream)), "csd_mlp_predict")
+        _span(q, n, stride_n, stride_j, self.in_dim)
+        check(lib().csd_mlp_predict(self.handle, _dptr(q, "q"), n, stride_n, stride_j,
+                                    _dptr(out, "out"), _stream(stream)), "csd_mlp_predict")
+
+    def _batch_args(self, q, target, batch, loss):
+        """q [batch][in_dim] and target [batch] as the kernels read them:
+        contiguous float32 on the GPU."""
+        import torch
+        if batch <= 0 or batch % 128:
+            raise ValueError("batch must be a positive multiple of 128")
+        if q.dim() != 2 or q.shape[0] < batch or q.shape[1] != self.in_dim:
+            raise ValueError(f"q: shape {tuple(q.shape)}, expected ({batch}, {self.in_dim})")
+        return (_dtensor(q, "q", torch.float32, batch * self.in_dim),
+                _dtensor(target, "target", torch.float32, batch),
+                None if loss is None else _dtensor(loss, "loss", torch.float32, 1))
 
     def gradients(self, q, target, batch, grads=None, loss=None, stream=None):
-        check(lib().csd_mlp_gradients(self.handle, _ptr(q), _ptr(target), batch,
-                                      _ptr(grads) if grads is not None else None,
-                                      _ptr(loss) if loss is not None else None,
-                                      _stream(stream)), "csd_mlp_gradients")
+        import torch
+        qp, tp, lp = self._batch_args(q, target, batch, loss)
+        gp = None if grads is None else _dtensor(grads, "grads", torch.float32, self.num_params)
+        check(lib().csd_mlp_gradients(self.handle, qp, tp, batch, gp, lp, _stream(stream)),
+              "csd_mlp_gradients")
 
     def train_step(self, q, target, batch, lr, loss=None, stream=None):
-        check(lib().csd_mlp_train_step(self.handle, _ptr(q), _ptr(target), batch, lr,
-                                       _ptr(loss) if loss is not None else None,
-                                       _stream(stream)), "csd_mlp_train_step")
+        qp, tp, lp = self._batch_args(q, target, batch, loss)
+        check(lib().csd_mlp_train_step(self.handle, qp, tp, batch, lr, lp, _stream(stream)),
+              "csd_mlp_train_step")
 
     def __del__(self):
+        # csd_mlp_destroy refuses while an EI-ZO workspace holds the model
+        # (it is then left allocated rather than freed under the workspace)
         try:
-            if getattr(self, "handle", None):
-                lib().csd_mlp_destroy(self.handle)
-                self.handle = None
+            if getattr(self, "handle", None) and _lib is not None:
+                if _lib.csd_mlp_destroy(self.handle) == 0:
+                    self.handle = None
         except Exception:
             pass
 
@@ -369,7 +434,7 @@ class DistanceField:
               "csd_distance_field_reset_occupancy")
 
     def set_occupancy(self, points, n, stream=None):
-        check(lib().csd_distance_field_set_occupancy(self.handle, _ptr(points), n,
+        check(lib().csd_distance_field_set_occupancy(self.handle, _dptr(points), n,
                                                      _stream(stream)),
               "csd_distance_field_set_occupancy")
 
@@ -378,7 +443,7 @@ class DistanceField:
               "csd_distance_field_compute")
 
     def get_distance(self, points, n, out, stream=None):
-        check(lib().csd_distance_field_get_distance(self.handle, _ptr(points), n, _ptr(out),
+        check(lib().csd_distance_field_get_distance(self.handle, _dptr(points), n, _dptr(out),
                                                     _stream(stream)),
               "csd_distance_field_get_distance")
 

@@ -495,12 +495,14 @@ int csd_eizo_create(const csd_plant *plant, const int32_t *robot_geoms, int32_t 
 
 int csd_eizo_destroy(csd_eizo *w) {
   if (!w) return CSD_OK;
+  hipDeviceSynchronize();
   for (void *p : w->owned) hipFree(p);
   if (w->D.stage) hipFree(w->D.stage);
   if (w->D.blk_cnt) hipFree(w->D.blk_cnt);
   if (w->D.blk_first) hipFree(w->D.blk_first);
   if (w->pred) hipFree(w->pred);
   if (w->gs) hipStreamDestroy(w->gs);
+  mlp_attach(w->surrogate, -1);
   delete w;
   return CSD_OK;
 }
@@ -516,6 +518,10 @@ int csd_eizo_set_guidance(csd_eizo *w, uint32_t pilot_samples, float pilot_facto
     return fail(CSD_ERR_INVALID_ARGUMENT, "keep_particles must be 1..1024 and line_points 1..64");
   w->pilot = pilot_samples;
   w->pilot_factor = pilot_factor;
+  if (surrogate != w->surrogate) {
+    mlp_attach(surrogate, +1);
+    mlp_attach(w->surrogate, -1);
+  }
   w->surrogate = surrogate;
   if (surrogate) {
     w->keep = keep_particles;

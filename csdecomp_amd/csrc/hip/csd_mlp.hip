@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "csd_collide.h"
@@ -515,6 +516,8 @@ struct csd_mlp {
   uint16_t *wt[kHid + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   uint16_t *w5b = nullptr;
   int64_t step = 0;
+  // EI-ZO workspaces currently guided by this model
+  mutable std::atomic<int> attached{0};
   // training workspace, sized for the largest batch seen
   int cap = 0;
   void *ws = nullptr;
@@ -763,6 +766,12 @@ int csd_mlp_create(int32_t in_dim, const float *lower, const float *upper, uint6
 
 int csd_mlp_destroy(csd_mlp *m) {
   if (!m) return CSD_OK;
+  if (m->attached.load() > 0)
+    return fail(CSD_ERR_INVALID_ARGUMENT,
+                "the surrogate still guides an EI-ZO workspace: clear the guidance or destroy "
+                "the workspace first");
+  // kernels launched asynchronously may still read the model
+  hipDeviceSynchronize();
   if (m->ws) hipFree(m->ws);
   if (m->arena) hipFree(m->arena);
   delete m;
@@ -845,6 +854,10 @@ int csd_mlp_train_step(csd_mlp *m, const float *q, const float *target, int32_t 
 }
 
 }  // extern "C"
+
+void csd::mlp_attach(const csd_mlp *m, int delta) {
+  if (m) m->attached.fetch_add(delta);
+}
 
 int csd::mlp_predict_segment_points(const csd_mlp *m, const SegmentPoints &s, int num_slots,
                                     float *out, hipStream_t st) {

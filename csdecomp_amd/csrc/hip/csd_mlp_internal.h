@@ -21,4 +21,9 @@ struct SegmentPoints {
 int mlp_predict_segment_points(const csd_mlp *m, const SegmentPoints &s, int num_slots,
                                float *out, hipStream_t st);
 
+// EI-ZO workspaces that hold the surrogate register here (delta +1 on
+// attach, -1 on detach); csd_mlp_destroy refuses while the count is positive,
+// so a workspace never keeps a freed model.
+void mlp_attach(const csd_mlp *m, int delta);
+
 }  // namespace csd

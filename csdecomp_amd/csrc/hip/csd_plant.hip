@@ -598,6 +598,8 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
 
 int csd_plant_destroy(csd_plant *plant) {
   if (!plant) return CSD_OK;
+  // kernels launched asynchronously may still read the image
+  hipDeviceSynchronize();
   if (plant->arena) hipFree(plant->arena);
   if (plant->screen_arena) hipFree(plant->screen_arena);
   delete plant;

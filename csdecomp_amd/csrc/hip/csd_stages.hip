@@ -5,7 +5,7 @@
 // round.
 #include <algorithm>
 
-#include "csd_eizo_device.h"
+#include "csd_bisect.h"
 
 namespace csd {
 
@@ -36,41 +36,6 @@ __global__ __launch_bounds__(kSampleBlock) void k_hit_and_run(
 #pragma unroll
   for (int d = 0; d < DP; ++d)
     if (d < dim) out[((size_t)p * n + i) * dim + d] = x[d];
-}
-
-// DPS = 8 * DP + S: DP padded dimensions, S waves per 64 particles
-template <int VAR, int DPS>
-__global__ __launch_bounds__(kSampleBlock) void k_bisect_standalone(
-    PlantImage P, VoxelArgs V, RobotGeomList R, const float *__restrict__ part,
-    const float *__restrict__ ls, const float *__restrict__ le,
-    const uint32_t *__restrict__ seg, int n, int dim, int steps, float *__restrict__ scol,
-    float *__restrict__ proj) {
-  constexpr int DP = DPS / 8, S = DPS % 8;
-  constexpr int kPer = kSampleBlock / S;
-  extern __shared__ __attribute__((aligned(16))) float csd_smem[];
-  __shared__ uint8_t verdict[kSampleBlock];
-  const float *tab = stage_tables<VAR>(P, csd_smem);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int sub = wave % S, grp = wave / S;
-  const int i = blockIdx.x * kPer + grp * 64 + lane;
-  const bool valid = i < n;
-  const int ii = valid ? i : n - 1;
-  const uint32_t s_idx = seg[ii];
-  float p[DP], q1[DP], q2[DP], sc[DP], pr[DP];
-#pragma unroll
-  for (int d = 0; d < DP; ++d) {
-    p[d] = d < dim ? part[(size_t)ii * dim + d] : 0.f;
-    q1[d] = d < dim ? ls[(size_t)s_idx * dim + d] : 0.f;
-    q2[d] = d < dim ? le[(size_t)s_idx * dim + d] : 0.f;
-  }
-  bisect_particle<VAR, DP, S>(P, V, R, p, q1, q2, steps, !valid, sc, pr, tab, sub, verdict);
-  if (!valid || sub != 0) return;
-#pragma unroll
-  for (int d = 0; d < DP; ++d)
-    if (d < dim) {
-      scol[(size_t)i * dim + d] = sc[d];
-      proj[(size_t)i * dim + d] = pr[d];
-    }
 }
 
 __global__ void k_face_prep(const float *__restrict__ scol, const float *__restrict__ proj,
@@ -252,20 +217,10 @@ int launch_bisect(const csd_plant *plant, const float *particles, const float *l
   const size_t lds = check_lds_bytes(plant->img, var);
   const dim3 g((unsigned)((num_particles + kSampleBlock / split - 1) / (kSampleBlock / split)));
   const dim3 blk(kSampleBlock);
-#define CSD_BISECT(DPS)                                                                         \
-  CSD_DISPATCH_VARIANT_X(var, k_bisect_standalone, DPS, g, blk, lds, st, plant->img, V, R,      \
-                         particles, line_start, line_end, segment_idx, num_particles, dim, steps, \
-                         scol, proj)
-  if (dim <= 8) {
-    if (split == 4) CSD_BISECT(68);
-    else if (split == 2) CSD_BISECT(66);
-    else CSD_BISECT(65);
-  } else {
-    if (split == 4) CSD_BISECT(132);
-    else if (split == 2) CSD_BISECT(130);
-    else CSD_BISECT(129);
-  }
-#undef CSD_BISECT
+  const BisectLaunch L{plant->img, V, R, particles, line_start, line_end, segment_idx,
+                      num_particles, dim, steps, scol, proj, g, blk, lds, st};
+  if (dim <= 8) launch_bisect_dp8(L, var, split);
+  else launch_bisect_dp16(L, var, split);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

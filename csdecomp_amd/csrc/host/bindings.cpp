@@ -453,15 +453,22 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def(
           "set_guidance",
           [](CudaEdgeInflator &self, uint32_t pilot_samples, float pilot_factor,
-             uintptr_t surrogate, int keep_particles, int line_points, bool plain_fallback) {
-            self.setGuidance(pilot_samples, pilot_factor,
-                             reinterpret_cast<const void *>(surrogate), keep_particles,
-                             line_points, plain_fallback);
+             const py::object &surrogate, int keep_particles, int line_points,
+             bool plain_fallback) {
+            uintptr_t addr = 0;
+            if (py::hasattr(surrogate, "address")) addr = surrogate.attr("address").cast<uintptr_t>();
+            else if (!surrogate.is_none()) addr = surrogate.cast<uintptr_t>();
+            self.setGuidance(pilot_samples, pilot_factor, reinterpret_cast<const void *>(addr),
+                             keep_particles, line_points, plain_fallback);
           },
           py::arg("pilot_samples") = 0, py::arg("pilot_factor") = 10.0f,
-          py::arg("surrogate") = 0, py::arg("keep_particles") = 256, py::arg("line_points") = 4,
-          py::arg("plain_fallback") = false,
-          "surrogate: the address of a csd_mlp (csdecomp_amd.capi.Mlp.address), 0 for none")
+          py::arg("surrogate") = py::none(), py::arg("keep_particles") = 256,
+          py::arg("line_points") = 4, py::arg("plain_fallback") = false,
+          // the inflator keeps the surrogate object alive: it reads the model
+          // in every guided round
+          py::keep_alive<1, 4>(),
+          "surrogate: a csdecomp_amd.capi.Mlp (kept alive by the inflator), the address of a "
+          "csd_mlp, or None")
       .def("set_graph", &CudaEdgeInflator::setGraph, py::arg("rounds_per_sync") = 1,
            "replay one graph-captured round per launch; the host reads the region status "
            "every rounds_per_sync rounds (0: host-driven rounds)")

@@ -96,6 +96,7 @@ int csd_stream_destroy(void *stream);
  * a compact device image (fixed joints folded, world-fixed geometry
  * pre-transformed, pairs grouped by type). */
 int csd_plant_create(const csd_plant_desc *desc, csd_plant **out);
+/* Waits for the device (kernels may still read the image), then frees. */
 int csd_plant_destroy(csd_plant *plant);
 int csd_plant_info(const csd_plant *plant, int32_t *num_links,
                    int32_t *num_configuration_variables,
@@ -253,6 +254,8 @@ struct csd_mlp;
  *    particle's projection and the particle) and only the keep_particles (1..1024) best
  *    of the first 1024 are bisected; the exact checker still judges every
  *    bisection step and every sample. */
+/* The workspace holds the surrogate until the guidance is changed or the
+ * workspace destroyed; csd_mlp_destroy refuses a model held this way. */
 int csd_eizo_set_guidance(csd_eizo *ws, uint32_t pilot_samples, float pilot_factor,
                           const struct csd_mlp *surrogate, int32_t keep_particles,
                           int32_t line_points);
@@ -443,6 +446,8 @@ int csd_knn_page(const float *nodes, int32_t num_nodes, int32_t dim,
 typedef struct csd_mlp csd_mlp;
 int csd_mlp_create(int32_t in_dim, const float *lower, const float *upper,
                    uint64_t seed, csd_mlp **out);  /* lower/upper: host [in_dim] */
+/* CSD_ERR_INVALID_ARGUMENT (model kept) while an EI-ZO workspace holds the
+ * model as its surrogate; waits for the device before freeing. */
 int csd_mlp_destroy(csd_mlp *model);
 int csd_mlp_num_params(const csd_mlp *model, int64_t *count);
 int csd_mlp_get_params(const csd_mlp *model, float *host_params);

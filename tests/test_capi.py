@@ -6,6 +6,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from conftest import ROOT, parse
 
@@ -93,3 +94,28 @@ def test_hpolyhedron_host_logic():
     T = csd.HPolyhedron(np.array([[-1, 0], [0, -1], [1, 1]], np.float32), np.array([0, 0, 1], np.float32))
     r = 1 / (2 + np.sqrt(2))
     np.testing.assert_allclose(T.ChebyshevCenter(), [r, r], atol=1e-5)
+
+
+def test_device_pointer_checks_refuse_host_memory():
+    """Entry points that take device memory refuse host arrays and CPU
+    tensors before any kernel sees them; strided reads must stay inside the
+    tensor. Host logic only (no GPU call is made)."""
+    import torch
+    from csdecomp_amd import capi
+    with pytest.raises(TypeError):
+        capi._dptr(np.zeros(4, np.float32), "q")
+    with pytest.raises(TypeError):
+        capi._dptr(torch.zeros(4), "q")
+    assert capi._dptr(None) is None
+    assert capi._dptr(0x1000).value == 0x1000  # a raw address the caller vouches for
+    with pytest.raises(TypeError):
+        capi._ptr(torch.zeros(4))  # host entry points take numpy arrays
+    q = torch.zeros(7 * 100)
+    capi._span(q, 100, 1, 100, 7)  # SoA, in range
+    capi._span(q, 100, 7, 1, 7)  # (dim, N) column-major, in range
+    with pytest.raises(ValueError):
+        capi._span(q, 101, 1, 100, 7)
+    with pytest.raises(ValueError):
+        capi._span(q, 100, 1, 101, 7)
+    with pytest.raises(TypeError):
+        capi._dtensor(torch.zeros(8), "q", torch.float32, 8)  # on the CPU

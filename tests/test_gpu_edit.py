@@ -124,7 +124,8 @@ def _run(gpu, mp, cols, seg, ls, le, regions, vox, radius, rg, margin, steps, ca
     dpr = torch.empty(max(C, 1) * dim, device=gpu)
     dst = torch.full((R,), -99, dtype=torch.int32, device=gpu)
     dvox = t(vox.T) if vox is not None else None
-    capi.edit_regions(capi.DevicePlant(mp), t(cols) if C else None,
+    dplant = capi.DevicePlant(mp)
+    capi.edit_regions(dplant, t(cols) if C else None,
                       t(seg.view(np.int32)) if C else None, t(offsets), C, t(ls.T), t(le.T), R,
                       dA, db, dnf, cap, margin, steps, dsc, dpr, dst, voxels=dvox,
                       num_voxels=0 if vox is None else vox.shape[1], radius=radius,

@@ -345,7 +345,8 @@ def _train_surrogate(gpu, pl, steps=300, n=262144, seed=5):
     dof = len(lo)
     q = torch.from_numpy(np.random.default_rng(seed).uniform(lo, hi, (n, dof)).astype(np.float32)).to(gpu)
     sd = torch.empty(n, device=gpu)
-    capi.signed_distance(capi.DevicePlant(mp), q, n, dof, 1, sd)
+    dplant = capi.DevicePlant(mp)
+    capi.signed_distance(dplant, q, n, dof, 1, sd)
     t = (sd.clamp(-0.2, 0.2) / 0.2).contiguous()
     m = capi.Mlp(dof, lo, hi, seed)
     B = 8192
@@ -383,7 +384,7 @@ def test_guided_eizo_certifies(gpu, csd, orc):
             inf.set_guidance(pilot_samples=2048, pilot_factor=10.0)
         elif mode.startswith("guided"):
             # 32 of the ranked particles, or all 1,024 ranked (the bitonic order)
-            inf.set_guidance(pilot_samples=2048, pilot_factor=10.0, surrogate=model.address,
+            inf.set_guidance(pilot_samples=2048, pilot_factor=10.0, surrogate=model,
                              keep_particles=32 if mode == "guided" else 1024, line_points=4)
         regions, infos = inf.inflateEdges(ls, le, None, 0.0, [])
         again, _ = inf.inflateEdges(ls, le, None, 0.0, [])

@@ -123,11 +123,68 @@ def test_gradients_match_oracle(gpu):
 def test_batch_argument_checks(gpu):
     import torch
     from csdecomp_amd import capi
+    import ctypes
     m, _, _ = _model(7, 1)
-    q = torch.zeros(100 * 7, device=gpu)
-    t = torch.zeros(100, device=gpu)
-    with pytest.raises(capi.CsdError):
-        m.train_step(q, t, 100, 1e-3)  # not a multiple of 128
+    q = torch.zeros(256, 7, device=gpu)
+    t = torch.zeros(256, device=gpu)
+    with pytest.raises(ValueError):
+        m.train_step(q[:100], t, 100, 1e-3)  # not a multiple of 128
+    # the C-ABI refuses it too when called directly
+    rc = capi.lib().csd_mlp_train_step(m.handle, ctypes.c_void_p(q.data_ptr()),
+                                       ctypes.c_void_p(t.data_ptr()), 100, 1e-3, None, None)
+    assert rc == 1  # CSD_ERR_INVALID_ARGUMENT
+    with pytest.raises(TypeError):
+        m.train_step(q.cpu(), t, 128, 1e-3)  # host memory
+    with pytest.raises(TypeError):
+        m.train_step(q.double(), t, 128, 1e-3)
+    with pytest.raises(ValueError):
+        m.train_step(q[:, :6], t, 128, 1e-3)  # in_dim columns
+    with pytest.raises(ValueError):
+        m.train_step(torch.zeros(7, 256, device=gpu).t(), t, 128, 1e-3)  # not contiguous
+    with pytest.raises(ValueError):
+        m.train_step(q, t[:64], 128, 1e-3)  # too few targets
+    m.train_step(q, t, 256, 1e-3)
+    torch.cuda.synchronize()
+
+
+def test_surrogate_outlives_the_inflator_it_guides(gpu):
+    """The inflator keeps the Mlp object alive, and the C-ABI refuses to
+    destroy a model that an EI-ZO workspace holds."""
+    import gc
+    import weakref
+    import torch
+    from csdecomp_amd import capi, scenes
+    import csdecomp_amd as csd
+    pl = scenes.arm_scene(1001, 48, 16)
+    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
+    dom = csd.HPolyhedron()
+    dom.MakeBox(lo, hi)
+    opts = csd.EizoOptions()
+    opts.num_particles = 256
+    opts.max_iterations = 3
+    inf = csd.CudaEdgeInflator(pl.getMinimalPlant(), list(pl.getRobotGeometryIds()), opts, dom,
+                               max_regions=2)
+    m = capi.Mlp(7, lo, hi, 3)
+    inf.set_guidance(pilot_samples=512, pilot_factor=10.0, surrogate=m, keep_particles=64)
+    assert capi.lib().csd_mlp_destroy(m.handle) != 0  # attached: refused
+    ref = weakref.ref(m)
+    del m
+    gc.collect()
+    assert ref() is not None  # held by the inflator
+    ls = np.zeros((7, 1), np.float32)
+    le = np.full((7, 1), 0.05, np.float32)
+    regions, infos = inf.inflateEdges(ls, le, None, 0.0, [256])
+    assert infos[0].iterations >= 1
+    inf.set_guidance()  # detached: the model may now be destroyed
+    m = ref()
+    assert capi.lib().csd_mlp_destroy(m.handle) == 0
+    m.handle = None
+    del m
+    del inf
+    gc.collect()
+    assert ref() is None  # released with the inflator
+    torch.cuda.synchronize()
 
 
 def test_training_learns_signed_distance(gpu):
