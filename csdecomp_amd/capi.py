@@ -36,6 +36,8 @@ _SIGS = {
     "csd_check_collision_free_voxels": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _F, _VP, _I32, _VP, _VP]),
     "csd_voxel_flags": (C.c_int, [_VP, _VP, _I64, _I64, _I64, _VP, _I32, _F, _VP, _I32, _VP, _VP]),
     "csd_hit_and_run": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _I32, _VP, _I64, _I32, _U64, _U32, _VP, _VP]),
+    "csd_hit_and_run_segments": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _I32, _VP, _VP, _I64, _I32,
+                                           _U64, _U32, _U32, _VP, _VP]),
     "csd_bisect": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _I32, _I32, _VP, _I32, _F, _VP, _I32, _VP, _VP, _VP]),
     "csd_place_faces": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _VP, _VP, _I32, _VP, _VP, _F, _I32, _VP, _VP]),
     "csd_edit_regions": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _VP, _VP, _I32, _VP, _VP, _VP, _I32,
@@ -246,6 +248,17 @@ def hit_and_run(A, b, nf, face_stride, dim, num_polytopes, starts, n, mixing, se
     check(lib().csd_hit_and_run(_dptr(A), _dptr(b), _dptr(nf), face_stride, dim, num_polytopes,
                                 _dptr(starts), n, mixing, seed, round_, _dptr(out), _stream(stream)),
           "csd_hit_and_run")
+
+
+def hit_and_run_segments(A, b, nf, face_stride, dim, num_polytopes, seg_start, seg_end, n,
+                         mixing, seed, stream_offset, round_, out, stream=None):
+    """The EI-ZO round's sampler (csd_hit_and_run_segments): chain i of
+    polytope p starts at (i + 1/2) / n along segment p, stream p +
+    stream_offset."""
+    check(lib().csd_hit_and_run_segments(_dptr(A), _dptr(b), _dptr(nf), face_stride, dim,
+                                         num_polytopes, _dptr(seg_start), _dptr(seg_end), n,
+                                         mixing, seed, stream_offset, round_, _dptr(out),
+                                         _stream(stream)), "csd_hit_and_run_segments")
 
 
 def bisect(plant, particles, ls, le, seg_idx, n, steps, scol, proj, voxels=None, num_voxels=0,

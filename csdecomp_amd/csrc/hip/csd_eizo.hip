@@ -20,6 +20,7 @@
 //    region, all faces of a round in one launch; b is rounded up when stored
 //    in FP32 so the segment stays inside the stored face.
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "csd_eizo_device.h"
@@ -140,12 +141,14 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
 // `test`, and gather of the first budget_r colliding samples, one workgroup
 // per region. Without the test the collision count goes to pilot_col.
 __global__ __launch_bounds__(256) void k_eizo_compact(EizoDev D, int dim, int nblk, uint32_t M,
-                                                      float thresh, int U, int test) {
+                                                      float thresh, int U, int test,
+                                                      uint32_t round) {
   const int slot = blockIdx.x;
   if (D.ctl) {
     if (slot >= (int)D.ctl[1]) return;
     M = D.ctl[3];
     thresh = __uint_as_float(D.ctl[4]);
+    round = D.ctl[2];
   }
   const int r = D.active[slot];
   const uint32_t N = D.nsamp[r];
@@ -168,6 +171,21 @@ __global__ __launch_bounds__(256) void k_eizo_compact(EizoDev D, int dim, int nb
     if (!test) D.pilot_col[r] = acc;
     s_u = certified ? 0 : (int)min(acc, min(D.budget[r], (uint32_t)U));
     D.u[r] = s_u;
+    if (int32_t *st = round_stats(D, r, round)) {
+      if (test) {
+        st[kStatSamples] = (int32_t)N;
+        st[kStatFirstCol] = (int32_t)first;
+        st[kStatCol] = (int32_t)acc;
+      } else {  // pilot; a later test phase of the round overwrites these
+        st[kStatPilot] = (int32_t)N;
+        st[kStatPilotCol] = (int32_t)acc;
+        st[kStatSamples] = 0;
+        st[kStatFirstCol] = -1;
+        st[kStatCol] = 0;
+      }
+      st[kStatUpdated] = s_u;
+      st[kStatFacesStart] = D.nf[r];
+    }
   }
   __syncthreads();
   const int u = s_u;
@@ -196,7 +214,8 @@ constexpr int kRankMax = 1024;  // particles ranked per region (the first ones)
 
 __global__ __launch_bounds__(kRankBlock) void k_eizo_rank(EizoDev D, const float *__restrict__ pred,
                                                           int dim, int U, int L, int cand,
-                                                          int keep, float *__restrict__ tmp) {
+                                                          int keep, float *__restrict__ tmp,
+                                                          uint32_t round) {
   const int slot = blockIdx.x;
   const int r = D.active[slot];
   if (D.status[r] != 0) return;
@@ -269,7 +288,10 @@ __global__ __launch_bounds__(kRankBlock) void k_eizo_rank(EizoDev D, const float
   __syncthreads();
   for (int e = threadIdx.x; e < k * dim; e += kRankBlock)
     D.part[(size_t)r * U * dim + e] = t[e];
-  if (threadIdx.x == 0) D.u[r] = k;
+  if (threadIdx.x == 0) {
+    D.u[r] = k;
+    if (int32_t *st = round_stats(D, r, round)) st[kStatUpdated] = k;
+  }
 }
 
 // Stage 3: projection + all bisection steps per lane; FP64 distance from
@@ -385,6 +407,10 @@ struct csd_eizo {
   hipStream_t gs = nullptr;
   uint32_t *ctl = nullptr, *mtab = nullptr;
   float *ttab = nullptr;
+  // the last csd_eizo_run: regions, rounds per region, wall time per round
+  int last_regions = 0;
+  std::vector<int32_t> last_rounds;
+  std::vector<float> round_ms;
 };
 
 using namespace csd;
@@ -485,6 +511,8 @@ int csd_eizo_create(const csd_plant *plant, const int32_t *robot_geoms, int32_t 
   if (!rc) rc = dalloc(w, &w->ctl, 8);
   if (!rc) rc = dalloc(w, &w->mtab, std::max<uint32_t>(options->max_iterations, 1));
   if (!rc) rc = dalloc(w, &w->ttab, std::max<uint32_t>(options->max_iterations, 1));
+  D.T = (int)std::max<uint32_t>(options->max_iterations, 1);
+  if (!rc) rc = dalloc(w, &D.stats, R * (size_t)D.T * kStats);
   if (rc) {
     csd_eizo_destroy(w);
     return rc;
@@ -587,6 +615,10 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     CSD_HIP(hipMemcpy(p, zeros64.data(), Rn * 8, hipMemcpyHostToDevice));
   if (num_voxels)
     CSD_HIP(hipMemcpy(w->vox, voxels, (size_t)num_voxels * 12, hipMemcpyHostToDevice));
+  CSD_HIP(hipMemset(D.stats, 0, (size_t)Rn * D.T * kStats * sizeof(int32_t)));
+  w->last_regions = 0;
+  w->round_ms.assign((size_t)D.T, 0.f);
+  using clock = std::chrono::steady_clock;
 
   RobotGeomList RL;
   VoxelArgs V{w->vox, num_voxels, voxel_radius, 0, 0};
@@ -644,7 +676,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, 0, P, V, RL, D, dim,
                              nblk, M);
     hipLaunchKernelGGL(k_eizo_compact, dim3(na), dim3(256), 0, 0, D, dim, nblk, M, thresh, U,
-                       test ? 1 : 0);
+                       test ? 1 : 0, it);
     CSD_HIP(hipGetLastError());
     return CSD_OK;
   };
@@ -716,7 +748,8 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       else
         CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, s, P, V, RL, G, dim,
                                nblk, 0u);
-      hipLaunchKernelGGL(k_eizo_compact, dim3(Rn), dim3(256), 0, s, G, dim, nblk, 0u, 0.f, U, 1);
+      hipLaunchKernelGGL(k_eizo_compact, dim3(Rn), dim3(256), 0, s, G, dim, nblk, 0u, 0.f, U, 1,
+                         0u);
       if (dp == 8)
         CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 8, dim3(ublk, Rn), dim3(kSampleBlock),
                                tab_bytes, s, P, V, RL, G, dim, U, (int)o.bisection_steps);
@@ -724,17 +757,21 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
         CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 16, dim3(ublk, Rn), dim3(kSampleBlock),
                                tab_bytes, s, P, V, RL, G, dim, U, (int)o.bisection_steps);
       hipLaunchKernelGGL(k_eizo_faces, dim3(Rn), dim3(kFaceBlock), 0, s, G, dim, U, cap,
-                         o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1);
+                         o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1, 0u);
       hipError_t e = hipStreamEndCapture(s, &graph);
       if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (graph) hipGraphDestroy(graph);
       while (e == hipSuccess && it < end && !done) {
         const uint32_t n = std::min<uint32_t>((uint32_t)w->graph_sync, end - it);
+        const auto t0 = clock::now();
         for (uint32_t k = 0; k < n && e == hipSuccess; ++k) e = hipGraphLaunch(exec, s);
-        it += n;
         if (e == hipSuccess)
           e = hipMemcpyAsync(status.data(), D.status, Rn * 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
+        // the replays between two status reads share their wall time
+        const float ms = std::chrono::duration<float, std::milli>(clock::now() - t0).count();
+        for (uint32_t k = it; k < it + n && k < (uint32_t)D.T; ++k) w->round_ms[k] = ms / n;
+        it += n;
         done = std::none_of(status.begin(), status.end(), [](int32_t v) { return v == 0; });
       }
       if (exec) hipGraphExecDestroy(exec);
@@ -745,6 +782,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
   }
 
   for (uint32_t it = 0; it < o.max_iterations && !active.empty(); ++it) {
+    const auto t_round = clock::now();
     const int na = (int)active.size();
     for (int r : active) rounds[r] += 1;
     const uint32_t M = unadaptive_samples(o, it);
@@ -791,7 +829,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       rc = mlp_predict_segment_points(w->surrogate, sp, na, w->pred, 0);
       if (rc) return rc;
       hipLaunchKernelGGL(k_eizo_rank, dim3(na), dim3(kRankBlock), 0, 0, D, w->pred, dim, U, L,
-                         cand, w->keep, D.stage);
+                         cand, w->keep, D.stage, it);
     }
     const uint32_t bmax = w->surrogate ? (uint32_t)w->keep : max_budget;
     const int ublk = (int)((bmax + kSampleBlock - 1) / kSampleBlock);
@@ -802,7 +840,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 16, dim3(ublk, na), dim3(kSampleBlock),
                              tab_bytes, 0, P, V, RL, D, dim, U, (int)o.bisection_steps);
     hipLaunchKernelGGL(k_eizo_faces, dim3(na), dim3(kFaceBlock), 0, 0, D, dim, U, cap,
-                       o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1);
+                       o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1, it);
     CSD_HIP(hipGetLastError());
     CSD_HIP(hipMemcpy(status.data(), D.status, Rn * 4, hipMemcpyDeviceToHost));
     CSD_HIP(hipMemcpy(nfh.data(), D.nf, Rn * 4, hipMemcpyDeviceToHost));
@@ -810,6 +848,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     for (int r : active)
       if (status[r] == 0) next.push_back(r);  // 1 certified, 2 face capacity full
     active.swap(next);
+    w->round_ms[it] = std::chrono::duration<float, std::milli>(clock::now() - t_round).count();
   }
 
   CSD_HIP(hipMemcpy(out_A, D.A, (size_t)Rn * cap * dim * 4, hipMemcpyDeviceToHost));
@@ -821,6 +860,8 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
   CSD_HIP(hipMemcpy(smp.data(), D.samples, Rn * 8, hipMemcpyDeviceToHost));
   CSD_HIP(hipMemcpy(colls.data(), D.collisions, Rn * 8, hipMemcpyDeviceToHost));
   CSD_HIP(hipMemcpy(status.data(), D.status, Rn * 4, hipMemcpyDeviceToHost));
+  w->last_regions = Rn;
+  w->last_rounds = rounds;
   for (int r = 0; r < Rn; ++r) {
     infos[r].iterations = rounds[r];
     infos[r].converged = status[r] == 1;
@@ -831,6 +872,54 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     infos[r].samples_drawn = smp[r];
     infos[r].collisions_found = colls[r];
   }
+  return CSD_OK;
+}
+
+int csd_eizo_get_rounds(const csd_eizo *w, int32_t region, csd_eizo_round_info *out,
+                        int32_t capacity, int32_t *count) {
+  if (!w || !count) return fail(CSD_ERR_INVALID_ARGUMENT, "null argument");
+  if (region < 0 || region >= w->last_regions)
+    return fail(CSD_ERR_INVALID_ARGUMENT, "region outside the last csd_eizo_run");
+  const int n = std::min(w->last_rounds[region], w->D.T);
+  *count = n;
+  if (n == 0 || !out) return CSD_OK;
+  if (capacity < n) return fail(CSD_ERR_CAPACITY, "fewer entries than rounds");
+  std::vector<int32_t> st((size_t)n * kStats);
+  CSD_HIP(hipMemcpy(st.data(), w->D.stats + (size_t)region * w->D.T * kStats,
+                    st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; ++k) {
+    const int32_t *e = &st[(size_t)k * kStats];
+    csd_eizo_round_info &o = out[k];
+    o.iteration_index = k;
+    o.num_samples_drawn = e[kStatSamples];
+    o.collisions_threshold = (1 - w->opt.tau) * w->opt.epsilon * unadaptive_samples(w->opt, k);
+    o.num_collisions_unadaptive = e[kStatFirstCol];
+    o.num_collisions_found = e[kStatCol];
+    o.num_collisions_updated = e[kStatUpdated];
+    o.num_faces_placed = e[kStatFaces];
+    o.num_faces_at_start = e[kStatFacesStart];
+    o.pilot_samples = e[kStatPilot];
+    o.pilot_collisions = e[kStatPilotCol];
+    o.iteration_time_ms = k < (int)w->round_ms.size() ? w->round_ms[k] : 0.f;
+    o.pad = 0;
+  }
+  return CSD_OK;
+}
+
+int csd_eizo_last_particles(const csd_eizo *w, int32_t region, float *particles, float *scol,
+                            float *proj, int32_t capacity, int32_t *count) {
+  if (!w || !count) return fail(CSD_ERR_INVALID_ARGUMENT, "null argument");
+  if (region < 0 || region >= w->last_regions)
+    return fail(CSD_ERR_INVALID_ARGUMENT, "region outside the last csd_eizo_run");
+  int32_t u = 0;
+  CSD_HIP(hipMemcpy(&u, w->D.u + region, 4, hipMemcpyDeviceToHost));
+  *count = u;
+  if (u == 0 || (!particles && !scol && !proj)) return CSD_OK;  // count only
+  if (capacity < u) return fail(CSD_ERR_CAPACITY, "fewer rows than particles");
+  const size_t off = (size_t)region * w->U * w->dim, bytes = (size_t)u * w->dim * sizeof(float);
+  if (particles) CSD_HIP(hipMemcpy(particles, w->D.part + off, bytes, hipMemcpyDeviceToHost));
+  if (scol) CSD_HIP(hipMemcpy(scol, w->D.scol + off, bytes, hipMemcpyDeviceToHost));
+  if (proj) CSD_HIP(hipMemcpy(proj, w->D.proj + off, bytes, hipMemcpyDeviceToHost));
   return CSD_OK;
 }
 

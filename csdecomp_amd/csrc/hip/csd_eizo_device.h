@@ -176,6 +176,10 @@ struct EizoDev {
   int32_t *iters;          // [R]
   int64_t *samples;        // [R]
   int64_t *collisions;     // [R]
+  // per-round statistics [R][T][kStats] (null in the stage-wise entry
+  // points); round k of region r at (r * T + k) * kStats
+  int32_t *stats;
+  int T;
   // Device-resident rounds (graph-captured): null when the host drives the
   // rounds. ctl = {next round, active regions, current round, M_k, test
   // threshold bits}; mtab/ttab hold M_k and the threshold of every round.
@@ -183,6 +187,25 @@ struct EizoDev {
   const uint32_t *mtab;
   const float *ttab;
 };
+
+// Fields of a region's per-round statistics (EizoIterationInfo of
+// cuda_edge_inflation_zero_order.h:37-47, plus what guided rounds add).
+enum {
+  kStatSamples = 0,     // samples of the unadaptive test (0: test skipped)
+  kStatFirstCol = 1,    // collisions among the first M_k of them (-1: skipped)
+  kStatCol = 2,         // collisions among all of them
+  kStatUpdated = 3,     // particles bisected
+  kStatFaces = 4,       // faces placed
+  kStatFacesStart = 5,  // faces of the region when the round began
+  kStatPilot = 6,       // pilot samples
+  kStatPilotCol = 7,    // collisions among the pilot samples
+  kStats = 8
+};
+
+__device__ __forceinline__ int32_t *round_stats(const EizoDev &D, int r, uint32_t round) {
+  return D.stats && round < (uint32_t)D.T ? D.stats + ((size_t)r * D.T + round) * kStats
+                                          : nullptr;
+}
 
 // Projection parameter of p on the segment q1-q2 and the projected point
 // (projectSamplesOntoLineSegmentsKernel, cuda_set_builder_utils.cu:25-58),
@@ -287,8 +310,11 @@ __device__ __forceinline__ void argmin_pair(double &v, int &i, double ov, int oi
 
 static __global__ __launch_bounds__(kFaceBlock) void k_eizo_faces(EizoDev D, int dim, int U, int cap,
                                                            float margin, int max_faces,
-                                                           int by_slot) {
-  if (by_slot && D.ctl && blockIdx.x >= D.ctl[1]) return;  // slot past this round's regions
+                                                           int by_slot, uint32_t round) {
+  if (by_slot && D.ctl) {
+    if (blockIdx.x >= D.ctl[1]) return;  // slot past this round's regions
+    round = D.ctl[2];
+  }
   const int r = by_slot ? D.active[blockIdx.x] : blockIdx.x;
   if (D.status[r] != 0) return;
   const int u = D.u[r];
@@ -366,7 +392,10 @@ static __global__ __launch_bounds__(kFaceBlock) void k_eizo_faces(EizoDev D, int
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) D.faces_added[r] += added;
+  if (threadIdx.x == 0) {
+    D.faces_added[r] += added;
+    if (int32_t *st = round_stats(D, r, round)) st[kStatFaces] = added;
+  }
 }
 
 inline int fail(int code, const char *msg) {

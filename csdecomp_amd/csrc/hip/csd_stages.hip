@@ -9,12 +9,16 @@
 
 namespace csd {
 
-// standalone sampler: polytope p = blockIdx.y
-template <int DPL>
+// standalone sampler: polytope p = blockIdx.y. Without `ends` every chain
+// of polytope p starts at starts[p] and draws random stream p; with `ends`
+// chain i starts at fraction (i + 1/2) / n of the segment starts[p]-ends[p]
+// and draws stream p + stream_off, as the EI-ZO round's sampler does.
+template <int DPL, bool BROW = false>
 __global__ __launch_bounds__(kSampleBlock) void k_hit_and_run(
     const float *__restrict__ A, const float *__restrict__ b, const int32_t *__restrict__ nf,
-    int face_stride, int dim, const float *__restrict__ starts, int64_t n, int mixing,
-    uint32_t k0, uint32_t k1, uint32_t round, float *__restrict__ out) {
+    int face_stride, int dim, const float *__restrict__ starts, const float *__restrict__ ends,
+    uint32_t stream_off, int64_t n, int mixing, uint32_t k0, uint32_t k1, uint32_t round,
+    float *__restrict__ out) {
   constexpr int DP = SamplerShape<DPL>::DP;
   constexpr bool LDS = SamplerShape<DPL>::LDS;
   extern __shared__ float csd_smem[];
@@ -22,7 +26,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_hit_and_run(
   const int F = nf[p];
   const float *Ap = A + (size_t)p * face_stride * dim, *bp = b + (size_t)p * face_stride;
   if constexpr (LDS) {
-    stage_polytope<DP>(csd_smem, csd_smem + (size_t)F * DP, Ap, bp, F, dim);
+    stage_polytope<DP, BROW>(csd_smem, csd_smem + (size_t)F * DP, Ap, bp, F, dim);
     __syncthreads();
     Ap = csd_smem;
     bp = csd_smem + (size_t)F * DP;
@@ -30,9 +34,20 @@ __global__ __launch_bounds__(kSampleBlock) void k_hit_and_run(
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float x[DP];
+  if (ends) {
+    const float f = ((float)i + 0.5f) / (float)n;
 #pragma unroll
-  for (int d = 0; d < DP; ++d) x[d] = d < dim ? starts[(size_t)p * dim + d] : 0.f;
-  hit_and_run<DP, LDS>(Ap, bp, F, dim, x, (uint32_t)i, (uint32_t)p, round, mixing, k0, k1);
+    for (int d = 0; d < DP; ++d) {
+      const float a = d < dim ? starts[(size_t)p * dim + d] : 0.f;
+      const float e = d < dim ? ends[(size_t)p * dim + d] : 0.f;
+      x[d] = fmaf(f, e - a, a);
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < DP; ++d) x[d] = d < dim ? starts[(size_t)p * dim + d] : 0.f;
+  }
+  hit_and_run<DP, LDS, BROW>(Ap, bp, F, dim, x, (uint32_t)i,
+                             (uint32_t)p + (ends ? stream_off : 0u), round, mixing, k0, k1);
 #pragma unroll
   for (int d = 0; d < DP; ++d)
     if (d < dim) out[((size_t)p * n + i) * dim + d] = x[d];
@@ -230,10 +245,12 @@ using namespace csd;
 
 extern "C" {
 
-int csd_hit_and_run(const float *A, const float *b, const int32_t *num_faces, int32_t face_stride,
-                    int32_t dim, int32_t num_polytopes, const float *starts, int64_t num_samples,
-                    int32_t mixing_steps, uint64_t seed, uint32_t round, float *out,
-                    void *stream) {
+namespace {
+int launch_hit_and_run(const float *A, const float *b, const int32_t *num_faces,
+                       int32_t face_stride, int32_t dim, int32_t num_polytopes,
+                       const float *starts, const float *ends, uint32_t stream_offset,
+                       int64_t num_samples, int32_t mixing_steps, uint64_t seed, uint32_t round,
+                       float *out, void *stream) {
   if (dim < 1 || dim > kDimMax) return fail(CSD_ERR_CAPACITY, "sampler supports 1..16 dims");
   if (num_polytopes < 0 || num_samples < 0 || mixing_steps < 0 || face_stride < 0)
     return fail(CSD_ERR_INVALID_ARGUMENT, "negative size");
@@ -244,20 +261,42 @@ int csd_hit_and_run(const float *A, const float *b, const int32_t *num_faces, in
   dim3 g((unsigned)((num_samples + kSampleBlock - 1) / kSampleBlock), (unsigned)num_polytopes);
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   hipStream_t st = (hipStream_t)stream;
-  // polytopes of up to face_stride rows go to LDS when they fit
-  if (dim <= 8 && polytope_lds_bytes<8>(face_stride) <= kLdsLimit)
-    hipLaunchKernelGGL((k_hit_and_run<17>), g, dim3(kSampleBlock), polytope_lds_bytes<8>(face_stride),
-                       st, A, b, num_faces, face_stride, dim, starts, num_samples, mixing_steps,
-                       k0, k1, round, out);
-  else if (polytope_lds_bytes<16>(face_stride) <= kLdsLimit)
-    hipLaunchKernelGGL((k_hit_and_run<33>), g, dim3(kSampleBlock),
-                       polytope_lds_bytes<16>(face_stride), st, A, b, num_faces, face_stride, dim,
-                       starts, num_samples, mixing_steps, k0, k1, round, out);
-  else
-    hipLaunchKernelGGL((k_hit_and_run<32>), g, dim3(kSampleBlock), 0, st, A, b, num_faces,
-                       face_stride, dim, starts, num_samples, mixing_steps, k0, k1, round, out);
+  const dim3 blk(kSampleBlock);
+  // polytopes of up to face_stride rows go to LDS when they fit, with b in
+  // the rows' padding slot when there is one (the EI-ZO sampler's shapes)
+#define CSD_HNR(DPL, BROW, LDS)                                                                 \
+  hipLaunchKernelGGL((k_hit_and_run<DPL, BROW>), g, blk, LDS, st, A, b, num_faces, face_stride, \
+                     dim, starts, ends, stream_offset, num_samples, mixing_steps, k0, k1, round, \
+                     out)
+  const size_t l8 = polytope_lds_bytes<8>(face_stride), l16 = polytope_lds_bytes<16>(face_stride);
+  if (dim <= 7 && l8 <= kLdsLimit) CSD_HNR(17, true, l8);
+  else if (dim <= 8 && l8 <= kLdsLimit) CSD_HNR(17, false, l8);
+  else if (dim <= 15 && l16 <= kLdsLimit) CSD_HNR(33, true, l16);
+  else if (l16 <= kLdsLimit) CSD_HNR(33, false, l16);
+  else CSD_HNR(32, false, 0);
+#undef CSD_HNR
   CSD_HIP(hipGetLastError());
   return CSD_OK;
+}
+}  // namespace
+
+int csd_hit_and_run(const float *A, const float *b, const int32_t *num_faces, int32_t face_stride,
+                    int32_t dim, int32_t num_polytopes, const float *starts, int64_t num_samples,
+                    int32_t mixing_steps, uint64_t seed, uint32_t round, float *out,
+                    void *stream) {
+  return launch_hit_and_run(A, b, num_faces, face_stride, dim, num_polytopes, starts, nullptr, 0u,
+                            num_samples, mixing_steps, seed, round, out, stream);
+}
+
+int csd_hit_and_run_segments(const float *A, const float *b, const int32_t *num_faces,
+                             int32_t face_stride, int32_t dim, int32_t num_polytopes,
+                             const float *segment_start, const float *segment_end,
+                             int64_t num_samples, int32_t mixing_steps, uint64_t seed,
+                             uint32_t stream_offset, uint32_t round, float *out, void *stream) {
+  if (!segment_end) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  return launch_hit_and_run(A, b, num_faces, face_stride, dim, num_polytopes, segment_start,
+                            segment_end, stream_offset, num_samples, mixing_steps, seed, round,
+                            out, stream);
 }
 
 int csd_bisect(const csd_plant *plant, const float *particles, const float *line_start,
@@ -354,7 +393,7 @@ int csd_place_faces(float *A, float *b, int32_t *num_faces, int32_t capacity, in
   D.faces_added = tmp + 2;
   if (e == hipSuccess)
     hipLaunchKernelGGL(k_eizo_faces, dim3(1), dim3(kFaceBlock), 0, st, D, dim,
-                       std::max(num_particles, 1), capacity, margin, max_faces, 0);
+                       std::max(num_particles, 1), capacity, margin, max_faces, 0, 0u);
   if (e == hipSuccess) e = hipGetLastError();
   int32_t res[4] = {0, 0, 0, 0};
   if (e == hipSuccess) e = hipMemcpyAsync(res, tmp, sizeof(res), hipMemcpyDeviceToHost, st);

@@ -416,6 +416,20 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def_readwrite("verbose", &EizoOptions::verbose)
       .def_readwrite("track_iteration_information", &EizoOptions::track_iteration_information);
 
+  py::class_<EizoIterationInfo>(m, "EizoIterationInfo")
+      .def_readonly("iteration_index", &EizoIterationInfo::iteration_index)
+      .def_readonly("num_samples_drawn", &EizoIterationInfo::num_samples_drawn)
+      .def_readonly("collisions_threshold", &EizoIterationInfo::collisions_threshold)
+      .def_readonly("num_collisions_unadaptive", &EizoIterationInfo::num_collisions_unadaptive)
+      .def_readonly("num_collisions_updated", &EizoIterationInfo::num_collisions_updated)
+      .def_readonly("num_faces_placed", &EizoIterationInfo::num_faces_placed)
+      .def_readonly("iteration_time_ms", &EizoIterationInfo::iteration_time_ms)
+      .def_readonly("num_collisions_found", &EizoIterationInfo::num_collisions_found)
+      .def_readonly("num_faces_at_start", &EizoIterationInfo::num_faces_at_start)
+      .def_readonly("num_pilot_samples", &EizoIterationInfo::num_pilot_samples)
+      .def_readonly("num_pilot_collisions", &EizoIterationInfo::num_pilot_collisions);
+  m.def("printEizoSummary", &printEizoSummary, py::arg("info"));
+
   py::class_<EizoInfo>(m, "EizoInfo")
       .def(py::init<>())
       .def_readonly("iterations", &EizoInfo::iterations)
@@ -423,7 +437,8 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def_readonly("face_capacity_reached", &EizoInfo::face_capacity_reached)
       .def_readonly("plain_fallback", &EizoInfo::plain_fallback)
       .def_readonly("samples_drawn", &EizoInfo::samples_drawn)
-      .def_readonly("faces_added", &EizoInfo::faces_added);
+      .def_readonly("faces_added", &EizoInfo::faces_added)
+      .def_readonly("iteration_info", &EizoInfo::iteration_info);
 
   py::class_<CudaEdgeInflator>(m, "CudaEdgeInflator")
       .def(py::init<const MinimalPlant &, const std::vector<GeometryIndex> &, const EizoOptions &,
@@ -473,6 +488,15 @@ PYBIND11_MODULE(_pycsdecomp, m) {
            "replay one graph-captured round per launch; the host reads the region status "
            "every rounds_per_sync rounds (0: host-driven rounds)")
       .def("last_info", &CudaEdgeInflator::lastInfo)
+      .def(
+          "last_particles",
+          [](const CudaEdgeInflator &self, int region) {
+            EizoParticles p = self.lastParticles(region);
+            return py::make_tuple(from_mat(p.particles), from_mat(p.scol), from_mat(p.proj));
+          },
+          py::arg("region") = 0,
+          "(particles, scol, proj), each dim x count: region's particles in its last round of "
+          "the last inflateEdge(s) call, their bisected boundary points and projections")
       .def("face_capacity", &CudaEdgeInflator::faceCapacity);
 
   m.def(

@@ -991,6 +991,32 @@ std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, cons
                             voxels.data.data(), voxels.cols, voxel_radius, A.data(), b.data(),
                             inf.data()),
                "EI-ZO");
+  std::vector<std::vector<EizoIterationInfo>> rounds(R);
+  const bool track = options_.track_iteration_information && infos;
+  auto collect = [&](int k, int r) {
+    int32_t n = 0;
+    check_status(csd_eizo_get_rounds(ws_, k, nullptr, 0, &n), "EI-ZO rounds");
+    std::vector<csd_eizo_round_info> v(n);
+    if (n) check_status(csd_eizo_get_rounds(ws_, k, v.data(), n, &n), "EI-ZO rounds");
+    const int base = (int)rounds[r].size();
+    for (const csd_eizo_round_info &e : v) {
+      EizoIterationInfo it;
+      it.iteration_index = base + e.iteration_index;
+      it.num_samples_drawn = e.num_samples_drawn;
+      it.collisions_threshold = (int)e.collisions_threshold;
+      it.num_collisions_unadaptive = e.num_collisions_unadaptive;
+      it.num_collisions_updated = e.num_collisions_updated;
+      it.num_faces_placed = e.num_faces_placed;
+      it.iteration_time_ms = e.iteration_time_ms;
+      it.num_collisions_found = e.num_collisions_found;
+      it.num_faces_at_start = e.num_faces_at_start;
+      it.num_pilot_samples = e.pilot_samples;
+      it.num_pilot_collisions = e.pilot_collisions;
+      rounds[r].push_back(it);
+    }
+  };
+  if (track)
+    for (int r = 0; r < R; ++r) collect(r, r);
   std::vector<char> fell_back(R, 0);
   if (plain_fallback_ && (pilot_samples_ || surrogate_)) {
     // plain EI-ZO for the regions the guided rounds left uncertified, in the
@@ -1036,6 +1062,7 @@ std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, cons
         inf[r].samples_drawn += g.samples_drawn;
         inf[r].collisions_found += g.collisions_found;
         fell_back[r] = 1;
+        if (track) collect(k, r);  // the plain rounds follow the guided ones
       }
     }
   }
@@ -1058,9 +1085,50 @@ std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, cons
       e.samples_drawn = inf[r].samples_drawn;
       e.faces_added = inf[r].faces_added;
       e.plain_fallback = fell_back[r] != 0;
+      e.iteration_info = std::move(rounds[r]);
     }
   }
   return out;
+}
+
+EizoParticles CudaEdgeInflator::lastParticles(int region) const {
+  int32_t n = 0;
+  check_status(csd_eizo_last_particles(ws_, region, nullptr, nullptr, nullptr, 0, &n),
+               "EI-ZO particles");
+  const int dim = domain_.ambient_dimension();
+  EizoParticles p;
+  p.particles = MatrixXf(dim, n);
+  p.scol = MatrixXf(dim, n);
+  p.proj = MatrixXf(dim, n);
+  if (n)
+    check_status(csd_eizo_last_particles(ws_, region, p.particles.data.data(), p.scol.data.data(),
+                                         p.proj.data.data(), n, &n),
+                 "EI-ZO particles");
+  return p;
+}
+
+void printEizoSummary(const EizoInfo &info) {
+  printf("EI-ZO summary\n=============================\nTotal iterations: %zu\n\n",
+         info.iteration_info.size());
+  printf("%-8s%-10s%-12s%-10s%-12s%-8s%-12s\n", "Iter.", "Samples", "Col. (UT)", "Thresh.",
+         "Col. (OPT)", "Faces", "Time (ms)");
+  printf("%s\n", std::string(72, '-').c_str());
+  long long samples = 0, col_ut = 0, col_opt = 0, faces = 0;
+  double ms = 0.0;
+  for (const EizoIterationInfo &it : info.iteration_info) {
+    printf("%-8d%-10d%-12d%-10d%-12d%-8d%-12.2f\n", it.iteration_index, it.num_samples_drawn,
+           it.num_collisions_unadaptive, it.collisions_threshold, it.num_collisions_updated,
+           it.num_faces_placed, it.iteration_time_ms);
+    samples += it.num_samples_drawn;
+    col_ut += std::max(it.num_collisions_unadaptive, 0);
+    col_opt += it.num_collisions_updated;
+    faces += it.num_faces_placed;
+    ms += it.iteration_time_ms;
+  }
+  printf("%s\nTotals:\n  Samples drawn: %lld\n  Collisions (UT): %lld\n  Collisions (OPT): %lld\n"
+         "  Faces placed: %lld\n  Total time: %.2f ms\n",
+         std::string(72, '-').c_str(), samples, col_ut, col_opt, faces, ms);
+  fflush(stdout);
 }
 
 HPolyhedron CudaEdgeInflator::inflateEdge(const std::vector<float> &line_start,

@@ -287,6 +287,9 @@ struct EizoOptions {
   bool track_iteration_information{true};
 };
 
+// One round (cuda_edge_inflation_zero_order.h:37-47). The first seven
+// fields are the reference's; num_collisions_unadaptive is -1 in a guided
+// round whose pilot skipped the test. points_post_bisection is not kept.
 struct EizoIterationInfo {
   int iteration_index = 0;
   int num_samples_drawn = 0;
@@ -295,6 +298,11 @@ struct EizoIterationInfo {
   int num_collisions_updated = 0;
   int num_faces_placed = 0;
   float iteration_time_ms = 0.f;
+  // batched and guided rounds
+  int num_collisions_found = 0;  // among all samples of the test
+  int num_faces_at_start = 0;
+  int num_pilot_samples = 0;
+  int num_pilot_collisions = 0;
 };
 
 struct EizoInfo {
@@ -312,6 +320,16 @@ struct EizoInfo {
 };
 
 uint32_t numSamplesUnadaptive(float epsilon, float delta, float tau);
+
+// Round-by-round table of an EizoInfo (printEizoSummary,
+// cuda_edge_inflation_zero_order.h:168-213).
+void printEizoSummary(const EizoInfo &info);
+
+// The particles of one region in its last round: colliding samples, their
+// bisected boundary points and their projections, each dim x count.
+struct EizoParticles {
+  MatrixXf particles, scol, proj;
+};
 
 class CudaEdgeInflator {
  public:
@@ -340,6 +358,8 @@ class CudaEdgeInflator {
   // rounds. Same polytopes either way.
   void setGraph(int rounds_per_sync);
   const EizoInfo &lastInfo() const { return last_info_; }
+  // region r of the last inflateEdge(s) call, in its last round
+  EizoParticles lastParticles(int region) const;
   int faceCapacity() const { return face_capacity_; }
 
  private:

@@ -180,6 +180,21 @@ int csd_hit_and_run(const float *A, const float *b, const int32_t *num_faces,
                     int32_t mixing_steps, uint64_t seed, uint32_t round,
                     float *out, void *stream);
 
+/* The EI-ZO round's sampler on its own (no counterpart in the reference,
+ * whose chains all start at line_start, cuda_edge_inflation_zero_order.cu
+ * :122-131): as csd_hit_and_run, but chain i of polytope p starts at
+ * fraction (i + 1/2) / num_samples of the segment segment_start[p] ->
+ * segment_end[p] (device [num_polytopes][dim] each) and draws counters
+ * (i, step, p + stream_offset, round*16 + k). With p the region's index in
+ * a csd_eizo_run batch, stream_offset 0, round k and num_samples the
+ * round's sample count max(M_k, budget), it yields that round's samples bit
+ * for bit. */
+int csd_hit_and_run_segments(const float *A, const float *b, const int32_t *num_faces,
+                             int32_t face_stride, int32_t dim, int32_t num_polytopes,
+                             const float *segment_start, const float *segment_end,
+                             int64_t num_samples, int32_t mixing_steps, uint64_t seed,
+                             uint32_t stream_offset, uint32_t round, float *out, void *stream);
+
 /* ---- EI-ZO ----
  * Options: field-for-field EizoOptions
  * (planning/cuda_edge_inflation_zero_order.h:18-32). */
@@ -239,6 +254,42 @@ int csd_eizo_run(csd_eizo *ws, const float *domain_A, const float *domain_b,
                  const uint32_t *particle_budgets, const float *voxels,
                  int32_t num_voxels, float voxel_radius, float *out_A,
                  float *out_b, csd_eizo_region_info *infos);
+
+/* One round of one region in the last csd_eizo_run: the fields of
+ * EizoIterationInfo (planning/cuda_edge_inflation_zero_order.h:37-47,
+ * filled at cuda_edge_inflation_zero_order.cu:181-195,309-336) plus what the
+ * batched and guided rounds add. */
+typedef struct csd_eizo_round_info {
+  int32_t iteration_index;
+  int32_t num_samples_drawn;         /* samples of the unadaptive test (0: skipped) */
+  float collisions_threshold;        /* (1 - tau) epsilon M_k */
+  int32_t num_collisions_unadaptive; /* among the first M_k samples (-1: skipped) */
+  int32_t num_collisions_found;      /* among all samples of the test */
+  int32_t num_collisions_updated;    /* particles bisected */
+  int32_t num_faces_placed;
+  int32_t num_faces_at_start;        /* faces of the region when the round began */
+  int32_t pilot_samples;             /* csd_eizo_set_guidance pilot */
+  int32_t pilot_collisions;
+  float iteration_time_ms;           /* wall time of the round, shared by the batch */
+  int32_t pad;
+} csd_eizo_round_info;
+
+/* Per-round record of `region` in the last csd_eizo_run (replaces the
+ * reference's EizoInfo::iteration_info). *count receives the region's
+ * rounds; out (host, capacity entries, may be NULL to only count) receives
+ * them. */
+int csd_eizo_get_rounds(const csd_eizo *ws, int32_t region, csd_eizo_round_info *out,
+                        int32_t capacity, int32_t *count);
+
+/* The particles of `region` in its last round of the last csd_eizo_run, as
+ * EizoGpuMemory holds them for C++ callers after executeEizo
+ * (cuda_edge_inflation_zero_order.h:59-161): the colliding samples taken
+ * (particles), their bisected boundary points (scol) and their projections
+ * onto the segment (proj), each [count][dim] on the host (any may be NULL;
+ * all NULL only counts). count is 0 when the region's last round certified
+ * it. */
+int csd_eizo_last_particles(const csd_eizo *ws, int32_t region, float *particles, float *scol,
+                            float *proj, int32_t capacity, int32_t *count);
 
 struct csd_mlp;
 

@@ -783,21 +783,21 @@ static float u01(uint32_t x) {
 /* hit-and-run -- cuda_hit_and_run_sampling.cu:74-112                        */
 /* ------------------------------------------------------------------------ */
 #define ORC_MAX_DIM 64
-void orc_hit_and_run(const float *A, const float *b, int32_t F, int32_t dim,
-                     const float *start, int64_t num_chains,
-                     int64_t chain_offset, int32_t mixing_steps, uint64_t seed,
-                     uint32_t stream, uint32_t round, float *out) {
+/* one chain of `mixing_steps` steps from x (in place); its randomness comes
+ * from counters (chain, step, stream, round*16 + call) */
+static void hnr_chain(const float *A, const float *b, int32_t F, int32_t dim,
+                      float *x, uint32_t chain, int32_t mixing_steps,
+                      uint64_t seed, uint32_t stream, uint32_t round) {
   const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   const int npairs = (dim + 1) / 2;
   const int nuni = 2 * npairs + 1; /* normals + one for theta */
   const int ncalls = (nuni + 3) / 4;
-  for (int64_t i = 0; i < num_chains; ++i) {
-    float x[ORC_MAX_DIM], dir[ORC_MAX_DIM + 1], u[32];
-    for (int d = 0; d < dim; ++d) x[d] = start[d];
+  {
+    const uint32_t i = chain;
+    float dir[ORC_MAX_DIM + 1], u[32];
     for (int s = 0; s < mixing_steps; ++s) {
       for (int c = 0; c < ncalls; ++c) {
-        uint32_t ctr[4] = {(uint32_t)(chain_offset + i), (uint32_t)s, stream,
-                           round * 16u + (uint32_t)c};
+        uint32_t ctr[4] = {i, (uint32_t)s, stream, round * 16u + (uint32_t)c};
         uint32_t o[4];
         orc_philox4x32_10(ctr, key, o);
         for (int k = 0; k < 4; ++k) u[4 * c + k] = u01(o[k]);
@@ -832,7 +832,34 @@ void orc_hit_and_run(const float *A, const float *b, int32_t F, int32_t dim,
                             (double)theta_min);
       for (int d = 0; d < dim; ++d) x[d] = x[d] + theta * dir[d];
     }
-    for (int d = 0; d < dim; ++d) out[i * dim + d] = x[d];
+  }
+}
+
+void orc_hit_and_run(const float *A, const float *b, int32_t F, int32_t dim,
+                     const float *start, int64_t num_chains,
+                     int64_t chain_offset, int32_t mixing_steps, uint64_t seed,
+                     uint32_t stream, uint32_t round, float *out) {
+  for (int64_t i = 0; i < num_chains; ++i) {
+    float *x = out + i * dim;
+    for (int d = 0; d < dim; ++d) x[d] = start[d];
+    hnr_chain(A, b, F, dim, x, (uint32_t)(chain_offset + i), mixing_steps, seed,
+              stream, round);
+  }
+}
+
+/* chains spread along the segment ls -> le: chain c of `total` starts at
+ * fraction (c + 1/2) / total (fmaf(f, le - ls, ls)) */
+void orc_hit_and_run_segment(const float *A, const float *b, int32_t F,
+                             int32_t dim, const float *ls, const float *le,
+                             int64_t num_chains, int64_t chain_offset,
+                             int64_t total, int32_t mixing_steps, uint64_t seed,
+                             uint32_t stream, uint32_t round, float *out) {
+  for (int64_t i = 0; i < num_chains; ++i) {
+    const uint32_t c = (uint32_t)(chain_offset + i);
+    const float f = ((float)c + 0.5f) / (float)total;
+    float *x = out + i * dim;
+    for (int d = 0; d < dim; ++d) x[d] = fmaf(f, le[d] - ls[d], ls[d]);
+    hnr_chain(A, b, F, dim, x, c, mixing_steps, seed, stream, round);
   }
 }
 
@@ -1096,7 +1123,7 @@ int32_t orc_eizo(const orc_plant *p, const int32_t *robot_geoms,
                  int32_t num_voxels, float voxel_radius, float *A, float *b,
                  int32_t *num_faces, int32_t cap, const float *ls,
                  const float *le, const orc_eizo_options *opt, uint32_t stream,
-                 int rule, orc_eizo_info *info) {
+                 int rule, int spread_starts, orc_eizo_info *info) {
   const int dim = p->num_config_vars;
   info->iterations = 0;
   info->converged = 0;
@@ -1108,8 +1135,13 @@ int32_t orc_eizo(const orc_plant *p, const int32_t *robot_geoms,
     uint32_t N = M > opt->num_particles ? M : opt->num_particles;
     float *samples = (float *)malloc(sizeof(float) * (size_t)N * dim);
     uint8_t *flags = (uint8_t *)malloc(N);
-    orc_hit_and_run(A, b, *num_faces, dim, ls, N, 0, (int32_t)opt->mixing_steps,
-                    opt->seed, stream, it, samples);
+    if (spread_starts)
+      orc_hit_and_run_segment(A, b, *num_faces, dim, ls, le, N, 0, N,
+                              (int32_t)opt->mixing_steps, opt->seed, stream, it,
+                              samples);
+    else
+      orc_hit_and_run(A, b, *num_faces, dim, ls, N, 0, (int32_t)opt->mixing_steps,
+                      opt->seed, stream, it, samples);
     info->samples_drawn += N;
     orc_check_voxels_batch(p, samples, N, voxels, num_voxels, voxel_radius,
                            robot_geoms, num_robot_geoms, rule, flags, 0);

@@ -97,6 +97,17 @@ void orc_hit_and_run(const float *A, const float *b, int32_t num_faces,
                      int64_t chain_offset, int32_t mixing_steps, uint64_t seed,
                      uint32_t stream, uint32_t round, float *out /*[n][dim]*/);
 
+/* chain c = chain_offset + i of `total` starts at fraction (c + 1/2) / total
+ * of the segment ls -> le, with counters (c, step, stream, round*16 + call)
+ * (the batched EI-ZO round's sampler; the reference starts every chain at
+ * line_start, cuda_edge_inflation_zero_order.cu:122-131) */
+void orc_hit_and_run_segment(const float *A, const float *b, int32_t num_faces,
+                             int32_t dim, const float *ls, const float *le,
+                             int64_t num_chains, int64_t chain_offset,
+                             int64_t total, int32_t mixing_steps, uint64_t seed,
+                             uint32_t stream, uint32_t round,
+                             float *out /*[n][dim]*/);
+
 /* ---- EI-ZO building blocks (cuda_set_builder_utils.*) ---- */
 uint32_t orc_num_samples_unadaptive(float epsilon, float delta, float tau);
 float orc_delta_k(float delta, int32_t iteration);
@@ -146,13 +157,15 @@ typedef struct {
 } orc_eizo_info;
 
 /* full EI-ZO for one edge; A/b hold the domain on entry (num_faces rows)
- * and the region on exit. Returns 0 ok, -1 capacity overflow. */
+ * and the region on exit. Returns 0 ok, -1 capacity overflow.
+ * spread_starts = 0: every chain starts at ls (the reference);
+ * 1: the batched round's spread starts (orc_hit_and_run_segment). */
 int32_t orc_eizo(const orc_plant *p, const int32_t *robot_geoms,
                  int32_t num_robot_geoms, const float *voxels,
                  int32_t num_voxels, float voxel_radius, float *A, float *b,
                  int32_t *num_faces, int32_t cap, const float *ls,
                  const float *le, const orc_eizo_options *opt, uint32_t stream,
-                 int rule, orc_eizo_info *info);
+                 int rule, int spread_starts, orc_eizo_info *info);
 
 #ifdef __cplusplus
 }

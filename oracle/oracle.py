@@ -70,6 +70,9 @@ def _load():
                                      np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")]),
         "orc_hit_and_run": (None, [_f32p, _f32p, C.c_int32, C.c_int32, _f32p, C.c_int64,
                                    C.c_int64, C.c_int32, C.c_uint64, C.c_uint32, C.c_uint32, _f32p]),
+        "orc_hit_and_run_segment": (None, [_f32p, _f32p, C.c_int32, C.c_int32, _f32p, _f32p,
+                                           C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_uint64,
+                                           C.c_uint32, C.c_uint32, _f32p]),
         "orc_num_samples_unadaptive": (C.c_uint32, [C.c_float, C.c_float, C.c_float]),
         "orc_delta_k": (C.c_float, [C.c_float, C.c_int32]),
         "orc_project_onto_segment": (None, [_f32p, _f32p, _f32p, C.c_int32, _f32p, C.c_void_p]),
@@ -84,7 +87,8 @@ def _load():
                                         _f32p, _f32p, C.c_int32, _f32p, _f32p, C.c_float]),
         "orc_eizo": (C.c_int32, [P, _i32p, C.c_int32, C.c_void_p, C.c_int32, C.c_float, _f32p,
                                  _f32p, C.POINTER(C.c_int32), C.c_int32, _f32p, _f32p,
-                                 C.POINTER(EizoOptions), C.c_uint32, C.c_int, C.POINTER(EizoInfo)]),
+                                 C.POINTER(EizoOptions), C.c_uint32, C.c_int, C.c_int,
+                                 C.POINTER(EizoInfo)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -249,6 +253,20 @@ def hit_and_run(A, b, start, n, mixing, seed=1337, stream=0, round_=0, chain_off
     return out
 
 
+def hit_and_run_segment(A, b, ls, le, n, mixing, seed=1337, stream=0, round_=0, chain_offset=0,
+                        total=None):
+    """Chains spread along ls -> le (orc_hit_and_run_segment): chain c of
+    `total` (default n) starts at fraction (c + 1/2) / total."""
+    A = np.ascontiguousarray(A, np.float32)
+    F, dim = A.shape
+    out = np.zeros((n, dim), np.float32)
+    lib().orc_hit_and_run_segment(A, np.ascontiguousarray(b, np.float32), F, dim,
+                                  np.ascontiguousarray(ls, np.float32),
+                                  np.ascontiguousarray(le, np.float32), n, chain_offset,
+                                  n if total is None else total, mixing, seed, stream, round_, out)
+    return out
+
+
 def num_samples_unadaptive(eps, delta, tau):
     return lib().orc_num_samples_unadaptive(eps, delta, tau)
 
@@ -352,7 +370,8 @@ def edit_regions(plant, collisions, segment_idxs, line_starts, line_ends, region
     return out, proj, scol
 
 
-def eizo(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, stream=0, rule=RULE_GPU):
+def eizo(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, stream=0, rule=RULE_GPU,
+         spread_starts=False):
     dim = plant.dof
     A = np.zeros((cap, dim), np.float32)
     b = np.zeros(cap, np.float32)
@@ -365,7 +384,7 @@ def eizo(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, stream=0, rule
     rg = np.ascontiguousarray(robot_geoms, np.int32)
     rc = lib().orc_eizo(plant.ref, rg, len(rg), None, 0, 0.0, A.reshape(-1), b, C.byref(nf), cap,
                         np.ascontiguousarray(ls, np.float32), np.ascontiguousarray(le, np.float32),
-                        C.byref(o), stream, rule, C.byref(info))
+                        C.byref(o), stream, rule, int(spread_starts), C.byref(info))
     if rc != 0:
         raise RuntimeError("oracle EI-ZO face capacity exceeded")
     return A[:nf.value], b[:nf.value], info
