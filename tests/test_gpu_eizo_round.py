@@ -255,3 +255,106 @@ def test_first_rounds_parity_dual_arm(gpu, orc):
         if seen == 2:
             break
     assert seen >= 1
+
+
+def _fresh_fraction(gpu, mp, P, ls, le, n=20000, stream=0x30000000):
+    """Collision fraction (GPU checker) of n fresh samples of the
+    certificate's own distribution: 40-step chains spread along the segment,
+    random streams no round uses."""
+    import csdecomp_amd as csd
+    A, b = np.asarray(P.A(), np.float32), np.asarray(P.b(), np.float32)
+    s = _round_samples(gpu, A, b, ls[:, None], le[:, None], n, round_=0, seed=4242,
+                       stream_offset=stream)[0]
+    return 1.0 - float(np.mean(csd.CheckCollisionFreeCuda(np.asfortranarray(s.T), mp)))
+
+
+def _uniform_fraction(mp, P, ls, le, n=20000):
+    """Collision fraction of n samples of 200-step chains from the segment
+    midpoint: closer to uniform over the region than 40 steps."""
+    import csdecomp_amd as csd
+    c = ((ls + le) / 2).astype(np.float32)
+    s = csd.UniformSampleInHPolyhedraCuda([P], c[:, None], n, 200, seed=77)[0]
+    return 1.0 - float(np.mean(csd.CheckCollisionFreeCuda(s, mp)))
+
+
+def _region_properties(gpu, mp, regions, infos, ls, le, eps, what):
+    """Both endpoints inside every region; certified regions pass a fresh
+    test of their certificate's distribution; the round records add up."""
+    fractions = []
+    for r, (P, info) in enumerate(zip(regions, infos)):
+        assert P.PointInSet(ls[:, r], 1e-5) and P.PointInSet(le[:, r], 1e-5), (what, r)
+        rounds = info.iteration_info
+        assert len(rounds) == info.iterations, (what, r)
+        assert sum(x.num_faces_placed for x in rounds) == info.faces_added, (what, r)
+        assert sum(x.num_samples_drawn for x in rounds) == info.samples_drawn, (what, r)
+        if info.converged:
+            last = rounds[-1]
+            assert last.num_collisions_unadaptive <= last.collisions_threshold + 1, (what, r)
+            f = _fresh_fraction(gpu, mp, P, ls[:, r], le[:, r])
+            assert f < 2.5 * eps, (what, r, f)
+            fractions.append(f)
+        else:
+            assert info.face_capacity_reached or info.iterations == len(rounds), (what, r)
+    return fractions
+
+
+def _graph_equals_host(csd, mp, pl, opt, dom, ls, le, budgets, cap, regions, infos, what):
+    inf = csd.CudaEdgeInflator(mp, list(pl.getRobotGeometryIds()), opt, dom, ls.shape[1], 10, cap)
+    inf.set_graph(4)
+    g_regions, g_infos = inf.inflateEdges(ls, le, None, 0.0, budgets)
+    for r in range(ls.shape[1]):
+        np.testing.assert_array_equal(g_regions[r].A(), regions[r].A(), err_msg=f"{what} {r}")
+        np.testing.assert_array_equal(g_regions[r].b(), regions[r].b(), err_msg=f"{what} {r}")
+        assert (g_infos[r].iterations, g_infos[r].converged, g_infos[r].samples_drawn) == \
+            (infos[r].iterations, infos[r].converged, infos[r].samples_drawn), (what, r)
+        assert [x.num_faces_placed for x in g_infos[r].iteration_info] == \
+            [x.num_faces_placed for x in infos[r].iteration_info], (what, r)
+
+
+def test_single_edge_workload_properties(gpu, orc):
+    """The single-edge workload's own options (epsilon 1e-3, delta 0.05,
+    tau 0.5, 4,096 particles, 10 bisection steps, <= 10 faces per round,
+    <= 100 faces beyond the domain) on 16 of its segments, to the end."""
+    from csdecomp_amd import scenes
+    import csdecomp_amd as csd
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    dom, A0, _ = _domain(csd, pl)
+    ls, le = scenes.free_segments(pl, 16, 3001, (0.2, 1.5))
+    cap = A0.shape[0] + 100
+    opt = _options(csd, 4096, 500)
+    inf = csd.CudaEdgeInflator(mp, list(pl.getRobotGeometryIds()), opt, dom, 16, 10, cap)
+    regions, infos = inf.inflateEdges(ls, le, None, 0.0, [])
+    fr = _region_properties(gpu, mp, regions, infos, ls, le, EPS, "arm7")
+    assert len(fr) >= 10  # most regions certify at this face budget
+    # the uniform check is not the certificate's distribution; in 7 dims the
+    # 40-step chains reach the region well enough that it holds too
+    for r, (P, info) in enumerate(zip(regions, infos)):
+        if info.converged:
+            assert _uniform_fraction(mp, P, ls[:, r], le[:, r]) < 3 * EPS, r
+    _graph_equals_host(csd, mp, pl, opt, dom, ls, le, [], cap, regions, infos, "arm7 graph")
+
+
+def test_batched_dual_arm_workload_properties(gpu, orc):
+    """The batched workload: 14-DOF dual arm, segments log-uniform in
+    [0.05, 2] rad, budgets log-uniform in [512, 16384], epsilon 1e-3, <= 100
+    faces beyond the domain; 24 regions in one batch, host-driven and
+    graph-captured rounds."""
+    from csdecomp_amd import scenes
+    import csdecomp_amd as csd
+    pl = scenes.dual_arm_scene(1002, 72, 24)
+    mp = pl.getMinimalPlant()
+    dom, A0, _ = _domain(csd, pl)
+    R = 24
+    ls, le = scenes.free_segments(pl, R, 3002, (0.05, 2.0), log_uniform=True)
+    budgets = scenes.log_uniform_budgets(3003, R)
+    assert min(budgets) < 2000 and max(budgets) > 5000  # uneven
+    cap = A0.shape[0] + 100
+    opt = _options(csd, 16384, 500)
+    inf = csd.CudaEdgeInflator(mp, list(pl.getRobotGeometryIds()), opt, dom, R, 10, cap)
+    regions, infos = inf.inflateEdges(ls, le, None, 0.0, budgets)
+    fr = _region_properties(gpu, mp, regions, infos, ls, le, EPS, "dual arm")
+    assert len(fr) >= 1, [i.converged for i in infos]
+    assert all(i.converged or i.face_capacity_reached for i in infos)
+    _graph_equals_host(csd, mp, pl, opt, dom, ls, le, budgets, cap, regions, infos,
+                       "dual arm graph")
