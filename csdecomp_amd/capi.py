@@ -40,6 +40,8 @@ _SIGS = {
                                            _U64, _U32, _U32, _VP, _VP]),
     "csd_bisect": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _I32, _I32, _VP, _I32, _F, _VP, _I32, _VP, _VP, _VP]),
     "csd_place_faces": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _VP, _VP, _I32, _VP, _VP, _F, _I32, _VP, _VP]),
+    "csd_place_faces_noalloc": (C.c_int, [_VP, _VP, _VP, _I32, _I32, _VP, _VP, _I32, _VP, _VP,
+                                          _F, _I32, _VP, _VP, _VP]),
     "csd_edit_regions": (C.c_int, [_VP, _VP, _VP, _VP, _I32, _VP, _VP, _I32, _VP, _VP, _VP, _I32,
                                    _VP, _I32, _F, _VP, _I32, _F, _I32, _VP, _VP, _VP, _VP]),
     "csd_validate_edges": (C.c_int, [_VP, _VP, _I32, _VP, _I64, _F, _I32, _VP, _VP]),
@@ -273,6 +275,17 @@ def place_faces(A, b, nf, cap, dim, scol, proj, n, ls, le, margin, max_faces, ad
     check(lib().csd_place_faces(_dptr(A), _dptr(b), _dptr(nf), cap, dim, _dptr(scol), _dptr(proj), n,
                                 _dptr(ls), _dptr(le), margin, max_faces, _dptr(added), _stream(stream)),
           "csd_place_faces")
+
+
+def place_faces_noalloc(A, b, nf, cap, dim, scol, proj, n, ls, le, margin, max_faces, added,
+                        status=None, stream=None):
+    """csd_place_faces_noalloc: the same faces without allocating (fixed
+    scratch of the library, at most 65536 particles); asynchronous, the
+    status (0, or 2 for face capacity exhausted) goes to `status`."""
+    check(lib().csd_place_faces_noalloc(_dptr(A), _dptr(b), _dptr(nf), cap, dim, _dptr(scol),
+                                        _dptr(proj), n, _dptr(ls), _dptr(le), margin, max_faces,
+                                        _dptr(added), _dptr(status), _stream(stream)),
+          "csd_place_faces_noalloc")
 
 
 def edit_regions(plant, collisions, seg_idx, offsets, num_collisions, ls, le, num_regions, A, b,

@@ -4,6 +4,7 @@
 // The batched driver in csd_eizo.hip runs the same device code fused per
 // round.
 #include <algorithm>
+#include <mutex>
 
 #include "csd_bisect.h"
 
@@ -239,6 +240,53 @@ int launch_bisect(const csd_plant *plant, const float *particles, const float *l
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }
+// csd_place_faces_noalloc: its scratch (the four counters of the one-region
+// EizoDev, then the FP64 distance of every particle) lives in this
+// file-scope device array instead of per-call allocations. Each GPU holds
+// its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device.
+constexpr int kNoAllocMaxParticles = 65536;
+constexpr size_t kNoAllocCounterBytes = 256;
+constexpr size_t kNoAllocBytes = kNoAllocCounterBytes + (size_t)kNoAllocMaxParticles * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// counters: u = n, status = 0, faces added = 0
+__global__ void k_noalloc_init(int32_t *ctr, int n) {
+  if (threadIdx.x == 0) {
+    ctr[0] = n;
+    ctr[1] = 0;
+    ctr[2] = 0;
+    ctr[3] = 0;
+  }
+}
+
+__global__ void k_noalloc_finish(const int32_t *ctr, int32_t *added, int32_t *status) {
+  if (threadIdx.x == 0) {
+    *added = ctr[2];
+    if (status) *status = ctr[1];
+  }
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int noalloc_scratch(void **out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = 0;
+  CSD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CSD_ERR_CAPACITY, "device index above 63");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress (face placement scratch)");
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return CSD_OK;
+}
+
 }  // namespace csd
 
 using namespace csd;
@@ -403,6 +451,50 @@ int csd_place_faces(float *A, float *b, int32_t *num_faces, int32_t capacity, in
   hipFree(dist);
   if (e != hipSuccess) return hip_fail(e, "face placement");
   if (res[1] == 2) return fail(CSD_ERR_CAPACITY, "face capacity exhausted");
+  return CSD_OK;
+}
+
+int csd_place_faces_noalloc(float *A, float *b, int32_t *num_faces, int32_t capacity,
+                            int32_t dim, const float *scol, const float *proj,
+                            int32_t num_particles, const float *line_start,
+                            const float *line_end, float margin, int32_t max_faces,
+                            int32_t *added, int32_t *status, void *stream) {
+  if (dim < 1 || dim > kDimMax) return fail(CSD_ERR_CAPACITY, "face placement supports 1..16 dims");
+  if (num_particles < 0 || capacity < 1 || max_faces < 0)
+    return fail(CSD_ERR_INVALID_ARGUMENT, "bad size");
+  if (num_particles > kNoAllocMaxParticles)
+    return fail(CSD_ERR_CAPACITY,
+                "csd_place_faces_noalloc takes at most 65536 particles (its fixed scratch)");
+  if (!A || !b || !num_faces || !line_start || !line_end || !added ||
+      (num_particles && (!scol || !proj)))
+    return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
+  void *base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  int32_t *ctr = (int32_t *)base;
+  double *dist = (double *)((unsigned char *)base + kNoAllocCounterBytes);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_noalloc_init, dim3(1), dim3(64), 0, st, ctr, num_particles);
+  if (num_particles)
+    hipLaunchKernelGGL(k_face_prep, dim3((num_particles + 255) / 256), dim3(256), 0, st, scol,
+                       proj, num_particles, dim, dist);
+  EizoDev D;
+  memset(&D, 0, sizeof(D));
+  D.A = A;
+  D.b = b;
+  D.nf = num_faces;
+  D.ls = const_cast<float *>(line_start);
+  D.le = const_cast<float *>(line_end);
+  D.scol = const_cast<float *>(scol);
+  D.proj = const_cast<float *>(proj);
+  D.dist = dist;
+  D.u = ctr;
+  D.status = ctr + 1;
+  D.faces_added = ctr + 2;
+  hipLaunchKernelGGL(k_eizo_faces, dim3(1), dim3(kFaceBlock), 0, st, D, dim,
+                     std::max(num_particles, 1), capacity, margin, max_faces, 0, 0u);
+  hipLaunchKernelGGL(k_noalloc_finish, dim3(1), dim3(64), 0, st, ctr, added, status);
+  CSD_HIP(hipGetLastError());
   return CSD_OK;
 }
 

@@ -344,6 +344,24 @@ int csd_place_faces(float *A, float *b, int32_t *num_faces, int32_t capacity,
                     const float *line_end, float margin, int32_t max_faces,
                     int32_t *added, void *stream);
 
+/* csd_place_faces without allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It places
+ * the same faces by the same kernels, but keeps its scratch (the per-particle
+ * FP64 distances and four counters) in a fixed device array of the library,
+ * one per GPU, allocated with the library (520 KiB); so num_particles is at
+ * most 65536 (CSD_ERR_CAPACITY above). Fully asynchronous on `stream`,
+ * nothing is read back: *added (device) receives the faces added and
+ * *status (device, may be NULL) 0, or 2 when the face capacity ran out
+ * (csd_place_faces returns CSD_ERR_CAPACITY then). Every call on a GPU uses
+ * that GPU's one scratch array: calls that overlap on one GPU (on different
+ * streams or threads) race and must be serialised by the caller. */
+int csd_place_faces_noalloc(float *A, float *b, int32_t *num_faces, int32_t capacity,
+                            int32_t dim, const float *scol, const float *proj,
+                            int32_t num_particles, const float *line_start,
+                            const float *line_end, float margin, int32_t max_faces,
+                            int32_t *added, int32_t *status, void *stream);
+
 /* ---- region editing ----
  * Replaces EditRegionsCuda (planning/cuda_edit_regions.cu:44-300; declared
  * planning/cuda_edit_regions.h:58-67) below its host-side argument checks
