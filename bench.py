@@ -48,6 +48,10 @@ def algorithmic_flops_per_config(mp, plant):
     return 132 * j_rev + 21 * s_mov + 36 * p_sb + 9 * p_ss + 60 * p_bb
 
 
+def plant_dof(plant):
+    return len(plant.getPositionLowerLimits())
+
+
 def cpu_baseline(mp, q, seconds):
     """CPU restatement of checkCollisionFreeVoxels with no voxels (early exit,
     silent), OpenMP over configurations on all host cores."""
@@ -134,6 +138,25 @@ def train_surrogate(plant, n=2_000_000, batch=8192, epochs=3, seed=7):
                    "train_s": round(t2 - t1, 3), "final_mse": round(loss.item(), 5)}
 
 
+def eizo_region_flops(info, dim, mixing, steps, f_cfg):
+    """Algorithmic FLOPs of one region, summed over its rounds from the
+    kernels' own counts (EizoInfo.iteration_info):
+      N_k mix (2 F d + 3 F + 2 d)   hit-and-run: two dot products, the chord
+                                    bound and the step per face and step
+    + (N_k + u steps) F_cfg         collision checks of samples and bisection
+    + faces (2 d + u 2 d)           FP64 face placement and redundancy test
+    with N_k the samples of the round (test + pilot), F the region's faces
+    when the round began, u the particles bisected."""
+    tot = 0
+    for it in info.iteration_info:
+        nk = it.num_samples_drawn + it.num_pilot_samples
+        F, u, faces = it.num_faces_at_start, it.num_collisions_updated, it.num_faces_placed
+        tot += nk * mixing * (2 * F * dim + 3 * F + 2 * dim)
+        tot += (nk + u * steps) * f_cfg
+        tot += faces * (2 * dim + u * 2 * dim)
+    return tot
+
+
 def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_budget=100):
     """Regions/s of batched EI-ZO (epsilon 1e-3, delta 0.05, tau 0.5, 10
     bisection steps, 40 mixing steps, <= 10 faces per round, <= face_budget
@@ -150,6 +173,7 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
     opt.max_hyperplanes_per_iteration = 10
     opt.verbose = False
     R = starts.shape[1]
+    f_cfg = algorithmic_flops_per_config(mp, plant)
     out = {"segments": R, "dof": int(starts.shape[0])}
     faces = {}
     for mode in modes:
@@ -167,12 +191,19 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         el = time.perf_counter() - t0
         conv = np.array([i.converged for i in infos])
         faces[mode] = [(r.A(), r.b()) for r in regions]
+        flops = sum(eizo_region_flops(i, starts.shape[0], opt.mixing_steps, opt.bisection_steps,
+                                      f_cfg) for i in infos)
         out[mode] = {"seconds": round(el, 3), "regions_per_s": round(R / el, 2),
                      "certified_per_s": round(float(conv.sum()) / el, 2),
                      "certified_fraction": round(float(conv.mean()), 4),
                      "rounds_mean": round(float(np.mean([i.iterations for i in infos])), 2),
                      "faces_mean": round(float(np.mean([r.A().shape[0] - nd for r in regions])), 1),
-                     "samples": int(sum(i.samples_drawn for i in infos))}
+                     "samples": int(sum(i.samples_drawn for i in infos)),
+                     "roofline": {"bound": "valu", "achieved": flops / el / 1e12,
+                                  "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": flops / el / 1e12 / FP32_VECTOR_PEAK_TFLOPS,
+                                  "flops": flops, "flops_per_region": flops / R,
+                                  "f_cfg": f_cfg}}
         if mode == "guided_fallback":
             out[mode]["fallback_regions"] = int(sum(i.plain_fallback for i in infos))
     if "plain" in faces and "plain_graph" in faces:
@@ -527,6 +558,9 @@ def main():
     ap.add_argument("--no-eizo", action="store_true", help="skip the EI-ZO regions/s legs")
     ap.add_argument("--no-large", action="store_true", help="skip the large-batch checker leg")
     ap.add_argument("--eizo-segments", type=int, default=2048)
+    ap.add_argument("--shape", type=int, default=0,
+                    help="checker shape for the headline (0: autotune; 1, 2, 4, 11, 12, 14, 20: "
+                         "that shape, e.g. to profile it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions", "distance_field",
@@ -575,8 +609,9 @@ def main():
     def step():
         capi.check_collision_free(dp, dq, n, 1, n, out, stream)
 
-    # autotuning (outside the timed region): waves per group of 64 configurations
-    split = capi.tune_check(dp, dq, n, 1, n, out, stream)
+    # autotuning (outside the timed region): waves per group of 64
+    # configurations, one or two passes, or the screened checker
+    split = capi.tune_check(dp, dq, n, 1, n, out, stream, force=args.shape)
 
     for _ in range(args.warmup):
         step()
@@ -590,7 +625,10 @@ def main():
     torch.cuda.synchronize()
     ms = t0.elapsed_time(t1) / args.steps
 
-    # per-launch kernel duration (events around each launch, same stream)
+    # one launch per step, back to back on one stream: the HIP events around
+    # the timed region give the kernel's average launch duration
+    kern_ms = ms
+    # a launch on its own (events around each launch) for reference
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(min(args.steps, 50))]
     for a, b in evs:
@@ -598,16 +636,18 @@ def main():
         step()
         b.record(stream)
     torch.cuda.synchronize()
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    single_ms = float(np.median([a.elapsed_time(b) for a, b in evs]))
 
     free_frac = float(out.float().mean().item())
     f_cfg = algorithmic_flops_per_config(mp, plant)
     achieved_tflops = n * f_cfg / (kern_ms * 1e-3) / 1e12
+    # HBM bytes per launch of the shape timed here, from its rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (null for a shape not profiled)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "k_check_arm7_16box_traffic.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and n == 65536:
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("shapes", {}).get(str(split), {}).get("hbm_bytes_per_launch")
 
     line = {
         "metric": "collision-checked configs/sec",
@@ -634,7 +674,9 @@ def main():
                    "waves_per_64_configs": split if split < 10 else split - 10},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_VECTOR_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP32_VECTOR_PEAK_TFLOPS,
-                     "traffic": traffic, "flops_per_config": f_cfg, "kernel_ms": kern_ms},
+                     "traffic": traffic, "flops_per_config": f_cfg, "kernel_ms": kern_ms,
+                     "single_launch_ms": single_ms,
+                     "algorithmic_bytes": n * (4 * plant_dof(plant) + 1)},
     }
     line["graph_replay"] = graph_replay_benchmark(dp, dq, n, out.clone())
     if not args.no_large:
