@@ -474,6 +474,19 @@ def eizo_benchmarks(segments=2048, cpu=True):
     return res
 
 
+def eizo_plain_benchmarks(segments=256):
+    """Plain EI-ZO only (no surrogate, no training), `segments` segments of
+    each workload: the leg counter passes run over."""
+    from csdecomp_amd import scenes
+    arm = scenes.arm_scene(1001, 48, 16)
+    s1, e1 = scenes.free_segments(arm, segments, 3001, (0.2, 1.5))
+    dual = scenes.dual_arm_scene(1002, 72, 24)
+    s2, e2 = scenes.free_segments(dual, segments, 3002, (0.05, 2.0), log_uniform=True)
+    b2 = scenes.log_uniform_budgets(3003, segments)
+    return {"single_edge_arm7": eizo_leg("single", arm, s1, e1, [], 4096, ("plain",), None),
+            "batched_dualarm14": eizo_leg("batched", dual, s2, e2, b2, 16384, ("plain",), None)}
+
+
 def graph_replay_benchmark(dp, dq, n, ref_flags, launches=50, replays=5):
     """The headline check captured as one hipGraph of `launches` launches
     (torch.cuda.graph over the C-ABI call, which only launches kernels) and
@@ -563,14 +576,16 @@ def main():
                          "that shape, e.g. to profile it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--leg", choices=["eizo", "mlp", "roadmap", "edit_regions", "distance_field",
+    ap.add_argument("--leg", choices=["eizo", "eizo_plain", "mlp", "roadmap", "edit_regions", "distance_field",
                                       "voxel_maps", "large_batches"],
                     help="only run this secondary leg and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.leg:
         from csdecomp_amd import capi
         capi.init(0)
-        fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments, not args.no_cpu), "mlp": mlp_benchmark,
+        fn = {"eizo": lambda: eizo_benchmarks(args.eizo_segments, not args.no_cpu),
+              "eizo_plain": lambda: eizo_plain_benchmarks(args.eizo_segments),
+              "mlp": mlp_benchmark,
               "roadmap": roadmap_benchmark, "edit_regions": edit_regions_benchmark,
               "distance_field": distance_field_benchmark,
               "voxel_maps": voxel_maps_benchmark,
