@@ -411,6 +411,8 @@ struct csd_eizo {
   int last_regions = 0;
   std::vector<int32_t> last_rounds;
   std::vector<float> round_ms;
+  // csd_eizo_set_metric
+  double *metric_buf = nullptr;
 };
 
 using namespace csd;
@@ -555,6 +557,42 @@ int csd_eizo_set_guidance(csd_eizo *w, uint32_t pilot_samples, float pilot_facto
     w->keep = keep_particles;
     w->line_points = line_points;
   }
+  return CSD_OK;
+}
+
+int csd_eizo_set_metric(csd_eizo *w, const float *metric) {
+  if (!w) return fail(CSD_ERR_INVALID_ARGUMENT, "null workspace");
+  const int n = w->dim;
+  if (!metric) {
+    w->D.metric = nullptr;  // Euclidean: the reference's placement
+    return CSD_OK;
+  }
+  // symmetric positive definite: a Cholesky factorisation in FP64 succeeds
+  std::vector<double> M((size_t)n * n), L((size_t)n * n, 0.0);
+  for (int k = 0; k < n * n; ++k) M[k] = metric[k];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) {
+      const double a = M[i * n + j], b = M[j * n + i];
+      if (!(fabs(a - b) <= 1e-6 * std::max(1.0, std::max(fabs(a), fabs(b)))))
+        return fail(CSD_ERR_INVALID_ARGUMENT, "metric is not symmetric");
+    }
+  for (int j = 0; j < n; ++j) {
+    double d = M[j * n + j];
+    for (int k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
+    if (!(d > 0.0)) return fail(CSD_ERR_INVALID_ARGUMENT, "metric is not positive definite");
+    L[j * n + j] = sqrt(d);
+    for (int i = j + 1; i < n; ++i) {
+      double v = M[i * n + j];
+      for (int k = 0; k < j; ++k) v -= L[i * n + k] * L[j * n + k];
+      L[i * n + j] = v / L[j * n + j];
+    }
+  }
+  if (!w->metric_buf) {
+    const int rc = dalloc(w, &w->metric_buf, (size_t)kDimMax * kDimMax);
+    if (rc) return rc;
+  }
+  CSD_HIP(hipMemcpy(w->metric_buf, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+  w->D.metric = w->metric_buf;
   return CSD_OK;
 }
 

@@ -180,6 +180,9 @@ struct EizoDev {
   // points); round k of region r at (r * T + k) * kStats
   int32_t *stats;
   int T;
+  // ellipsoid metric of the face placement, dim x dim row-major FP64 (null:
+  // the Euclidean metric of the reference)
+  const double *metric;
   // Device-resident rounds (graph-captured): null when the host drives the
   // rounds. ctl = {next round, active regions, current round, M_k, test
   // threshold bits}; mtab/ttab hold M_k and the threshold of every round.
@@ -324,10 +327,29 @@ static __global__ __launch_bounds__(kFaceBlock) void k_eizo_faces(EizoDev D, int
   __shared__ double s_a[kDimMax];
   __shared__ double s_b;
   __shared__ int s_best, s_stop;
+  __shared__ double s_M[kDimMax * kDimMax];
   double *dist = D.dist + (size_t)r * U;
   const float *scol = D.scol + (size_t)r * U * dim;
   const float *proj = D.proj + (size_t)r * U * dim;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (D.metric) {
+    // particles are ordered by |scol - proj|_M = sqrt(e' M e)
+    for (int k = threadIdx.x; k < dim * dim; k += kFaceBlock) s_M[k] = D.metric[k];
+    __syncthreads();
+    for (int i = threadIdx.x; i < u; i += kFaceBlock) {
+      double e[kDimMax];
+      for (int d = 0; d < dim; ++d)
+        e[d] = (double)scol[(size_t)i * dim + d] - (double)proj[(size_t)i * dim + d];
+      double q = 0.0;
+      for (int a = 0; a < dim; ++a) {
+        double me = 0.0;
+        for (int c = 0; c < dim; ++c) me += s_M[a * dim + c] * e[c];
+        q += e[a] * me;
+      }
+      dist[i] = sqrt(q);
+    }
+    __syncthreads();
+  }
   int added = 0;
   for (; added < max_faces; ++added) {
     double v = INFINITY;
@@ -358,10 +380,19 @@ static __global__ __launch_bounds__(kFaceBlock) void k_eizo_faces(EizoDev D, int
         const float *sc = scol + (size_t)bi * dim, *pr = proj + (size_t)bi * dim;
         const float *ls = D.ls + (size_t)r * dim, *le = D.le + (size_t)r * dim;
         double a[kDimMax], nrm = 0.0;
-        for (int d = 0; d < dim; ++d) {
-          a[d] = (double)sc[d] - (double)pr[d];
-          nrm += a[d] * a[d];
+        for (int d = 0; d < dim; ++d) a[d] = (double)sc[d] - (double)pr[d];
+        if (D.metric) {
+          // normal M e: the hyperplane tangent at scol to the M-ellipsoid
+          // centred at proj
+          double e[kDimMax];
+          for (int d = 0; d < dim; ++d) e[d] = a[d];
+          for (int d = 0; d < dim; ++d) {
+            double v = 0.0;
+            for (int c = 0; c < dim; ++c) v += s_M[d * dim + c] * e[c];
+            a[d] = v;
+          }
         }
+        for (int d = 0; d < dim; ++d) nrm += a[d] * a[d];
         nrm = sqrt(nrm);
         const int f = D.nf[r];
         double bb = -(double)margin, v1 = 0.0, v2 = 0.0;

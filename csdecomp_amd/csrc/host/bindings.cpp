@@ -484,6 +484,15 @@ PYBIND11_MODULE(_pycsdecomp, m) {
           py::keep_alive<1, 4>(),
           "surrogate: a csdecomp_amd.capi.Mlp (kept alive by the inflator), the address of a "
           "csd_mlp, or None")
+      .def(
+          "set_metric",
+          [](CudaEdgeInflator &self, const py::object &m) {
+            self.setMetric(m.is_none() ? MatrixXf() : to_mat(m));
+          },
+          py::arg("metric") = py::none(),
+          "ellipsoid metric of the face placement: a symmetric positive definite dim x dim "
+          "matrix (faces tangent to M-ellipsoids around the segment), None for the Euclidean "
+          "metric of the reference")
       .def("set_graph", &CudaEdgeInflator::setGraph, py::arg("rounds_per_sync") = 1,
            "replay one graph-captured round per launch; the host reads the region status "
            "every rounds_per_sync rounds (0: host-driven rounds)")

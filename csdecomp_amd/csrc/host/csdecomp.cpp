@@ -955,6 +955,19 @@ void CudaEdgeInflator::setGuidance(uint32_t pilot_samples, float pilot_factor,
   plain_fallback_ = plain_fallback;
 }
 
+void CudaEdgeInflator::setMetric(const MatrixXf &metric) {
+  if (metric.rows == 0 && metric.cols == 0) {
+    check_status(csd_eizo_set_metric(ws_, nullptr), "EI-ZO metric");
+    return;
+  }
+  const int dim = domain_.ambient_dimension();
+  require(metric.rows == dim && metric.cols == dim, "metric must be dim x dim");
+  std::vector<float> rm((size_t)dim * dim);
+  for (int r = 0; r < dim; ++r)
+    for (int c = 0; c < dim; ++c) rm[(size_t)r * dim + c] = metric(r, c);
+  check_status(csd_eizo_set_metric(ws_, rm.data()), "EI-ZO metric");
+}
+
 void CudaEdgeInflator::setGraph(int rounds_per_sync) {
   check_status(csd_eizo_set_graph(ws_, rounds_per_sync), "EI-ZO device-resident rounds");
 }

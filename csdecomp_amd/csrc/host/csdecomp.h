@@ -357,6 +357,10 @@ class CudaEdgeInflator {
   // reads the region status every rounds_per_sync rounds; 0 for host-driven
   // rounds. Same polytopes either way.
   void setGraph(int rounds_per_sync);
+  // Ellipsoid metric of the face placement (csd_eizo_set_metric): a
+  // symmetric positive definite dim x dim matrix, or an empty one for the
+  // Euclidean metric of the reference.
+  void setMetric(const MatrixXf &metric);
   const EizoInfo &lastInfo() const { return last_info_; }
   // region r of the last inflateEdge(s) call, in its last round
   EizoParticles lastParticles(int region) const;

@@ -311,6 +311,18 @@ int csd_eizo_set_guidance(csd_eizo *ws, uint32_t pilot_samples, float pilot_fact
                           const struct csd_mlp *surrogate, int32_t keep_particles,
                           int32_t line_points);
 
+/* Ellipsoid metric of the face placement for later csd_eizo_run calls (no
+ * counterpart in the reference, whose faces are placed in the Euclidean
+ * metric, cuda_set_builder_utils.cu:359-424). metric (host, dim x dim
+ * row-major, symmetric positive definite; NULL restores the Euclidean
+ * metric, the default): each round takes its particles in ascending
+ * |scol - proj|_M = sqrt(e' M e), e = scol - proj, and places the face with
+ * the unit normal M e / |M e|, the hyperplane tangent at scol to the
+ * M-ellipsoid centred at proj; b = a.scol - margin, relaxed to keep the
+ * segment inside as before. In FP64 on the device. CSD_ERR_INVALID_ARGUMENT
+ * if the matrix is not symmetric positive definite. */
+int csd_eizo_set_metric(csd_eizo *ws, const float *metric);
+
 /* Device-resident rounds for later csd_eizo_run calls (no counterpart in the
  * reference, whose executeEizo launches every stage from the host and reads
  * counters back between them, cuda_edge_inflation_zero_order.cu:109-306).

@@ -930,17 +930,29 @@ static float round_up_f(double v) {
 
 /* greedy face placement: cuda_edge_inflation_zero_order.cu:268-306 with
  * maybePlaceFace / checkParticlesMadeRedundantByLastFace
- * (cuda_set_builder_utils.cu:359-424), placement and redundancy in FP64. */
-int32_t orc_place_faces(float *A, float *b, int32_t *num_faces, int32_t cap,
-                        int32_t dim, const float *scol, const float *proj,
-                        int32_t u, const float *ls, const float *le,
-                        float margin, int32_t max_faces) {
+ * (cuda_set_builder_utils.cu:359-424), placement and redundancy in FP64.
+ * With a metric M (SPD, dim x dim row-major; NULL = identity, the
+ * reference), particles are taken in ascending |e|_M = sqrt(e' M e),
+ * e = scol - proj, and the face normal is M e normalised to unit length:
+ * the hyperplane tangent at scol to the M-ellipsoid centred at proj. */
+int32_t orc_place_faces_metric(float *A, float *b, int32_t *num_faces,
+                               int32_t cap, int32_t dim, const float *scol,
+                               const float *proj, int32_t u, const float *ls,
+                               const float *le, float margin,
+                               int32_t max_faces, const double *M) {
   double *dist = (double *)malloc(sizeof(double) * (u > 0 ? u : 1));
   for (int i = 0; i < u; ++i) {
-    double s = 0;
-    for (int d = 0; d < dim; ++d) {
-      double e = (double)scol[i * dim + d] - (double)proj[i * dim + d];
-      s += e * e;
+    double e[ORC_MAX_DIM], s = 0;
+    for (int d = 0; d < dim; ++d)
+      e[d] = (double)scol[i * dim + d] - (double)proj[i * dim + d];
+    if (M) {
+      for (int r = 0; r < dim; ++r) {
+        double me = 0;
+        for (int c = 0; c < dim; ++c) me += M[r * dim + c] * e[c];
+        s += e[r] * me;
+      }
+    } else {
+      for (int d = 0; d < dim; ++d) s += e[d] * e[d];
     }
     dist[i] = sqrt(s);
   }
@@ -959,10 +971,16 @@ int32_t orc_place_faces(float *A, float *b, int32_t *num_faces, int32_t cap,
       return -1;
     }
     const float *sc = scol + best * dim, *pr = proj + best * dim;
-    double a[ORC_MAX_DIM], nrm = 0;
-    for (int d = 0; d < dim; ++d) {
-      a[d] = (double)sc[d] - (double)pr[d];
-      nrm += a[d] * a[d];
+    double e[ORC_MAX_DIM], a[ORC_MAX_DIM], nrm = 0;
+    for (int d = 0; d < dim; ++d) e[d] = (double)sc[d] - (double)pr[d];
+    for (int r = 0; r < dim; ++r) {
+      if (M) {
+        a[r] = 0;
+        for (int c = 0; c < dim; ++c) a[r] += M[r * dim + c] * e[c];
+      } else {
+        a[r] = e[r];
+      }
+      nrm += a[r] * a[r];
     }
     nrm = sqrt(nrm);
     float af[ORC_MAX_DIM];
@@ -992,6 +1010,14 @@ int32_t orc_place_faces(float *A, float *b, int32_t *num_faces, int32_t cap,
   }
   free(dist);
   return added;
+}
+
+int32_t orc_place_faces(float *A, float *b, int32_t *num_faces, int32_t cap,
+                        int32_t dim, const float *scol, const float *proj,
+                        int32_t u, const float *ls, const float *le,
+                        float margin, int32_t max_faces) {
+  return orc_place_faces_metric(A, b, num_faces, cap, dim, scol, proj, u, ls, le,
+                                margin, max_faces, NULL);
 }
 
 /* region editing (cuda_edit_regions.cu:236-293) for one region: the

@@ -127,6 +127,15 @@ int32_t orc_place_faces(float *A, float *b, int32_t *num_faces, int32_t cap,
                         int32_t u, const float *ls, const float *le,
                         float margin, int32_t max_faces);
 
+/* the same with an ellipsoid metric M (SPD, dim x dim row-major, NULL =
+ * identity): particles in ascending sqrt(e' M e), e = scol - proj, face
+ * normal M e / |M e| */
+int32_t orc_place_faces_metric(float *A, float *b, int32_t *num_faces,
+                               int32_t cap, int32_t dim, const float *scol,
+                               const float *proj, int32_t u, const float *ls,
+                               const float *le, float margin,
+                               int32_t max_faces, const double *M);
+
 /* region editing for one region (cuda_edit_regions.cu:236-293): u bisected
  * collisions scol with projections proj; appends faces to A/b (capacity
  * cap). Returns faces added, or -1 on capacity overflow. */

@@ -80,6 +80,9 @@ def _load():
                               C.c_float, C.c_void_p, C.c_int32, C.c_int, _f32p]),
         "orc_place_faces": (C.c_int32, [_f32p, _f32p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                         _f32p, _f32p, C.c_int32, _f32p, _f32p, C.c_float, C.c_int32]),
+        "orc_place_faces_metric": (C.c_int32, [_f32p, _f32p, C.POINTER(C.c_int32), C.c_int32,
+                                               C.c_int32, _f32p, _f32p, C.c_int32, _f32p, _f32p,
+                                               C.c_float, C.c_int32, C.c_void_p]),
         "orc_voxelize": (None, [_f32p, C.c_int64, _f32p, C.c_float, C.c_int32, _u8p]),
         "orc_edt_brute": (None, [_u8p, C.c_int32, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                  C.c_int]),
@@ -293,15 +296,18 @@ def bisect(plant, particle, proj, steps, rule=RULE_GPU):
     return sc
 
 
-def place_faces(A, b, nf, cap, scol, proj, ls, le, margin, max_faces):
-    """A [cap][dim], b [cap] float32 arrays modified in place; returns added."""
+def place_faces(A, b, nf, cap, scol, proj, ls, le, margin, max_faces, metric=None):
+    """A [cap][dim], b [cap] float32 arrays modified in place; returns
+    (added, faces). metric: SPD dim x dim (None = identity, the reference)."""
     n = C.c_int32(nf)
     dim = A.shape[1]
-    added = lib().orc_place_faces(A.reshape(-1), b, C.byref(n), cap, dim,
-                                  np.ascontiguousarray(scol, np.float32).reshape(-1),
-                                  np.ascontiguousarray(proj, np.float32).reshape(-1),
-                                  len(scol), np.ascontiguousarray(ls, np.float32),
-                                  np.ascontiguousarray(le, np.float32), margin, max_faces)
+    M = None if metric is None else np.ascontiguousarray(metric, np.float64)
+    added = lib().orc_place_faces_metric(A.reshape(-1), b, C.byref(n), cap, dim,
+                                         np.ascontiguousarray(scol, np.float32).reshape(-1),
+                                         np.ascontiguousarray(proj, np.float32).reshape(-1),
+                                         len(scol), np.ascontiguousarray(ls, np.float32),
+                                         np.ascontiguousarray(le, np.float32), margin, max_faces,
+                                         None if M is None else M.ctypes.data_as(C.c_void_p))
     return added, n.value
 
 

@@ -358,3 +358,69 @@ def test_batched_dual_arm_workload_properties(gpu, orc):
     assert all(i.converged or i.face_capacity_reached for i in infos)
     _graph_equals_host(csd, mp, pl, opt, dom, ls, le, budgets, cap, regions, infos,
                        "dual arm graph")
+
+
+def test_ellipsoid_metric_face_placement(gpu, orc):
+    """Faces placed against an ellipsoid metric (csd_eizo_set_metric): the
+    GPU's FP64 placement on its own bisected points equals the oracle's
+    orc_place_faces_metric; the identity metric gives the Euclidean run bit
+    for bit; full runs with the metric keep the segments inside."""
+    from csdecomp_amd import scenes
+    import csdecomp_amd as csd
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    dom, A0, b0 = _domain(csd, pl)
+    R = 8
+    ls, le = scenes.free_segments(pl, R, 3001, (0.2, 1.5))
+    cap = A0.shape[0] + 100
+    rng = np.random.default_rng(5)
+    G = rng.normal(size=(7, 7))
+    M = (G @ G.T / 7 + np.diag([4.0, 4.0, 2.0, 2.0, 1.0, 1.0, 0.5])).astype(np.float32)
+    M = (M + M.T) / 2
+    rg = list(pl.getRobotGeometryIds())
+    inf = csd.CudaEdgeInflator(mp, rg, _options(csd, 4096, 1), dom, R, 10, cap)
+    inf.set_metric(M)
+    regions, infos = inf.inflateEdges(ls, le, None, 0.0, [])
+    placed = 0
+    for r in range(R):
+        parts, scol, proj = (np.ascontiguousarray(x.T) for x in inf.last_particles(r))
+        if parts.shape[0] == 0:
+            continue
+        A = np.zeros((cap, 7), np.float32)
+        b = np.zeros(cap, np.float32)
+        A[:14], b[:14] = A0, b0
+        added, nf = orc.place_faces(A, b, 14, cap, scol, proj, ls[:, r], le[:, r], MARGIN, 10,
+                                    metric=M.astype(np.float64))
+        gA, gb = np.asarray(regions[r].A()), np.asarray(regions[r].b())
+        assert gA.shape[0] == nf, r
+        np.testing.assert_allclose(gA, A[:nf], rtol=1e-4, atol=1e-6, err_msg=f"region {r}")
+        np.testing.assert_allclose(gb, b[:nf], rtol=1e-4, atol=1e-6, err_msg=f"region {r}")
+        # the metric changed the faces: they are not the Euclidean ones
+        Ae = np.zeros((cap, 7), np.float32)
+        be = np.zeros(cap, np.float32)
+        Ae[:14], be[:14] = A0, b0
+        orc.place_faces(Ae, be, 14, cap, scol, proj, ls[:, r], le[:, r], MARGIN, 10)
+        assert not np.allclose(Ae[14:nf], gA[14:], atol=1e-3)
+        placed += added
+    assert placed > 0
+    # identity metric == Euclidean, bit for bit; None restores the Euclidean metric
+    inf.set_metric(np.eye(7, dtype=np.float32))
+    r_id, _ = inf.inflateEdges(ls, le, None, 0.0, [])
+    inf.set_metric(None)
+    r_eu, _ = inf.inflateEdges(ls, le, None, 0.0, [])
+    for r in range(R):
+        np.testing.assert_array_equal(r_id[r].A(), r_eu[r].A())
+        np.testing.assert_array_equal(r_id[r].b(), r_eu[r].b())
+    with pytest.raises(ValueError):
+        inf.set_metric(np.diag([1, 1, 1, -1, 1, 1, 1]).astype(np.float32))
+    with pytest.raises(ValueError):
+        bad = np.eye(7, dtype=np.float32)
+        bad[0, 1] = 0.5
+        inf.set_metric(bad)
+    # a full run against the metric: segments inside, certified regions pass
+    # a fresh test of their distribution
+    inf2 = csd.CudaEdgeInflator(mp, rg, _options(csd, 4096, 500), dom, R, 10, cap)
+    inf2.set_metric(M)
+    regions2, infos2 = inf2.inflateEdges(ls, le, None, 0.0, [])
+    fr = _region_properties(gpu, mp, regions2, infos2, ls, le, EPS, "metric")
+    assert len(fr) >= 1
