@@ -217,7 +217,7 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
     return out
 
 
-MLP_FLOPS_PER_ROW = 2 * (256 * 32 + 3 * 256 * 256 + 256)  # padded input, 4 hidden layers, head
+MLP_FLOPS_PER_ROW = 2 * (256 * 7 + 3 * 256 * 256 + 256)  # 7 inputs, 4 hidden layers, head
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense
 
 

@@ -261,80 +261,366 @@ __global__ __launch_bounds__(kBlock, 2) void k_mlp_infer(MlpDev M, In in, int64_
 }
 
 // ---------------------------------------------------------------------------
-// Training: tiled NT GEMM  C[m][n] = sum_k A[m][k] B[n][k]  (bf16 in, fp32
-// out; A [M][lda], B [N][ldb], k contiguous). Block tile 128 x 128, four
-// waves of 64 x 64, K in LDS chunks of 32. blockIdx.z splits K: partial sums
-// go to C + z * M * ldc. Needs K, kper multiples of 32 and lda, ldb
-// multiples of 8.
+// Training, 10 launches per step at any batch B (a multiple of 128):
+//   k_train_prep            x0 = bf16(normalised q) as [B][32] and [32][B];
+//                           clears the small gradient accumulators
+//   k_train_fwd<kFwd> x3    layer l = 1..3: z = a_{l-1} W_l^T + b_l,
+//                           a_l = bf16(SiLU(z)) as [B][256] and [256][B],
+//                           s_l = SiLU'(z) fp32 [B][256]
+//   k_train_fwd<kHead>      layer 4, then per row y = w5 . a_4 + b5, the
+//                           loss, dy = 2 (y - t) / B and delta_4 =
+//                           dy w5 SiLU'(z_4) (bf16, both layouts), with the
+//                           sums of db4, dw5, db5 and the loss
+//   k_train_back x4         l = 4..1: dW_l = delta_l^T a_{l-1}, each block a
+//                           tile of dW_l over a share of the batch (partial
+//                           sums, no atomics); for l > 1, other blocks of the
+//                           same launch compute delta_{l-1} =
+//                           (delta_l W_l) * s_{l-1} and the sums of db_{l-1}
+//   k_train_update          the gradient (partials summed in a fixed order),
+//                           then Adam and the bf16 weight copies (or only
+//                           the gradient, for csd_mlp_gradients)
+// GEMM operands are bf16 [rows][k] with k contiguous, staged through LDS in
+// K chunks of 32 with the next chunk's global loads in flight while the
+// current one feeds the MFMAs.
 // ---------------------------------------------------------------------------
-constexpr int kGT = 128, kGK = 32, kGP = kGK + 8;
+constexpr int kKC = 32, kKP = kKC + 8;  // K chunk, LDS row pitch (bf16)
+constexpr int kRB = 32;                 // rows per block of the row GEMMs
+enum { kFwd = 0, kHead = 1 };
 
-__global__ __launch_bounds__(kBlock) void k_gemm_nt(const uint16_t *__restrict__ A, int lda,
-                                                    const uint16_t *__restrict__ B, int ldb,
-                                                    float *__restrict__ C, int ldc, int M, int N,
-                                                    int K, int kper) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[kGT * kGP];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[kGT * kGP];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
-  const int kbeg = blockIdx.z * kper, kend = min(K, kbeg + kper);
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-  const int hr = lane >> 5, c = lane & 31;
-  const int lr = t >> 1, lk = (t & 1) * 16;
-  f32x16 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x16{};
-  for (int k0 = kbeg; k0 < kend; k0 += kGK) {
-    uint4 a0 = {0, 0, 0, 0}, a1 = a0, b0 = a0, b1 = a0;
-    if (m0 + lr < M) {
-      const uint4 *p = reinterpret_cast<const uint4 *>(A + (size_t)(m0 + lr) * lda + k0 + lk);
-      a0 = p[0];
-      a1 = p[1];
-    }
-    if (n0 + lr < N) {
-      const uint4 *p = reinterpret_cast<const uint4 *>(B + (size_t)(n0 + lr) * ldb + k0 + lk);
-      b0 = p[0];
-      b1 = p[1];
-    }
-    __syncthreads();
-    *reinterpret_cast<uint4 *>(As + lr * kGP + lk) = a0;
-    *reinterpret_cast<uint4 *>(As + lr * kGP + lk + 8) = a1;
-    *reinterpret_cast<uint4 *>(Bs + lr * kGP + lk) = b0;
-    *reinterpret_cast<uint4 *>(Bs + lr * kGP + lk + 8) = b1;
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < kGK; ks += 16) {
-      bf16x8 af[2], bfr[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = ld8(As + (wm + 32 * i + c) * kGP + ks + 8 * hr);
-#pragma unroll
-      for (int j = 0; j < 2; ++j) bfr[j] = ld8(Bs + (wn + 32 * j + c) * kGP + ks + 8 * hr);
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = mfma(af[i], bfr[j], acc[i][j]);
+// small gradient accumulators (summed with atomics): b1..b4, w5, b5, loss
+constexpr int kAccB = 0, kAccW5 = kHid * kW, kAccB5 = kAccW5 + kW, kAccLoss = kAccB5 + 1;
+constexpr int kAccLen = kAccLoss + 1;
+
+struct TrainDev {
+  int B;
+  const uint16_t *a[kHid + 1];   // a_0 = x0 [B][32], a_l [B][256] (l <= 3)
+  const uint16_t *at[kHid + 1];  // transposes [K][B] (l <= 3)
+  uint16_t *aw[kHid + 1], *atw[kHid + 1];
+  float *s[kHid + 1];            // SiLU'(z_l) [B][256], l <= 3
+  uint16_t *d[2], *dt[2];        // delta_l (ping-pong), bf16 [B][256] and [256][B]
+  float *wparts[kHid + 1];       // dW_l partial sums [S_l][256][K_l]
+  int split[kHid + 1];           // S_l
+  float *acc;                    // [kAccLen]
+  const float *target;
+};
+
+// Arguments of one launch (plain pointers: a runtime index into the arrays
+// of TrainDev would put the kernel argument block in scratch memory).
+struct FwdArgs {
+  int B, K;
+  const uint16_t *A, *W;  // a_{l-1} [B][K], W_l [256][K]
+  const float *bias;      // b_l
+  uint16_t *a, *at;       // a_l [B][256], [256][B]        (kFwd)
+  float *s;               // SiLU'(z_l) [B][256]           (kFwd)
+  uint16_t *d, *dt;       // delta_4 [B][256], [256][B]   (kHead)
+  float *acc;
+  const float *target;
+};
+struct BackArgs {
+  int B, K, split, nrow;
+  const uint16_t *dt, *at;    // delta_l^T [256][B], a_{l-1}^T [K][B]
+  float *wparts;              // [split][256][K]
+  const uint16_t *d, *wt;     // delta_l [B][256], W_l^T [256][256]
+  const float *sprev;         // SiLU'(z_{l-1}) [B][256]
+  uint16_t *dn, *dnt;         // delta_{l-1}, both layouts
+  float *accb;                // db_{l-1} accumulators [256]
+};
+
+// K-chunk pipeline of one block: rows [r0, r0 + NA) of A and [c0, c0 + NB)
+// of Bop (each [rows][ld] bf16), chunk k0 of 32 columns, 16-byte pieces,
+// thread t taking pieces t, t + 256, ... (at most five, in named registers:
+// an array here was kept in scratch memory)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int NA, int NB>
+struct ChunkLoader {
+  static constexpr int kPieces = (NA + NB) * 4;
+  static constexpr int kPer = (kPieces + 255) / 256;
+  static_assert(kPer <= 5, "at most five pieces per thread");
+  u32x4 v0, v1, v2, v3, v4;
+  template <int J>
+  __device__ __forceinline__ u32x4 &reg() {
+    if constexpr (J == 0) return v0;
+    else if constexpr (J == 1) return v1;
+    else if constexpr (J == 2) return v2;
+    else if constexpr (J == 3) return v3;
+    else return v4;
+  }
+  template <int J>
+  __device__ __forceinline__ void load1(const uint16_t *A, int lda, int r0, const uint16_t *Bm,
+                                        int ldb, int c0, int k0) {
+    if constexpr (J < kPer) {
+      const int pc = threadIdx.x + 256 * J;
+      if (J * 256 + 256 <= kPieces || pc < kPieces) {
+        const int row = pc >> 2, part = pc & 3;
+        const uint16_t *src = row < NA ? A + (size_t)(r0 + row) * lda
+                                       : Bm + (size_t)(c0 + row - NA) * ldb;
+        reg<J>() = *reinterpret_cast<const u32x4 *>(src + k0 + part * 8);
+      }
     }
   }
-  float *Cz = C + (size_t)blockIdx.z * M * ldc;
+  template <int J>
+  __device__ __forceinline__ void store1(uint16_t *sA) {
+    if constexpr (J < kPer) {
+      const int pc = threadIdx.x + 256 * J;
+      if (J * 256 + 256 <= kPieces || pc < kPieces)
+        *reinterpret_cast<u32x4 *>(sA + (pc >> 2) * kKP + (pc & 3) * 8) = reg<J>();
+    }
+  }
+  __device__ __forceinline__ void load(const uint16_t *A, int lda, int r0, const uint16_t *Bm,
+                                       int ldb, int c0, int k0) {
+    load1<0>(A, lda, r0, Bm, ldb, c0, k0);
+    load1<1>(A, lda, r0, Bm, ldb, c0, k0);
+    load1<2>(A, lda, r0, Bm, ldb, c0, k0);
+    load1<3>(A, lda, r0, Bm, ldb, c0, k0);
+    load1<4>(A, lda, r0, Bm, ldb, c0, k0);
+  }
+  __device__ __forceinline__ void store(uint16_t *sA) {  // sA: NA + NB rows
+    store1<0>(sA);
+    store1<1>(sA);
+    store1<2>(sA);
+    store1<3>(sA);
+    store1<4>(sA);
+  }
+};
+
+// Row GEMM of one block: C[32][256] = A[r0.. r0+32][K] Bop[256][K]^T; wave w
+// holds columns 64 w .. 64 w + 63 as two 32x32 MFMA tiles.
+__device__ __forceinline__ void rows_gemm(const uint16_t *A, const uint16_t *Bm, int K, int r0,
+                                          uint16_t *lds, f32x16 (&acc)[2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hr = lane >> 5, c = lane & 31;
+  constexpr int kRows = kRB + kW;
+  ChunkLoader<kRB, kW> ld;
+  acc[0] = acc[1] = f32x16{};
+  const int nk = K / kKC;
+  ld.load(A, K, r0, Bm, K, 0, 0);
+  ld.store(lds);
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    uint16_t *cur = lds + (kc & 1) * kRows * kKP;
+    if (kc + 1 < nk) ld.load(A, K, r0, Bm, K, 0, (kc + 1) * kKC);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int ks = 0; ks < kKC; ks += 16) {
+      const bf16x8 af = ld8(cur + c * kKP + ks + 8 * hr);
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int row = m0 + wm + 32 * i + 8 * (e >> 2) + 4 * hr + (e & 3);
-        const int col = n0 + wn + 32 * j + c;
-        if (row < M && col < N) Cz[(size_t)row * ldc + col] = acc[i][j][e];
-      }
+      for (int j = 0; j < 2; ++j)
+        acc[j] = mfma(af, ld8(cur + (kRB + 64 * wave + 32 * j + c) * kKP + ks + 8 * hr), acc[j]);
+    }
+    if (kc + 1 < nk) ld.store(lds + ((kc + 1) & 1) * kRows * kKP);
+    __syncthreads();
+  }
 }
 
-// x0 [B][32] bf16 and its transpose [32][B]
-__global__ void k_prep_input(const float *__restrict__ q, int B, int in_dim,
+// 4 bf16 values of consecutive rows to the transposed layout [col][B]
+__device__ __forceinline__ void store_t4(uint16_t *t, int B, int col, int row,
+                                         const uint16_t (&h)[4]) {
+  uint2 v;
+  v.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+  v.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+  *reinterpret_cast<uint2 *>(t + (size_t)col * B + row) = v;
+}
+
+constexpr size_t kRowsLds = 2 * (size_t)(kRB + kW) * kKP;  // bf16 elements
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_train_fwd(FwdArgs T, MlpDev M) {
+  __shared__ __attribute__((aligned(16))) uint16_t lds[kRowsLds];
+  __shared__ float s_row[4][kRB];
+  __shared__ float s_dy[kRB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hr = lane >> 5, c = lane & 31;
+  const int K = T.K, B = T.B;
+  const int r0 = blockIdx.x * kRB;
+  f32x16 acc[2];
+  rows_gemm(T.A, T.W, K, r0, lds, acc);
+  const float *bias = T.bias;
+  float av[2][16], sv[2][16];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = 64 * wave + 32 * j + c;
+    const float bc = bias[col];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint16_t h[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = 4 * q + e;
+        const float z = acc[j][i] + bc;
+        const float ex = expf(-z);
+        const float sg = 1.f / (1.f + ex);
+        h[e] = f2bf(z / (1.f + ex));
+        av[j][i] = bf2f(h[e]);
+        sv[j][i] = sg * (1.f + z * (1.f - sg));
+      }
+      const int row = r0 + 8 * q + 4 * hr;
+      if (MODE == kFwd) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          T.a[(size_t)(row + e) * kW + col] = h[e];
+          T.s[(size_t)(row + e) * kW + col] = sv[j][4 * q + e];
+        }
+        store_t4(T.at, B, col, row, h);
+      }
+    }
+  }
+  if (MODE == kHead) {
+    // y = w5 . a4 + b5 per row: lanes of a half-wave hold 32 columns of the
+    // same 16 rows; then the four waves' column ranges
+    float part[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) part[i] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const float w = bf2f(M.w5b[64 * wave + 32 * j + c]);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) part[i] = fmaf(av[j][i], w, part[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 32);
+    }
+    if (c == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) s_row[wave][8 * (i >> 2) + 4 * hr + (i & 3)] = part[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < kRB) {  // lanes 0..31 of wave 0, one row each
+      const int r = threadIdx.x;
+      const float y = s_row[0][r] + s_row[1][r] + s_row[2][r] + s_row[3][r] +
+                      M.p[MlpLayout::kB5];
+      const float e = y - T.target[r0 + r];
+      const float dy = 2.f * e / (float)B;
+      s_dy[r] = dy;
+      float sdy = dy, se = e * e;
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1) {
+        sdy += __shfl_xor(sdy, o, 32);
+        se += __shfl_xor(se, o, 32);
+      }
+      if (r == 0) {
+        atomicAdd(T.acc + kAccB5, sdy);
+        atomicAdd(T.acc + kAccLoss, se);
+      }
+    }
+    __syncthreads();
+    // delta_4 = dy w5 SiLU'(z4); column sums of delta_4 (db4) and dy a4 (dw5)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 64 * wave + 32 * j + c;
+      const float w = bf2f(M.w5b[col]);
+      float sd = 0.f, sw = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        uint16_t h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = 4 * q + e, rr = 8 * q + 4 * hr + e;
+          const float dy = s_dy[rr];
+          const float dv = dy * w * sv[j][i];
+          sd += dv;
+          sw = fmaf(dy, av[j][i], sw);
+          h[e] = f2bf(dv);
+          T.d[(size_t)(r0 + rr) * kW + col] = h[e];
+        }
+        store_t4(T.dt, B, col, r0 + 8 * q + 4 * hr, h);
+      }
+      sd += __shfl_xor(sd, 32);
+      sw += __shfl_xor(sw, 32);
+      if (hr == 0) {
+        atomicAdd(T.acc + kAccB + (kHid - 1) * kW + col, sd);
+        atomicAdd(T.acc + kAccW5 + col, sw);
+      }
+    }
+  }
+}
+
+// Backward launch for layer l. Blocks [0, nrow): delta_{l-1} (l > 1); blocks
+// [nrow, nrow + tiles * S): dW_l tile t of split s.
+// dW tiles are 32 WA x 32 WB (WA * WB = 4 waves of one 32x32 MFMA tile).
+template <int WA, int WB>
+__device__ __forceinline__ void wgrad_block(const BackArgs &T, int blk, uint16_t *lds) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hr = lane >> 5, c = lane & 31;
+  constexpr int BM = 32 * WA, BN = 32 * WB;
+  const int K = T.K, B = T.B, S = T.split;
+  const int tn = K / BN;
+  const int tile = blk / S, sp = blk - tile * S;
+  const int m0 = (tile / tn) * BM, n0 = (tile % tn) * BN;
+  const int nch = B / kKC, cb = sp * nch / S, ce = (sp + 1) * nch / S;
+  const uint16_t *A = T.dt, *Bm = T.at;  // delta_l^T [256][B], a_{l-1}^T [K][B]
+  const int wa = wave / WB, wb = wave % WB;
+  ChunkLoader<BM, BN> ld;
+  f32x16 acc = f32x16{};
+  if (cb < ce) {
+    ld.load(A, B, m0, Bm, B, n0, cb * kKC);
+    ld.store(lds);
+  }
+  __syncthreads();
+  for (int kc = cb; kc < ce; ++kc) {
+    uint16_t *cur = lds + ((kc - cb) & 1) * (BM + BN) * kKP;
+    if (kc + 1 < ce) ld.load(A, B, m0, Bm, B, n0, (kc + 1) * kKC);
+#pragma unroll
+    for (int ks = 0; ks < kKC; ks += 16)
+      acc = mfma(ld8(cur + (32 * wa + c) * kKP + ks + 8 * hr),
+                 ld8(cur + (BM + 32 * wb + c) * kKP + ks + 8 * hr), acc);
+    if (kc + 1 < ce) ld.store(lds + ((kc + 1 - cb) & 1) * (BM + BN) * kKP);
+    __syncthreads();
+  }
+  float *out = T.wparts + (size_t)sp * kW * K;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int o = m0 + 32 * wa + 8 * (i >> 2) + 4 * hr + (i & 3);
+    out[(size_t)o * K + n0 + 32 * wb + c] = acc[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_train_back(BackArgs T) {
+  __shared__ __attribute__((aligned(16))) uint16_t lds[kRowsLds];
+  if ((int)blockIdx.x >= T.nrow) {
+    const int blk = blockIdx.x - T.nrow;
+    if (T.K == kIn) wgrad_block<4, 1>(T, blk, lds);
+    else wgrad_block<2, 2>(T, blk, lds);
+    return;
+  }
+  // delta_{l-1} = (delta_l W_l) * s_{l-1}: Bop = W_l^T [256][256]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hr = lane >> 5, c = lane & 31;
+  const int B = T.B, r0 = blockIdx.x * kRB;
+  f32x16 acc[2];
+  rows_gemm(T.d, T.wt, kW, r0, lds, acc);
+  const float *sprev = T.sprev;
+  uint16_t *dn = T.dn, *dnt = T.dnt;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int col = 64 * wave + 32 * j + c;
+    float sd = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint16_t h[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + 8 * q + 4 * hr + e;
+        const float dv = acc[j][4 * q + e] * sprev[(size_t)row * kW + col];
+        sd += dv;
+        h[e] = f2bf(dv);
+        dn[(size_t)row * kW + col] = h[e];
+      }
+      store_t4(dnt, B, col, r0 + 8 * q + 4 * hr, h);
+    }
+    sd += __shfl_xor(sd, 32);
+    if (hr == 0) atomicAdd(T.accb + col, sd);
+  }
+}
+
+// x0 [B][32] bf16 and its transpose [32][B]; clears the accumulators
+__global__ void k_train_prep(const float *__restrict__ q, int B, int in_dim,
                              const float *__restrict__ lo, const float *__restrict__ scale,
-                             uint16_t *__restrict__ x0, uint16_t *__restrict__ x0t) {
+                             uint16_t *__restrict__ x0, uint16_t *__restrict__ x0t,
+                             float *__restrict__ acc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kAccLen) acc[i] = 0.f;
   if (i >= B * kIn) return;
   const int r = i / kIn, k = i % kIn;
   const float x = k < in_dim ? (q[(size_t)r * in_dim + k] - lo[k]) * scale[k] - 1.f : 0.f;
@@ -343,118 +629,77 @@ __global__ void k_prep_input(const float *__restrict__ q, int B, int in_dim,
   x0t[(size_t)k * B + r] = h;
 }
 
-// z [B][256] += bias; a = bf16(SiLU(z)) as [B][256] and [256][B]
-__global__ void k_bias_silu(float *__restrict__ z, const float *__restrict__ bias, int B,
-                            uint16_t *__restrict__ a, uint16_t *__restrict__ at) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B * kW) return;
-  const int r = i / kW, k = i % kW;
-  const float v = z[i] + bias[k];
-  z[i] = v;
-  const uint16_t h = f2bf(silu(v));
-  a[i] = h;
-  at[(size_t)k * B + r] = h;
-}
+// The gradient of each parameter (dW partials summed over the splits in a
+// fixed order; biases, w5 and b5 from the accumulators) into g and, with
+// lr > 0, one Adam step (FP32 master, bias-corrected moments) with the bf16
+// copies of the updated weights. Thread 0 also writes the loss.
+struct UpdateArgs {
+  int B;
+  const float *wparts[kHid];  // dW_l partials, l = 1..4
+  int split[kHid];
+  const float *acc;
+};
 
-// head forward + backward, one row per thread: y = w5 . a4 + b5, loss term,
-// dy = 2 (y - t) / B, delta4 = dy w5 SiLU'(z4) (fp32 [B][256]).
-__global__ void k_head(const uint16_t *__restrict__ a4, const float *__restrict__ z4,
-                       const uint16_t *__restrict__ w5b, const float *__restrict__ p,
-                       const float *__restrict__ target, int B, float *__restrict__ y,
-                       float *__restrict__ dy, float *__restrict__ d4) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= B) return;
-  float s = 0.f;
-  for (int k = 0; k < kW; ++k) s = fmaf(bf2f(a4[(size_t)r * kW + k]), bf2f(w5b[k]), s);
-  const float yy = s + p[MlpLayout::kB5];
-  y[r] = yy;
-  const float g = 2.f * (yy - target[r]) / (float)B;
-  dy[r] = g;
-  for (int k = 0; k < kW; ++k)
-    d4[(size_t)r * kW + k] = g * bf2f(w5b[k]) * dsilu(z4[(size_t)r * kW + k]);
-}
-
-// delta [B][256] (fp32, in place: dA -> dA * SiLU'(z) when z != null), to
-// bf16 [B][256] and [256][B]
-__global__ void k_delta(float *__restrict__ d, const float *__restrict__ z, int B,
-                        uint16_t *__restrict__ db, uint16_t *__restrict__ dbt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B * kW) return;
-  const int r = i / kW, k = i % kW;
-  float v = d[i];
-  if (z) {
-    v *= dsilu(z[i]);
-    d[i] = v;
-  }
-  const uint16_t h = f2bf(v);
-  db[i] = h;
-  dbt[(size_t)k * B + r] = h;
-}
-
-// column sums of X [rows][cols] (cols <= 256): partial[blockIdx.x][c]
-__global__ void k_colsum(const float *__restrict__ X, int rows, int cols, int rows_per_block,
-                         float *__restrict__ partial) {
-  const int c = threadIdx.x;
-  if (c >= cols) return;
-  const int r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
-  float s = 0.f;
-  for (int r = r0; r < r1; ++r) s += X[(size_t)r * cols + c];
-  partial[(size_t)blockIdx.x * cols + c] = s;
-}
-
-// out[i] = sum_s parts[s][i] (fixed order)
-__global__ void k_sum_parts(const float *__restrict__ parts, int S, int64_t len,
-                            float *__restrict__ out) {
+__global__ void k_train_update(UpdateArgs T, float *__restrict__ p, float *__restrict__ g,
+                               float *__restrict__ m, float *__restrict__ v, float lr, float b1,
+                               float b2, float eps, float bc1, float bc2, uint16_t *wb1,
+                               uint16_t *wb2, uint16_t *wb3, uint16_t *wb4, uint16_t *wt2,
+                               uint16_t *wt3, uint16_t *wt4, uint16_t *w5b, float *loss) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
-  float s = 0.f;
-  for (int k = 0; k < S; ++k) s += parts[(size_t)k * len + i];
-  out[i] = s;
-}
-
-// dW5 [256] and db5 partial sums over rows
-__global__ void k_head_grads(const uint16_t *__restrict__ a4, const float *__restrict__ dy,
-                             int B, int rows_per_block, float *__restrict__ partial) {
-  const int c = threadIdx.x;  // 0..256 (256 = bias)
-  const int r0 = blockIdx.x * rows_per_block, r1 = min(B, r0 + rows_per_block);
-  float s = 0.f;
-  if (c < kW)
-    for (int r = r0; r < r1; ++r) s = fmaf(dy[r], bf2f(a4[(size_t)r * kW + c]), s);
-  else
-    for (int r = r0; r < r1; ++r) s += dy[r];
-  partial[(size_t)blockIdx.x * (kW + 1) + c] = s;
-}
-
-// loss = mean (y - t)^2, one block
-__global__ void k_loss(const float *__restrict__ y, const float *__restrict__ t, int B,
-                       float *__restrict__ loss) {
-  __shared__ float sh[kBlock];
-  float s = 0.f;
-  for (int i = threadIdx.x; i < B; i += kBlock) {
-    const float e = y[i] - t[i];
-    s = fmaf(e, e, s);
+  if (i == 0 && loss) *loss = T.acc[kAccLoss] / (float)T.B;
+  if (i >= (int64_t)MlpLayout::kTotal) return;
+  int l = 0, o = 0, k = 0;  // W_l [o][k], or l = 0
+  float gi;
+  if (i < (int64_t)MlpLayout::kB1) {
+    l = 1;
+  } else if (i >= (int64_t)MlpLayout::kW2 && i < (int64_t)MlpLayout::kB2) {
+    l = 2;
+  } else if (i >= (int64_t)MlpLayout::kW3 && i < (int64_t)MlpLayout::kB3) {
+    l = 3;
+  } else if (i >= (int64_t)MlpLayout::kW4 && i < (int64_t)MlpLayout::kB4) {
+    l = 4;
   }
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
+  if (l) {
+    const int K = MlpLayout::K(l);
+    const int64_t e = i - (int64_t)MlpLayout::w(l);
+    o = (int)(e / K);
+    k = (int)(e - (int64_t)o * K);
+    const float *wp = (l == 1 ? T.wparts[0] : l == 2 ? T.wparts[1] : l == 3 ? T.wparts[2]
+                                                                           : T.wparts[3]) + e;
+    const int S = l == 1 ? T.split[0] : l == 2 ? T.split[1] : l == 3 ? T.split[2] : T.split[3];
+    const size_t stride = (size_t)kW * K;
+    float s = 0.f;
+    for (int sp = 0; sp < S; ++sp) s += wp[sp * stride];
+    gi = s;
+  } else if (i == (int64_t)MlpLayout::kB5) {
+    gi = T.acc[kAccB5];
+  } else if (i >= (int64_t)MlpLayout::kW5) {
+    k = (int)(i - (int64_t)MlpLayout::kW5);
+    gi = T.acc[kAccW5 + k];
+  } else {
+    const int lb = i < (int64_t)MlpLayout::kW2 ? 1 : i < (int64_t)MlpLayout::kW3 ? 2
+                   : i < (int64_t)MlpLayout::kW4 ? 3 : 4;
+    gi = T.acc[kAccB + (lb - 1) * kW + (int)(i - (int64_t)MlpLayout::b(lb))];
   }
-  if (threadIdx.x == 0) *loss = sh[0] / (float)B;
-}
-
-// Adam (FP32 master), bias-corrected: m, v moments; bc1 = 1 - b1^t, bc2 = 1 - b2^t
-__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
-                       float bc1, float bc2) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
+  g[i] = gi;
+  if (!(lr > 0.f)) return;
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  const float pi = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  p[i] = pi;
+  const uint16_t h = f2bf(pi);
+  if (l == 1) {
+    wb1[(size_t)o * kIn + k] = h;
+  } else if (l) {
+    uint16_t *wb = l == 2 ? wb2 : l == 3 ? wb3 : wb4;
+    uint16_t *wt = l == 2 ? wt2 : l == 3 ? wt3 : wt4;
+    wb[(size_t)o * kW + k] = h;
+    wt[(size_t)k * kW + o] = h;
+  } else if (i >= (int64_t)MlpLayout::kW5 && i < (int64_t)MlpLayout::kB5) {
+    w5b[k] = h;
+  }
 }
 
 // bf16 copies of the weights: W_l [256][K_l], transposes for l >= 2, w5
@@ -551,124 +796,129 @@ int refresh(csd_mlp *m, hipStream_t st) {
   return CSD_OK;
 }
 
-// training workspace for batch B (multiple of 128)
-struct TrainWs {
-  uint16_t *x0, *x0t;                 // [B][32], [32][B]
-  float *z[kHid + 1];                 // [B][256] fp32 (pre-activation, + bias)
-  uint16_t *a[kHid + 1], *at[kHid + 1];  // bf16 activations and transposes
-  float *y, *dy, *d;                  // [B], [B], [B][256] fp32 delta
-  uint16_t *db, *dbt;                 // bf16 delta and transpose
-  float *parts;                       // split-K / column partial sums
-  float *loss;
-};
-constexpr int kSplit = 32;        // split-K for the weight gradients
-constexpr int kColBlocks = 64;    // row blocks of the column sums
+// training workspace for batch B (multiple of 128), carved from m->ws
+int train_split(int l, int B) { return std::min(l == 1 ? 128 : 16, B / kKC); }
 
-size_t train_ws_bytes(int B) {
-  size_t s = 0;
-  s += 2 * align256((size_t)B * kIn * 2);
-  s += kHid * align256((size_t)B * kW * 4);
-  s += 2 * kHid * align256((size_t)B * kW * 2);
-  s += 2 * align256((size_t)B * 4) + align256((size_t)B * kW * 4);
-  s += 2 * align256((size_t)B * kW * 2);
-  s += align256((size_t)kSplit * kW * kW * 4 + (size_t)kColBlocks * (kW + 1) * 4);
-  s += align256(4);
-  return s;
-}
-
-TrainWs carve(void *base, int B) {
+size_t carve(void *base, int B, TrainDev *T, float **loss) {
   char *p = (char *)base;
   auto take = [&](size_t bytes) {
     char *r = p;
     p += align256(bytes);
     return (void *)r;
   };
-  TrainWs w;
-  w.x0 = (uint16_t *)take((size_t)B * kIn * 2);
-  w.x0t = (uint16_t *)take((size_t)B * kIn * 2);
-  w.z[0] = nullptr;
-  w.a[0] = w.x0;
-  w.at[0] = w.x0t;
-  for (int l = 1; l <= kHid; ++l) w.z[l] = (float *)take((size_t)B * kW * 4);
-  for (int l = 1; l <= kHid; ++l) {
-    w.a[l] = (uint16_t *)take((size_t)B * kW * 2);
-    w.at[l] = (uint16_t *)take((size_t)B * kW * 2);
+  const size_t act = (size_t)B * kW;
+  TrainDev t;
+  memset(&t, 0, sizeof(t));
+  t.B = B;
+  t.aw[0] = (uint16_t *)take((size_t)B * kIn * 2);
+  t.atw[0] = (uint16_t *)take((size_t)B * kIn * 2);
+  for (int l = 1; l < kHid; ++l) {
+    t.aw[l] = (uint16_t *)take(act * 2);
+    t.atw[l] = (uint16_t *)take(act * 2);
+    t.s[l] = (float *)take(act * 4);
   }
-  w.y = (float *)take((size_t)B * 4);
-  w.dy = (float *)take((size_t)B * 4);
-  w.d = (float *)take((size_t)B * kW * 4);
-  w.db = (uint16_t *)take((size_t)B * kW * 2);
-  w.dbt = (uint16_t *)take((size_t)B * kW * 2);
-  w.parts = (float *)take((size_t)kSplit * kW * kW * 4 + (size_t)kColBlocks * (kW + 1) * 4);
-  w.loss = (float *)take(4);
-  return w;
+  for (int k = 0; k < 2; ++k) {
+    t.d[k] = (uint16_t *)take(act * 2);
+    t.dt[k] = (uint16_t *)take(act * 2);
+  }
+  for (int l = 1; l <= kHid; ++l) {
+    t.split[l] = train_split(l, B);
+    t.wparts[l] = (float *)take((size_t)t.split[l] * kW * MlpLayout::K(l) * 4);
+  }
+  t.acc = (float *)take(kAccLen * 4);
+  float *lp = (float *)take(4);
+  for (int l = 0; l <= kHid; ++l) {
+    t.a[l] = t.aw[l];
+    t.at[l] = t.atw[l];
+  }
+  if (T) *T = t;
+  if (loss) *loss = lp;
+  return (size_t)(p - (char *)base);
 }
 
-// launches the NT GEMM with K split `split` ways; *parts = partial sums written
-int gemm_nt(const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C, int ldc, int M,
-            int N, int K, int split, hipStream_t st, int *parts = nullptr) {
-  const int kper = ((K / split + kGK - 1) / kGK) * kGK;
-  dim3 g((N + kGT - 1) / kGT, (M + kGT - 1) / kGT, (K + kper - 1) / kper);
-  hipLaunchKernelGGL(k_gemm_nt, g, dim3(kBlock), 0, st, A, lda, B, ldb, C, ldc, M, N, K, kper);
-  CSD_HIP(hipGetLastError());
-  if (parts) *parts = (int)g.z;
-  return CSD_OK;
-}
-
-// forward + backward for one batch; gradients into m->g, loss into w.loss
-int forward_backward(csd_mlp *m, const float *q, const float *target, int B, hipStream_t st) {
+// forward + backward for one batch, then the update: the gradient into m->g
+// and, with lr > 0, an Adam step; the loss into *loss_dev (workspace)
+int train_launches(csd_mlp *m, const float *q, const float *target, int B, float lr,
+                   hipStream_t st, float **loss_dev) {
   if (B > m->cap) {
     if (m->ws) hipFree(m->ws);
     m->ws = nullptr;
     m->cap = 0;
-    const size_t bytes = train_ws_bytes(B);
+    const size_t bytes = carve(nullptr, B, nullptr, nullptr);
     hipError_t e = hipMalloc(&m->ws, bytes);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc (MLP training workspace)");
     m->cap = B;
     m->ws_bytes = bytes;
   }
-  TrainWs w = carve(m->ws, B);
-  const int T = 256;
-  hipLaunchKernelGGL(k_prep_input, dim3((B * kIn + T - 1) / T), dim3(T), 0, st, q, B, m->in_dim,
-                     m->lo, m->scale, w.x0, w.x0t);
+  TrainDev T;
+  float *loss = nullptr;
+  carve(m->ws, B, &T, &loss);
+  T.target = target;
+  *loss_dev = loss;
+  const MlpDev M = m->dev();
+  const int nrow = B / kRB;
+  hipLaunchKernelGGL(k_train_prep, dim3((std::max(B * kIn, kAccLen) + 255) / 256), dim3(256), 0,
+                     st, q, B, m->in_dim, m->lo, m->scale, T.aw[0], T.atw[0], T.acc);
   for (int l = 1; l <= kHid; ++l) {
-    const int K = MlpLayout::K(l);
-    int rc = gemm_nt(w.a[l - 1], K, m->wb[l], K, w.z[l], kW, B, kW, K, 1, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_bias_silu, dim3(B * kW / T), dim3(T), 0, st, w.z[l],
-                       m->p + MlpLayout::b(l), B, w.a[l], w.at[l]);
-  }
-  hipLaunchKernelGGL(k_head, dim3((B + T - 1) / T), dim3(T), 0, st, w.a[kHid], w.z[kHid],
-                     m->w5b, m->p, target, B, w.y, w.dy, w.d);
-  hipLaunchKernelGGL(k_loss, dim3(1), dim3(kBlock), 0, st, w.y, target, B, w.loss);
-  // head gradients
-  const int rpb = (B + kColBlocks - 1) / kColBlocks;
-  float *colp = w.parts + (size_t)kSplit * kW * kW;
-  hipLaunchKernelGGL(k_head_grads, dim3(kColBlocks), dim3(kW + 1), 0, st, w.a[kHid], w.dy, B,
-                     rpb, colp);
-  hipLaunchKernelGGL(k_sum_parts, dim3(2), dim3(T), 0, st, colp, kColBlocks, (int64_t)(kW + 1),
-                     m->g + MlpLayout::kW5);
-  for (int l = kHid; l >= 1; --l) {
-    // delta_l: fp32 in w.d (already multiplied by SiLU' for l = 4)
-    hipLaunchKernelGGL(k_delta, dim3(B * kW / T), dim3(T), 0, st, w.d,
-                       l == kHid ? (const float *)nullptr : w.z[l], B, w.db, w.dbt);
-    // db_l
-    hipLaunchKernelGGL(k_colsum, dim3(kColBlocks), dim3(kW), 0, st, w.d, B, kW, rpb, colp);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kW), 0, st, colp, kColBlocks, (int64_t)kW,
-                       m->g + MlpLayout::b(l));
-    // dW_l [256][K_l] = delta_l^T a_{l-1}, split over the batch
-    const int K = MlpLayout::K(l);
-    int S = 0;
-    int rc = gemm_nt(w.dbt, B, w.at[l - 1], B, w.parts, K, kW, K, B, kSplit, st, &S);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_sum_parts, dim3((kW * K + T - 1) / T), dim3(T), 0, st, w.parts, S,
-                       (int64_t)kW * K, m->g + MlpLayout::w(l));
-    if (l > 1) {
-      // dA_{l-1} [B][256] = delta_l W_l  (B operand: W_l^T rows)
-      rc = gemm_nt(w.db, kW, m->wt[l], kW, w.d, kW, B, kW, kW, 1, st);
-      if (rc) return rc;
+    FwdArgs F;
+    memset(&F, 0, sizeof(F));
+    F.B = B;
+    F.K = MlpLayout::K(l);
+    F.A = T.a[l - 1];
+    F.W = m->wb[l];
+    F.bias = m->p + MlpLayout::b(l);
+    F.acc = T.acc;
+    F.target = target;
+    if (l < kHid) {
+      F.a = T.aw[l];
+      F.at = T.atw[l];
+      F.s = T.s[l];
+      hipLaunchKernelGGL(k_train_fwd<kFwd>, dim3(nrow), dim3(256), 0, st, F, M);
+    } else {
+      F.d = T.d[kHid & 1];
+      F.dt = T.dt[kHid & 1];
+      hipLaunchKernelGGL(k_train_fwd<kHead>, dim3(nrow), dim3(256), 0, st, F, M);
     }
   }
+  for (int l = kHid; l >= 1; --l) {
+    BackArgs A;
+    memset(&A, 0, sizeof(A));
+    A.B = B;
+    A.K = MlpLayout::K(l);
+    A.split = T.split[l];
+    A.nrow = l > 1 ? nrow : 0;
+    A.dt = T.dt[l & 1];
+    A.at = T.at[l - 1];
+    A.wparts = T.wparts[l];
+    if (l > 1) {
+      A.d = T.d[l & 1];
+      A.wt = m->wt[l];
+      A.sprev = T.s[l - 1];
+      A.dn = T.d[(l - 1) & 1];
+      A.dnt = T.dt[(l - 1) & 1];
+      A.accb = T.acc + kAccB + (l - 2) * kW;
+    }
+    const int tiles = l == 1 ? (kW / 128) * (kIn / 32) : (kW / 64) * (kW / 64);
+    hipLaunchKernelGGL(k_train_back, dim3(A.nrow + tiles * A.split), dim3(256), 0, st, A);
+  }
+  UpdateArgs U;
+  U.B = B;
+  for (int l = 1; l <= kHid; ++l) {
+    U.wparts[l - 1] = T.wparts[l];
+    U.split[l - 1] = T.split[l];
+  }
+  U.acc = T.acc;
+  float bc1 = 1.f, bc2 = 1.f;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  if (lr > 0.f) {
+    m->step += 1;
+    bc1 = (float)(1.0 - pow((double)b1, (double)m->step));
+    bc2 = (float)(1.0 - pow((double)b2, (double)m->step));
+  }
+  const int64_t P = MlpLayout::kTotal;
+  hipLaunchKernelGGL(k_train_update, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, U, m->p,
+                     m->g, m->m, m->v, lr, b1, b2, eps, bc1, bc2, m->wb[1], m->wb[2], m->wb[3],
+                     m->wb[4], m->wt[2], m->wt[3], m->wt[4], m->w5b, loss);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }
@@ -820,11 +1070,11 @@ int csd_mlp_gradients(csd_mlp *m, const float *q, const float *target, int32_t b
   if (rc) return rc;
   if (!q || !target) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
   hipStream_t st = (hipStream_t)stream;
-  rc = forward_backward(m, q, target, batch, st);
+  float *ld = nullptr;
+  rc = train_launches(m, q, target, batch, 0.f, st, &ld);
   if (rc) return rc;
-  TrainWs w = carve(m->ws, batch);
   if (grads) CSD_HIP(hipMemcpyAsync(grads, m->g, MlpLayout::kTotal * 4, hipMemcpyDeviceToDevice, st));
-  if (loss) CSD_HIP(hipMemcpyAsync(loss, w.loss, 4, hipMemcpyDeviceToDevice, st));
+  if (loss) CSD_HIP(hipMemcpyAsync(loss, ld, 4, hipMemcpyDeviceToDevice, st));
   return CSD_OK;
 }
 
@@ -835,21 +1085,10 @@ int csd_mlp_train_step(csd_mlp *m, const float *q, const float *target, int32_t 
   if (!q || !target) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
   if (!(lr > 0.f)) return fail(CSD_ERR_INVALID_ARGUMENT, "learning rate must be positive");
   hipStream_t st = (hipStream_t)stream;
-  rc = forward_backward(m, q, target, batch, st);
+  float *ld = nullptr;
+  rc = train_launches(m, q, target, batch, lr, st, &ld);
   if (rc) return rc;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  m->step += 1;
-  const float bc1 = (float)(1.0 - pow((double)b1, (double)m->step));
-  const float bc2 = (float)(1.0 - pow((double)b2, (double)m->step));
-  const int64_t P = MlpLayout::kTotal;
-  hipLaunchKernelGGL(k_adam, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, m->p, m->g, m->m,
-                     m->v, P, lr, b1, b2, eps, bc1, bc2);
-  rc = refresh(m, st);
-  if (rc) return rc;
-  if (loss) {
-    TrainWs w = carve(m->ws, batch);
-    CSD_HIP(hipMemcpyAsync(loss, w.loss, 4, hipMemcpyDeviceToDevice, st));
-  }
+  if (loss) CSD_HIP(hipMemcpyAsync(loss, ld, 4, hipMemcpyDeviceToDevice, st));
   return CSD_OK;
 }
 
