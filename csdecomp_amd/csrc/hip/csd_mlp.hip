@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_mlp_infer(MlpDev M, In in, int64_
 //                           clears the small gradient accumulators
 //   k_train_fwd<kFwd> x3    layer l = 1..3: z = a_{l-1} W_l^T + b_l,
 //                           a_l = bf16(SiLU(z)) as [B][256] and [256][B],
-//                           s_l = SiLU'(z) fp32 [B][256]
+//                           s_l = bf16(SiLU'(z)) [B][256]
 //   k_train_fwd<kHead>      layer 4, then per row y = w5 . a_4 + b5, the
 //                           loss, dy = 2 (y - t) / B and delta_4 =
 //                           dy w5 SiLU'(z_4) (bf16, both layouts), with the
@@ -296,7 +296,7 @@ struct TrainDev {
   const uint16_t *a[kHid + 1];   // a_0 = x0 [B][32], a_l [B][256] (l <= 3)
   const uint16_t *at[kHid + 1];  // transposes [K][B] (l <= 3)
   uint16_t *aw[kHid + 1], *atw[kHid + 1];
-  float *s[kHid + 1];            // SiLU'(z_l) [B][256], l <= 3
+  uint16_t *s[kHid + 1];         // bf16(SiLU'(z_l)) [B][256], l <= 3
   uint16_t *d[2], *dt[2];        // delta_l (ping-pong), bf16 [B][256] and [256][B]
   float *wparts[kHid + 1];       // dW_l partial sums [S_l][256][K_l]
   int split[kHid + 1];           // S_l
@@ -311,7 +311,7 @@ struct FwdArgs {
   const uint16_t *A, *W;  // a_{l-1} [B][K], W_l [256][K]
   const float *bias;      // b_l
   uint16_t *a, *at;       // a_l [B][256], [256][B]        (kFwd)
-  float *s;               // SiLU'(z_l) [B][256]           (kFwd)
+  uint16_t *s;            // bf16(SiLU'(z_l)) [B][256]      (kFwd)
   uint16_t *d, *dt;       // delta_4 [B][256], [256][B]   (kHead)
   float *acc;
   const float *target;
@@ -321,7 +321,7 @@ struct BackArgs {
   const uint16_t *dt, *at;    // delta_l^T [256][B], a_{l-1}^T [K][B]
   float *wparts;              // [split][256][K]
   const uint16_t *d, *wt;     // delta_l [B][256], W_l^T [256][256]
-  const float *sprev;         // SiLU'(z_{l-1}) [B][256]
+  const uint16_t *sprev;      // bf16(SiLU'(z_{l-1})) [B][256]
   uint16_t *dn, *dnt;         // delta_{l-1}, both layouts
   float *accb;                // db_{l-1} accumulators [256]
 };
@@ -446,10 +446,12 @@ __global__ __launch_bounds__(256) void k_train_fwd(FwdArgs T, MlpDev M) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int i = 4 * q + e;
+        // sigmoid from the hardware exp2 and reciprocal (~1 ulp): the
+        // activation is rounded to bf16 at once and SiLU' only scales the
+        // deltas (for z -> -inf, exp2 -> inf and sigmoid -> 0)
         const float z = acc[j][i] + bc;
-        const float ex = expf(-z);
-        const float sg = 1.f / (1.f + ex);
-        h[e] = f2bf(z / (1.f + ex));
+        const float sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+        h[e] = f2bf(z * sg);
         av[j][i] = bf2f(h[e]);
         sv[j][i] = sg * (1.f + z * (1.f - sg));
       }
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(256) void k_train_fwd(FwdArgs T, MlpDev M) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           T.a[(size_t)(row + e) * kW + col] = h[e];
-          T.s[(size_t)(row + e) * kW + col] = sv[j][4 * q + e];
+          T.s[(size_t)(row + e) * kW + col] = f2bf(sv[j][4 * q + e]);
         }
         store_t4(T.at, B, col, row, h);
       }
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(256) void k_train_back(BackArgs T) {
   const int B = T.B, r0 = blockIdx.x * kRB;
   f32x16 acc[2];
   rows_gemm(T.d, T.wt, kW, r0, lds, acc);
-  const float *sprev = T.sprev;
+  const uint16_t *sprev = T.sprev;
   uint16_t *dn = T.dn, *dnt = T.dnt;
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -602,7 +604,7 @@ __global__ __launch_bounds__(256) void k_train_back(BackArgs T) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = r0 + 8 * q + 4 * hr + e;
-        const float dv = acc[j][4 * q + e] * sprev[(size_t)row * kW + col];
+        const float dv = acc[j][4 * q + e] * bf2f(sprev[(size_t)row * kW + col]);
         sd += dv;
         h[e] = f2bf(dv);
         dn[(size_t)row * kW + col] = h[e];
@@ -668,9 +670,17 @@ __global__ void k_train_update(UpdateArgs T, float *__restrict__ p, float *__res
                                                                            : T.wparts[3]) + e;
     const int S = l == 1 ? T.split[0] : l == 2 ? T.split[1] : l == 3 ? T.split[2] : T.split[3];
     const size_t stride = (size_t)kW * K;
-    float s = 0.f;
-    for (int sp = 0; sp < S; ++sp) s += wp[sp * stride];
-    gi = s;
+    // four independent sums keep four loads in flight; fixed order
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int sp = 0;
+    for (; sp + 4 <= S; sp += 4) {
+      s0 += wp[(size_t)sp * stride];
+      s1 += wp[(size_t)(sp + 1) * stride];
+      s2 += wp[(size_t)(sp + 2) * stride];
+      s3 += wp[(size_t)(sp + 3) * stride];
+    }
+    for (; sp < S; ++sp) s0 += wp[(size_t)sp * stride];
+    gi = (s0 + s1) + (s2 + s3);
   } else if (i == (int64_t)MlpLayout::kB5) {
     gi = T.acc[kAccB5];
   } else if (i >= (int64_t)MlpLayout::kW5) {
@@ -797,7 +807,7 @@ int refresh(csd_mlp *m, hipStream_t st) {
 }
 
 // training workspace for batch B (multiple of 128), carved from m->ws
-int train_split(int l, int B) { return std::min(l == 1 ? 128 : 16, B / kKC); }
+int train_split(int l, int B) { return std::min(l == 1 ? 32 : 16, B / kKC); }
 
 size_t carve(void *base, int B, TrainDev *T, float **loss) {
   char *p = (char *)base;
@@ -815,7 +825,7 @@ size_t carve(void *base, int B, TrainDev *T, float **loss) {
   for (int l = 1; l < kHid; ++l) {
     t.aw[l] = (uint16_t *)take(act * 2);
     t.atw[l] = (uint16_t *)take(act * 2);
-    t.s[l] = (float *)take(act * 4);
+    t.s[l] = (uint16_t *)take(act * 2);
   }
   for (int k = 0; k < 2; ++k) {
     t.d[k] = (uint16_t *)take(act * 2);
