@@ -109,35 +109,6 @@ def eizo_cpu_baseline(plant, starts, ends, seconds=12.0, face_budget=100):
                       f"checks, sampling and bisection on one thread"}
 
 
-def train_surrogate(plant, n=2_000_000, batch=8192, epochs=3, seed=7):
-    """Surrogate for the plant: n exact FP32 signed-distance labels (clipped to
-    +-0.2 m, scaled to +-1) at uniform configurations, Adam lr 1e-3."""
-    import torch
-    from csdecomp_amd import capi
-    mp = plant.getMinimalPlant()
-    lo = np.asarray(plant.getPositionLowerLimits(), np.float32)
-    hi = np.asarray(plant.getPositionUpperLimits(), np.float32)
-    dof = len(lo)
-    t0 = time.perf_counter()
-    q = torch.from_numpy(np.random.default_rng(seed).uniform(lo, hi, (n, dof)).astype(np.float32)).cuda()
-    sd = torch.empty(n, device="cuda")
-    dplant = capi.DevicePlant(mp)  # alive until the labels are computed
-    capi.signed_distance(dplant, q, n, dof, 1, sd)
-    target = (sd.clamp(-0.2, 0.2) / 0.2).contiguous()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    model = capi.Mlp(dof, lo, hi, 1)
-    loss = torch.empty(1, device="cuda")
-    steps = epochs * n // batch
-    for s in range(steps):
-        k = (s * batch) % (n - batch)
-        model.train_step(q[k:k + batch], target[k:k + batch], batch, 1e-3, loss)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    return model, {"labels": n, "label_s": round(t1 - t0, 3), "train_steps": steps,
-                   "train_s": round(t2 - t1, 3), "final_mse": round(loss.item(), 5)}
-
-
 def eizo_region_flops(info, dim, mixing, steps, f_cfg):
     """Algorithmic FLOPs of one region, summed over its rounds from the
     kernels' own counts (EizoInfo.iteration_info):
@@ -157,10 +128,15 @@ def eizo_region_flops(info, dim, mixing, steps, f_cfg):
     return tot
 
 
-def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_budget=100):
+def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_budget=100,
+             train_s=0.0, pilot=8192, keep=256):
     """Regions/s of batched EI-ZO (epsilon 1e-3, delta 0.05, tau 0.5, 10
     bisection steps, 40 mixing steps, <= 10 faces per round, <= face_budget
-    faces beyond the domain) in each mode."""
+    faces beyond the domain) in each mode. Guided modes count the
+    surrogate's labelling and training time (train_s) in their regions/s and
+    also report the inflation alone. "guided_untrained" runs the same guided
+    rounds with a freshly initialised network (no training time): the
+    difference to "guided" is what the training itself gains."""
     import torch
     import csdecomp_amd as csd
     mp = plant.getMinimalPlant()
@@ -181,27 +157,37 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         if mode == "plain_graph":
             inf.set_graph(1)  # one graph-captured round per launch, status read each round
         if mode in ("guided", "guided_fallback"):
-            inf.set_guidance(pilot_samples=4096, pilot_factor=10.0, surrogate=model,
-                             keep_particles=256, line_points=4,
+            inf.set_guidance(pilot_samples=pilot, pilot_factor=10.0, surrogate=model,
+                             keep_particles=keep, line_points=4,
                              plain_fallback=mode == "guided_fallback")
+        if mode == "guided_untrained":
+            from csdecomp_amd import capi
+            lo = np.asarray(plant.getPositionLowerLimits(), np.float32)
+            hi = np.asarray(plant.getPositionUpperLimits(), np.float32)
+            inf.set_guidance(pilot_samples=pilot, pilot_factor=10.0,
+                             surrogate=capi.Mlp(len(lo), lo, hi, 7), keep_particles=keep,
+                             line_points=4)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         regions, infos = inf.inflateEdges(starts, ends, None, 0.0, budgets)
         torch.cuda.synchronize()
-        el = time.perf_counter() - t0
+        infl = time.perf_counter() - t0
+        el = infl + (train_s if mode in ("guided", "guided_fallback") else 0.0)
         conv = np.array([i.converged for i in infos])
         faces[mode] = [(r.A(), r.b()) for r in regions]
         flops = sum(eizo_region_flops(i, starts.shape[0], opt.mixing_steps, opt.bisection_steps,
                                       f_cfg) for i in infos)
         out[mode] = {"seconds": round(el, 3), "regions_per_s": round(R / el, 2),
+                     "inflation_seconds": round(infl, 3),
+                     "inflation_regions_per_s": round(R / infl, 2),
                      "certified_per_s": round(float(conv.sum()) / el, 2),
                      "certified_fraction": round(float(conv.mean()), 4),
                      "rounds_mean": round(float(np.mean([i.iterations for i in infos])), 2),
                      "faces_mean": round(float(np.mean([r.A().shape[0] - nd for r in regions])), 1),
                      "samples": int(sum(i.samples_drawn for i in infos)),
-                     "roofline": {"bound": "valu", "achieved": flops / el / 1e12,
+                     "roofline": {"bound": "valu", "achieved": flops / infl / 1e12,
                                   "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                  "frac": flops / el / 1e12 / FP32_VECTOR_PEAK_TFLOPS,
+                                  "frac": flops / infl / 1e12 / FP32_VECTOR_PEAK_TFLOPS,
                                   "flops": flops, "flops_per_region": flops / R,
                                   "f_cfg": f_cfg}}
         if mode == "guided_fallback":
@@ -212,8 +198,14 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
             for a, g in zip(faces["plain"], faces["plain_graph"]))
     if "plain" in out and "guided" in out:
         out["guided_vs_plain_regions_per_s"] = round(out["guided"]["regions_per_s"] / out["plain"]["regions_per_s"], 3)
+        out["guided_vs_plain_samples"] = round(out["guided"]["samples"] / out["plain"]["samples"], 3)
         out["guided_vs_plain_certified_per_s"] = round(
             out["guided"]["certified_per_s"] / max(out["plain"]["certified_per_s"], 1e-9), 3)
+    if "guided" in out and "guided_untrained" in out:
+        g, u = out["guided"], out["guided_untrained"]
+        out["trained_vs_untrained_inflation_regions_per_s"] = round(
+            g["inflation_regions_per_s"] / u["inflation_regions_per_s"], 3)
+        out["trained_vs_untrained_samples"] = round(g["samples"] / u["samples"], 3)
     return out
 
 
@@ -455,22 +447,38 @@ def eizo_benchmarks(segments=2048, cpu=True):
     """EI-ZO legs: single-edge inflation (7-DOF arm, 48 boxes + 16 spheres, 256
     segments, 4096 particles) and batched inflation of uneven sizes (14-DOF
     dual arm, 96 obstacles, 2048 segments, lengths log-U[0.05, 2] rad, budgets
-    log-U[512, 16384]); plain rounds vs surrogate-guided rounds."""
-    from csdecomp_amd import scenes
+    log-U[512, 16384]); plain rounds vs surrogate-guided rounds. Each
+    surrogate is trained for its workload (2M labels, half around the
+    segments, 6,000 Adam steps of 8,192) and its labelling and training
+    time is counted in the guided regions/s."""
+    from csdecomp_amd import scenes, surrogate
     res = {}
     arm = scenes.arm_scene(1001, 48, 16)
     s1, e1 = scenes.free_segments(arm, 256, 3001, (0.2, 1.5))
-    m7, tr7 = train_surrogate(arm)
-    res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, ("plain", "plain_graph", "guided", "guided_fallback"), m7)
+    m7, tr7 = surrogate.train_surrogate(arm, s1, e1)
+    modes = ("plain", "plain_graph", "guided", "guided_untrained", "guided_fallback")
+    res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, modes, m7,
+                                       train_s=tr7["label_s"] + tr7["train_s"])
     res["single_edge_arm7"]["surrogate"] = tr7
     if cpu:
         res["single_edge_arm7"]["cpu_baseline"] = eizo_cpu_baseline(arm, s1, e1)
     dual = scenes.dual_arm_scene(1002, 72, 24)
     s2, e2 = scenes.free_segments(dual, segments, 3002, (0.05, 2.0), log_uniform=True)
     b2 = scenes.log_uniform_budgets(3003, segments)
-    m14, tr14 = train_surrogate(dual)
-    res["batched_dualarm14"] = eizo_leg("batched", dual, s2, e2, b2, 16384, ("plain", "plain_graph", "guided", "guided_fallback"), m14)
+    m14, tr14 = surrogate.train_surrogate(dual, s2, e2)
+    t14 = tr14["label_s"] + tr14["train_s"]
+    res["batched_dualarm14"] = eizo_leg("batched", dual, s2, e2, b2, 16384, modes, m14,
+                                        train_s=t14)
     res["batched_dualarm14"]["surrogate"] = tr14
+    # room for 1,000 faces, where plain certifies the majority of the
+    # regions: the first 128 segments (a surrogate trained on them)
+    k = min(128, segments)
+    sk, ek = np.asfortranarray(s2[:, :k]), np.asfortranarray(e2[:, :k])
+    mk, trk = surrogate.train_surrogate(dual, sk, ek)
+    res["batched_dualarm14_1000faces"] = eizo_leg(
+        "batched", dual, sk, ek, b2[:k], 16384, ("plain", "guided", "guided_untrained"), mk,
+        face_budget=1000, train_s=trk["label_s"] + trk["train_s"])
+    res["batched_dualarm14_1000faces"]["surrogate"] = trk
     return res
 
 

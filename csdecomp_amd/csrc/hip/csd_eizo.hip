@@ -21,6 +21,7 @@
 //    in FP32 so the segment stays inside the stored face.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <vector>
 
 #include "csd_eizo_device.h"
@@ -79,11 +80,16 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
 // Round stage 1b: collision check of the block's samples and compaction of
 // the colliding ones, in sample order, to the front of the same segment as
 // [count][dim] (block ballot + prefix sums; all samples are read before the
-// first write).
+// first write). With `pred` (a pilot pruned by the surrogate): only samples
+// whose prediction is below `pred_below` get the exact check, the others
+// count as free, and blk_first counts the samples checked instead of the
+// collisions among the first M.
 template <int VAR, int DP>
 __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, VoxelArgs V,
                                                              RobotGeomList R, EizoDev D, int dim,
-                                                             int nblk, uint32_t M) {
+                                                             int nblk, uint32_t M,
+                                                             const float *__restrict__ pred,
+                                                             float pred_below) {
   const int slot = blockIdx.y;
   if (D.ctl) {
     if (slot >= (int)D.ctl[1]) return;
@@ -106,12 +112,16 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
   float x[DP];
 #pragma unroll
   for (int d = 0; d < DP; ++d) x[d] = d < dim ? seg[(size_t)d * kSampleBlock + threadIdx.x] : 0.f;
-  const bool col = collides_variant<VAR>(P, QReg<DP>{x}, V, R, !valid, tab) && valid;
+  const bool check =
+      valid && (!pred || pred[((size_t)slot * nblk + blockIdx.x) * kSampleBlock + threadIdx.x] <
+                             pred_below);
+  // a wave with nothing to check skips the check (and its forward kinematics)
+  const bool col = __any(check) && collides_variant<VAR>(P, QReg<DP>{x}, V, R, !check, tab) && check;
 
   __shared__ uint32_t wcnt[kSampleBlock / 64], wfirst[kSampleBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t m = __ballot(col);
-  const uint64_t mf = __ballot(col && i < M);
+  const uint64_t mf = pred ? __ballot(check) : __ballot(col && i < M);
   if (lane == 0) {
     wcnt[wave] = (uint32_t)__popcll(m);
     wfirst[wave] = (uint32_t)__popcll(mf);
@@ -166,19 +176,20 @@ __global__ __launch_bounds__(256) void k_eizo_compact(EizoDev D, int dim, int nb
     s_first = first;
     D.samples[r] += N;
     D.collisions[r] += acc;
-    const bool certified = test && (float)first <= thresh;
+    const bool certified = test == 1 && (float)first <= thresh;
     if (certified) D.status[r] = 1;
-    if (!test) D.pilot_col[r] = acc;
+    if (test != 1) D.pilot_col[r] = acc;
     s_u = certified ? 0 : (int)min(acc, min(D.budget[r], (uint32_t)U));
     D.u[r] = s_u;
     if (int32_t *st = round_stats(D, r, round)) {
-      if (test) {
+      if (test == 1) {
         st[kStatSamples] = (int32_t)N;
         st[kStatFirstCol] = (int32_t)first;
         st[kStatCol] = (int32_t)acc;
       } else {  // pilot; a later test phase of the round overwrites these
         st[kStatPilot] = (int32_t)N;
         st[kStatPilotCol] = (int32_t)acc;
+        st[kStatPilotChecked] = test == 2 ? (int32_t)first : (int32_t)N;
         st[kStatSamples] = 0;
         st[kStatFirstCol] = -1;
         st[kStatCol] = 0;
@@ -401,6 +412,12 @@ struct csd_eizo {
   int keep = 64, line_points = 4;
   float *pred = nullptr;  // [slots][cand][L] surrogate outputs
   size_t pred_bytes = 0;
+  // pilot pruned by the surrogate (csd_eizo_set_pruning): only samples
+  // predicted below prune_below are checked exactly
+  bool prune = false;
+  float prune_below = 0.f;
+  float *ppred = nullptr;  // [slots][nblk][256] predictions at the pilot samples
+  size_t ppred_bytes = 0;
   // device-resident rounds (csd_eizo_set_graph): host status read every
   // graph_sync rounds, 0 for host-driven rounds
   int graph_sync = 0;
@@ -531,6 +548,7 @@ int csd_eizo_destroy(csd_eizo *w) {
   if (w->D.blk_cnt) hipFree(w->D.blk_cnt);
   if (w->D.blk_first) hipFree(w->D.blk_first);
   if (w->pred) hipFree(w->pred);
+  if (w->ppred) hipFree(w->ppred);
   if (w->gs) hipStreamDestroy(w->gs);
   mlp_attach(w->surrogate, -1);
   delete w;
@@ -557,6 +575,15 @@ int csd_eizo_set_guidance(csd_eizo *w, uint32_t pilot_samples, float pilot_facto
     w->keep = keep_particles;
     w->line_points = line_points;
   }
+  return CSD_OK;
+}
+
+int csd_eizo_set_pruning(csd_eizo *w, int32_t enable, float predicted_below) {
+  if (!w) return fail(CSD_ERR_INVALID_ARGUMENT, "null workspace");
+  if (enable && !std::isfinite(predicted_below))
+    return fail(CSD_ERR_INVALID_ARGUMENT, "the pruning threshold must be finite");
+  w->prune = enable != 0;
+  w->prune_below = predicted_below;
   return CSD_OK;
 }
 
@@ -676,8 +703,10 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
   // one sampling phase over the regions in `slots`: nsamp[r] samples each
   // (stream offset: independent random numbers for the pilot), collision
   // check, compaction (+ the unadaptive test when `test`)
+  // prune (pilot only): the surrogate predicts every sample and only those
+  // predicted below prune_below get the exact check
   auto phase = [&](const std::vector<int32_t> &slots, uint32_t it, uint32_t M, bool test,
-                   uint32_t stream_off) -> int {
+                   uint32_t stream_off, bool prune) -> int {
     const int na = (int)slots.size();
     uint32_t nmax = 0;
     for (int r : slots) nmax = std::max(nmax, nsamp[r]);
@@ -707,14 +736,28 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     else
       hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, 0, D, dim, cap, nblk,
                          (int)o.mixing_steps, k0, k1, it, stream_off);
+    const float *pr = nullptr;
+    if (prune) {
+      const size_t need = (size_t)na * nblk * kSampleBlock * sizeof(float);
+      if (need > w->ppred_bytes) {
+        if (w->ppred) hipFree(w->ppred);
+        w->ppred = nullptr;
+        w->ppred_bytes = 0;
+        CSD_HIP(hipMalloc(&w->ppred, need));
+        w->ppred_bytes = need;
+      }
+      const int prc = mlp_predict_staged(w->surrogate, D.stage, na * nblk, dim, w->ppred, 0);
+      if (prc) return prc;
+      pr = w->ppred;
+    }
     if (dp == 8)
       CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 8, sg, sb, tab_bytes, 0, P, V, RL, D, dim, nblk,
-                             M);
+                             M, pr, w->prune_below);
     else
       CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, 0, P, V, RL, D, dim,
-                             nblk, M);
+                             nblk, M, pr, w->prune_below);
     hipLaunchKernelGGL(k_eizo_compact, dim3(na), dim3(256), 0, 0, D, dim, nblk, M, thresh, U,
-                       test ? 1 : 0, it);
+                       test ? 1 : prune ? 2 : 0, it);
     CSD_HIP(hipGetLastError());
     return CSD_OK;
   };
@@ -782,10 +825,10 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
                            (int)o.mixing_steps, k0, k1, 0u, 0u);
       if (dp == 8)
         CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 8, sg, sb, tab_bytes, s, P, V, RL, G, dim, nblk,
-                               0u);
+                               0u, (const float *)nullptr, 0.f);
       else
         CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, s, P, V, RL, G, dim,
-                               nblk, 0u);
+                               nblk, 0u, (const float *)nullptr, 0.f);
       hipLaunchKernelGGL(k_eizo_compact, dim3(Rn), dim3(256), 0, s, G, dim, nblk, 0u, 0.f, U, 1,
                          0u);
       if (dp == 8)
@@ -829,7 +872,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     std::vector<int32_t> full = active;
     if (w->pilot) {
       for (int r : active) nsamp[r] = w->pilot;
-      rc = phase(active, it, M, false, 0x40000000u);
+      rc = phase(active, it, M, false, 0x40000000u, w->prune && w->surrogate);
       if (rc) return rc;
       CSD_HIP(hipMemcpy(pcol.data(), D.pilot_col, Rn * 4, hipMemcpyDeviceToHost));
       full.clear();
@@ -839,7 +882,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     }
     if (!full.empty()) {
       for (int r : full) nsamp[r] = std::max(M, budget[r]);
-      rc = phase(full, it, M, true, 0u);
+      rc = phase(full, it, M, true, 0u, false);
       if (rc) return rc;
     }
     if (w->pilot || full.size() != active.size())
@@ -938,8 +981,8 @@ int csd_eizo_get_rounds(const csd_eizo *w, int32_t region, csd_eizo_round_info *
     o.num_faces_at_start = e[kStatFacesStart];
     o.pilot_samples = e[kStatPilot];
     o.pilot_collisions = e[kStatPilotCol];
+    o.pilot_checked = e[kStatPilotChecked];
     o.iteration_time_ms = k < (int)w->round_ms.size() ? w->round_ms[k] : 0.f;
-    o.pad = 0;
   }
   return CSD_OK;
 }

@@ -202,7 +202,8 @@ enum {
   kStatFacesStart = 5,  // faces of the region when the round began
   kStatPilot = 6,       // pilot samples
   kStatPilotCol = 7,    // collisions among the pilot samples
-  kStats = 8
+  kStatPilotChecked = 8,  // pilot samples checked exactly (pruned pilots)
+  kStats = 9
 };
 
 __device__ __forceinline__ int32_t *round_stats(const EizoDev &D, int r, uint32_t round) {

@@ -162,6 +162,18 @@ struct SegmentInput {
   }
 };
 
+// raw inputs of row r: sample r % 256 of staging segment r / 256, stored as
+// [dim][256] (the EI-ZO sampler's layout)
+struct StagedInput {
+  const float *stage;
+  int dim;
+  __device__ __forceinline__ void load(int64_t r, float (&x)[kIn]) const {
+    const float *seg = stage + (size_t)(r >> 8) * 256 * dim + (r & 255);
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) x[k] = k < dim ? seg[(size_t)k * 256] : 0.f;
+  }
+};
+
 template <class In>
 __global__ __launch_bounds__(kBlock, 2) void k_mlp_infer(MlpDev M, In in, int64_t n,
                                                       float *__restrict__ out) {
@@ -1103,6 +1115,19 @@ int csd_mlp_train_step(csd_mlp *m, const float *q, const float *target, int32_t 
 }
 
 }  // extern "C"
+
+int csd::mlp_predict_staged(const csd_mlp *m, const float *stage, int num_blocks, int dim,
+                            float *out, hipStream_t st) {
+  if (!m) return fail(CSD_ERR_INVALID_ARGUMENT, "null surrogate");
+  if (dim != m->in_dim) return fail(CSD_ERR_INVALID_ARGUMENT, "surrogate input dimension differs from the plant's");
+  const int64_t n = (int64_t)num_blocks * 256;
+  if (n == 0) return CSD_OK;
+  const int64_t blocks = (n + 4 * kRowsWave - 1) / (4 * kRowsWave);
+  hipLaunchKernelGGL((k_mlp_infer<StagedInput>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
+                     m->dev(), StagedInput{stage, dim}, n, out);
+  CSD_HIP(hipGetLastError());
+  return CSD_OK;
+}
 
 void csd::mlp_attach(const csd_mlp *m, int delta) {
   if (m) m->attached.fetch_add(delta);

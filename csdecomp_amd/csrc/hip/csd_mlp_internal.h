@@ -21,6 +21,11 @@ struct SegmentPoints {
 int mlp_predict_segment_points(const csd_mlp *m, const SegmentPoints &s, int num_slots,
                                float *out, hipStream_t st);
 
+// out [num_blocks * 256]: the surrogate's output at the samples of an EI-ZO
+// staging buffer, num_blocks segments of 256 samples as [dim][256]
+int mlp_predict_staged(const csd_mlp *m, const float *stage, int num_blocks, int dim, float *out,
+                       hipStream_t st);
+
 // EI-ZO workspaces that hold the surrogate register here (delta +1 on
 // attach, -1 on detach); csd_mlp_destroy refuses while the count is positive,
 // so a workspace never keeps a freed model.

@@ -427,7 +427,8 @@ PYBIND11_MODULE(_pycsdecomp, m) {
       .def_readonly("num_collisions_found", &EizoIterationInfo::num_collisions_found)
       .def_readonly("num_faces_at_start", &EizoIterationInfo::num_faces_at_start)
       .def_readonly("num_pilot_samples", &EizoIterationInfo::num_pilot_samples)
-      .def_readonly("num_pilot_collisions", &EizoIterationInfo::num_pilot_collisions);
+      .def_readonly("num_pilot_collisions", &EizoIterationInfo::num_pilot_collisions)
+      .def_readonly("num_pilot_checked", &EizoIterationInfo::num_pilot_checked);
   m.def("printEizoSummary", &printEizoSummary, py::arg("info"));
 
   py::class_<EizoInfo>(m, "EizoInfo")
@@ -469,21 +470,24 @@ PYBIND11_MODULE(_pycsdecomp, m) {
           "set_guidance",
           [](CudaEdgeInflator &self, uint32_t pilot_samples, float pilot_factor,
              const py::object &surrogate, int keep_particles, int line_points,
-             bool plain_fallback) {
+             bool plain_fallback, const py::object &prune_below) {
             uintptr_t addr = 0;
             if (py::hasattr(surrogate, "address")) addr = surrogate.attr("address").cast<uintptr_t>();
             else if (!surrogate.is_none()) addr = surrogate.cast<uintptr_t>();
             self.setGuidance(pilot_samples, pilot_factor, reinterpret_cast<const void *>(addr),
-                             keep_particles, line_points, plain_fallback);
+                             keep_particles, line_points, plain_fallback,
+                             prune_below.is_none() ? NAN : prune_below.cast<float>());
           },
           py::arg("pilot_samples") = 0, py::arg("pilot_factor") = 10.0f,
           py::arg("surrogate") = py::none(), py::arg("keep_particles") = 256,
           py::arg("line_points") = 4, py::arg("plain_fallback") = false,
+          py::arg("prune_below") = py::none(),
           // the inflator keeps the surrogate object alive: it reads the model
           // in every guided round
           py::keep_alive<1, 4>(),
           "surrogate: a csdecomp_amd.capi.Mlp (kept alive by the inflator), the address of a "
-          "csd_mlp, or None")
+          "csd_mlp, or None; prune_below: with a pilot and a surrogate, only the pilot samples "
+          "the surrogate predicts below this value get the exact check (None: all)")
       .def(
           "set_metric",
           [](CudaEdgeInflator &self, const py::object &m) {

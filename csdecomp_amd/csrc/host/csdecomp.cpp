@@ -942,11 +942,14 @@ CudaEdgeInflator::CudaEdgeInflator(const MinimalPlant &plant,
 
 void CudaEdgeInflator::setGuidance(uint32_t pilot_samples, float pilot_factor,
                                    const void *surrogate, int keep_particles, int line_points,
-                                   bool plain_fallback) {
+                                   bool plain_fallback, float prune_below) {
   check_status(csd_eizo_set_guidance(ws_, pilot_samples, pilot_factor,
                                      static_cast<const csd_mlp *>(surrogate), keep_particles,
                                      line_points),
                "EI-ZO guidance");
+  const bool prune = std::isfinite(prune_below);
+  check_status(csd_eizo_set_pruning(ws_, prune ? 1 : 0, prune ? prune_below : 0.f),
+               "EI-ZO pruned pilot");
   pilot_samples_ = pilot_samples;
   pilot_factor_ = pilot_factor;
   surrogate_ = surrogate;
@@ -1025,6 +1028,7 @@ std::vector<HPolyhedron> CudaEdgeInflator::inflateEdges(const MatrixXf &ls, cons
       it.num_faces_at_start = e.num_faces_at_start;
       it.num_pilot_samples = e.pilot_samples;
       it.num_pilot_collisions = e.pilot_collisions;
+      it.num_pilot_checked = e.pilot_checked;
       rounds[r].push_back(it);
     }
   };

@@ -5,6 +5,7 @@
 // collision, a transform or a sample on the CPU.
 #pragma once
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -303,6 +304,7 @@ struct EizoIterationInfo {
   int num_faces_at_start = 0;
   int num_pilot_samples = 0;
   int num_pilot_collisions = 0;
+  int num_pilot_checked = 0;  // pilot samples checked exactly (fewer when pruned)
 };
 
 struct EizoInfo {
@@ -351,8 +353,12 @@ class CudaEdgeInflator {
   // particles to bisect. surrogate: a csd_mlp* (or null).
   // plain_fallback: regions that guided rounds leave uncertified are
   // inflated again by plain EI-ZO, whose result replaces the guided one.
+  // prune_below (finite): the pilot's samples are predicted by the surrogate
+  // and only those predicted below it are checked exactly
+  // (csd_eizo_set_pruning); NaN checks every pilot sample.
   void setGuidance(uint32_t pilot_samples, float pilot_factor, const void *surrogate,
-                   int keep_particles = 256, int line_points = 4, bool plain_fallback = false);
+                   int keep_particles = 256, int line_points = 4, bool plain_fallback = false,
+                   float prune_below = NAN);
   // Device-resident, graph-captured rounds (csd_eizo_set_graph): the host
   // reads the region status every rounds_per_sync rounds; 0 for host-driven
   // rounds. Same polytopes either way.

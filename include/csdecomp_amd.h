@@ -269,9 +269,9 @@ typedef struct csd_eizo_round_info {
   int32_t num_faces_placed;
   int32_t num_faces_at_start;        /* faces of the region when the round began */
   int32_t pilot_samples;             /* csd_eizo_set_guidance pilot */
-  int32_t pilot_collisions;
+  int32_t pilot_collisions;          /* found by the exact check */
   float iteration_time_ms;           /* wall time of the round, shared by the batch */
-  int32_t pad;
+  int32_t pilot_checked;             /* pilot samples checked exactly (all unless pruned) */
 } csd_eizo_round_info;
 
 /* Per-round record of `region` in the last csd_eizo_run (replaces the
@@ -310,6 +310,18 @@ struct csd_mlp;
 int csd_eizo_set_guidance(csd_eizo *ws, uint32_t pilot_samples, float pilot_factor,
                           const struct csd_mlp *surrogate, int32_t keep_particles,
                           int32_t line_points);
+
+/* Surrogate-pruned pilot for later csd_eizo_run calls (no counterpart in the
+ * reference). With enable != 0, and while guidance sets both a pilot and a
+ * surrogate, each round's pilot samples are first predicted by the
+ * surrogate and only those whose prediction is below predicted_below (in
+ * the surrogate's output units) get the exact collision check; the others
+ * count as free. The pilot then decides from the collisions the exact check
+ * confirmed, and only those become particles: a wrong prediction can make a
+ * round run its unadaptive test needlessly or draw fewer particles, never
+ * certify a region (the test always checks all its samples exactly).
+ * enable = 0 (the default) checks every pilot sample. */
+int csd_eizo_set_pruning(csd_eizo *ws, int32_t enable, float predicted_below);
 
 /* Ellipsoid metric of the face placement for later csd_eizo_run calls (no
  * counterpart in the reference, whose faces are placed in the Euclidean

@@ -336,25 +336,9 @@ def test_knn_vs_brute_force(gpu):
     assert 17 not in idx[5] and 5 not in idx[17]
 
 
-def _train_surrogate(gpu, pl, steps=300, n=262144, seed=5):
-    import torch
-    from csdecomp_amd import capi
-    mp = pl.getMinimalPlant()
-    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
-    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
-    dof = len(lo)
-    q = torch.from_numpy(np.random.default_rng(seed).uniform(lo, hi, (n, dof)).astype(np.float32)).to(gpu)
-    sd = torch.empty(n, device=gpu)
-    dplant = capi.DevicePlant(mp)
-    capi.signed_distance(dplant, q, n, dof, 1, sd)
-    t = (sd.clamp(-0.2, 0.2) / 0.2).contiguous()
-    m = capi.Mlp(dof, lo, hi, seed)
-    B = 8192
-    for s in range(steps):
-        k = (s * B) % n
-        m.train_step(q[k:k + B], t[k:k + B], B, 1e-3)
-    torch.cuda.synchronize()
-    return m
+def _train_surrogate(gpu, pl, ls=None, le=None, steps=300, n=262144, seed=5):
+    from csdecomp_amd import surrogate
+    return surrogate.train_surrogate(pl, ls, le, n=n, steps=steps, seed=seed, device=gpu)[0]
 
 
 def test_guided_eizo_certifies(gpu, csd, orc):
@@ -376,7 +360,7 @@ def test_guided_eizo_certifies(gpu, csd, orc):
     opt.bisection_steps = 9
     opt.max_iterations = 80
     rg = list(pl.getRobotGeometryIds())
-    model = _train_surrogate(gpu, pl)
+    model = _train_surrogate(gpu, pl, ls, le)
     results = {}
     for mode in ("plain", "pilot", "guided", "guided_wide"):
         inf = csd.CudaEdgeInflator(mp, rg, opt, dom, R, 10)

@@ -424,3 +424,57 @@ def test_ellipsoid_metric_face_placement(gpu, orc):
     regions2, infos2 = inf2.inflateEdges(ls, le, None, 0.0, [])
     fr = _region_properties(gpu, mp, regions2, infos2, ls, le, EPS, "metric")
     assert len(fr) >= 1
+
+
+def test_trained_surrogate_gain(gpu, orc):
+    """What the guidance and the trained network gain, on the single-edge
+    workload's options with room for 300 faces (every region certifies in
+    all modes); the exact checker judges every sample that counts.
+    - Guided rounds (an exact pilot of 8,192 samples, 256 surrogate-ranked
+      particles bisected) certify plain EI-ZO's fraction with fewer exact
+      collision checks of samples.
+    - The trained network's own gain is in the pruned pilot: the pilot
+      samples it predicts free (above 0.1, i.e. 2 cm) skip the exact check.
+      Trained, it prunes a large share of the pilot; a freshly initialised
+      network of the same shape prunes far less. Certification is unchanged:
+      the unadaptive test is never pruned."""
+    from csdecomp_amd import capi, scenes
+    import csdecomp_amd as csd
+    from test_gpu_eizo import _train_surrogate
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    dom, A0, _ = _domain(csd, pl)
+    R = 48
+    ls, le = scenes.free_segments(pl, R, 3001, (0.2, 1.5))
+    cap = A0.shape[0] + 300
+    rg = list(pl.getRobotGeometryIds())
+    trained = _train_surrogate(gpu, pl, ls, le, steps=6000, n=2_000_000, seed=7)
+    lo = np.asarray(pl.getPositionLowerLimits(), np.float32)
+    hi = np.asarray(pl.getPositionUpperLimits(), np.float32)
+    untrained = capi.Mlp(7, lo, hi, 7)
+    out = {}
+    runs = (("plain", None, None), ("guided", trained, None), ("pruned_trained", trained, 0.1),
+            ("pruned_untrained", untrained, 0.1))
+    for mode, model, prune in runs:
+        opt = _options(csd, 4096, 500)
+        opt.track_iteration_information = True
+        inf = csd.CudaEdgeInflator(mp, rg, opt, dom, R, 10, cap)
+        if model is not None:
+            inf.set_guidance(pilot_samples=8192, pilot_factor=10.0, surrogate=model,
+                             keep_particles=256, line_points=4, prune_below=prune)
+        regions, infos = inf.inflateEdges(ls, le, None, 0.0, [])
+        for r in range(R):
+            assert regions[r].PointInSet(ls[:, r], 1e-5) and regions[r].PointInSet(le[:, r], 1e-5)
+        rounds = [x for i in infos for x in i.iteration_info]
+        pilot = sum(x.num_pilot_samples for x in rounds)
+        checked = sum(x.num_pilot_checked for x in rounds)
+        out[mode] = (sum(i.converged for i in infos), sum(i.samples_drawn for i in infos),
+                     checked / max(pilot, 1))
+    (cp, sp, _), (cg, sg, fg) = out["plain"], out["guided"]
+    assert cg >= cp >= R - 1, out
+    assert sg < 0.9 * sp, out  # fewer exact sample checks
+    assert fg == 1.0, out  # unpruned pilot: every sample checked
+    ct, _, ft = out["pruned_trained"]
+    cu, _, fu = out["pruned_untrained"]
+    assert ct >= cp - 1 and cu >= cp - 1, out
+    assert ft < 0.8 and ft < fu - 0.15, out  # the training is what prunes the pilot
