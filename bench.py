@@ -192,6 +192,9 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
                                   "f_cfg": f_cfg}}
         if mode == "guided_fallback":
             out[mode]["fallback_regions"] = int(sum(i.plain_fallback for i in infos))
+        print(f"eizo {name} R={R} faces<={face_budget} {mode}: {out[mode]['regions_per_s']} "
+              f"regions/s, certified {out[mode]['certified_fraction']}", file=sys.stderr,
+              flush=True)
     if "plain" in faces and "plain_graph" in faces:
         out["plain_graph_identical"] = all(
             np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1])
@@ -250,6 +253,88 @@ def mlp_benchmark(rows=1 << 20, batch=8192, reps=20):
             "train_tflops": round(3 * batch * MLP_FLOPS_PER_ROW / (train_ms * 1e-3) / 1e12, 1)}
 
 
+def edge_point_counts(dp, q, edges, step, chunk=1 << 24):
+    """Work of the checkEdgeCollisionFree discretisation (n = ceil(|q2 - q1| /
+    step) points at t = k / n) over `edges` [E][2] of nodes q [N][dim]
+    (device tensors), measured with the checker on the materialised points
+    outside any timed region: the points up to and including each edge's
+    first colliding one (all n for a free edge), which is the least work a
+    sequential check can do, and the lane evaluations of the edge kernel
+    (64 points per pass, passes until one holds a collision). Also returns
+    how many edges the point checks call free."""
+    import torch
+    from csdecomp_amd import capi
+    E, dim = edges.shape[0], q.shape[1]
+    q1, q2 = q[edges[:, 0].long()], q[edges[:, 1].long()]
+    npts = torch.ceil(torch.linalg.vector_norm(q2 - q1, dim=1) / step).to(torch.int64)
+    needed = evaluated = free = 0
+    e0 = 0
+    csum = torch.cumsum(npts, 0)
+    while e0 < E:
+        base = int(csum[e0 - 1].item()) if e0 else 0
+        e1 = int(torch.searchsorted(csum, torch.tensor(base + chunk, device=q.device)).item())
+        e1 = max(e1, e0 + 1)
+        e1 = min(e1, E)
+        n = npts[e0:e1]
+        P = int(n.sum().item())
+        if P == 0:
+            free += e1 - e0
+            e0 = e1
+            continue
+        eid = torch.repeat_interleave(torch.arange(e1 - e0, device=q.device), n)
+        off = torch.cumsum(n, 0) - n
+        k = torch.arange(P, device=q.device) - off[eid]
+        t = (k.double() / n[eid].double()).float()[:, None]
+        pts = (t * q2[e0:e1][eid] + (1 - t) * q1[e0:e1][eid]).contiguous()
+        flags = torch.empty(P, dtype=torch.uint8, device=q.device)
+        capi.check_collision_free(dp, pts, P, dim, 1, flags)
+        first = n.clone()
+        col = flags == 0
+        first.scatter_reduce_(0, eid[col], k[col], reduce="amin")
+        hit = first < n
+        needed += int(torch.where(hit, first + 1, n).sum().item())
+        passes = torch.where(hit, first // 64 + 1, (n + 63) // 64)
+        evaluated += int((passes * 64).sum().item())
+        free += int((~hit).sum().item())
+        e0 = e1
+    return needed, evaluated, free
+
+
+def occupancy_tests(dp, plant, q, center, size, res):
+    """Sphere-voxel tests of the occupancy kernel: for every configuration
+    and robot sphere, the candidate voxels of its bounding box (the kernel's
+    own bounds: extent r * 1.0001 + voxel radius + 1e-6, one voxel of slack
+    on each side, clipped to the grid), from FK on the GPU."""
+    import torch
+    from csdecomp_amd import capi
+    mp = plant.getMinimalPlant()
+    geoms = mp.scene_collision_geometries
+    n = q.shape[0]
+    L = mp.kin_tree.num_links
+    T = torch.empty(n, L, 16, device=q.device)
+    capi.forward_kinematics(dp, q, n, q.shape[1], 1, T)
+    T = T.view(n, L, 4, 4).transpose(2, 3)  # column-major to row-major
+    vr = float(np.sqrt(3.0) * res)
+    two = 2.0 * res
+    nn = [int(np.ceil(size[k] / two)) for k in range(3)]
+    tests = 0
+    for gid in plant.getRobotGeometryIds():
+        g = geoms[gid]
+        if int(g.type) != 1:
+            raise ValueError("occupancy test count covers sphere geometries only")
+        X = torch.as_tensor(np.asarray(g.X_L_B, np.float32), device=q.device)
+        p = (T[:, g.link_index] @ X)[:, :3, 3]
+        ext = float(np.float32(g.dimensions[0]) * np.float32(1.0001)) + vr + 1e-6
+        cnt = torch.ones(n, dtype=torch.int64, device=q.device)
+        for k in range(3):
+            origin = center[k] - 0.5 * size[k] + res
+            lo = torch.clamp(torch.floor((p[:, k] - ext - origin) / two).long() - 1, min=0)
+            hi = torch.clamp(torch.ceil((p[:, k] + ext - origin) / two).long() + 1, max=nn[k] - 1)
+            cnt *= torch.clamp(hi - lo + 1, min=0)
+        tests += int(cnt.sum().item())
+    return tests
+
+
 def roadmap_benchmark(nodes=200_000, k=10, step=0.01):
     """Roadmap construction on the 7-DOF arm with 48 boxes + 16 spheres:
     `nodes` configurations uniform in the limits, brute-force k-NN (squared L2
@@ -291,12 +376,33 @@ def roadmap_benchmark(nodes=200_000, k=10, step=0.01):
     ev[5].record(s)
     torch.cuda.synchronize()
     knn_ms, edge_ms, occ_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3]), ev[4].elapsed_time(ev[5])
+    # algorithmic work, counted outside the timed region
+    f_cfg = algorithmic_flops_per_config(mp, pl)
+    needed, evaluated, free_pts = edge_point_counts(dp, q, edges, step)
+    j_rev = sum(1 for j in mp.kin_tree.joints if int(j.type) == 1)
+    s_mov = len(pl.getRobotGeometryIds())
+    tests = occupancy_tests(dp, pl, q, [0, 0, 0], [2, 2, 2], 1.0 / 64)
+    occ_flops = nodes * (132 * j_rev + 21 * s_mov) + 9 * tests
+    edge_tf = needed * f_cfg / (edge_ms * 1e-3) / 1e12
+    occ_tf = occ_flops / (occ_ms * 1e-3) / 1e12
     return {"nodes": nodes, "k": k, "knn_ms": round(knn_ms, 3), "edges": E,
             "edge_points_max": points, "edges_free": int(free.sum().item()),
+            "edges_free_by_point_checks": free_pts,
             "edge_validation_ms": round(edge_ms, 3),
             "edges_per_s": round(E / (edge_ms * 1e-3)),
+            "edge_points_needed": needed, "edge_lane_evaluations": evaluated,
+            "edge_points_per_s": round(needed / (edge_ms * 1e-3)),
+            "edge_roofline": {"bound": "valu", "achieved": edge_tf, "peak": FP32_VECTOR_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": edge_tf / FP32_VECTOR_PEAK_TFLOPS,
+                              "flops_per_point": f_cfg,
+                              "note": "points up to each edge's first collision x F_cfg"},
             "occupancy_voxels": int((grid > 0).sum().item()), "occupancy_ms": round(occ_ms, 3),
-            "occupancy_configs_per_s": round(nodes / (occ_ms * 1e-3))}
+            "occupancy_configs_per_s": round(nodes / (occ_ms * 1e-3)),
+            "occupancy_voxel_tests": tests,
+            "occupancy_roofline": {"bound": "valu", "achieved": occ_tf,
+                                   "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": occ_tf / FP32_VECTOR_PEAK_TFLOPS,
+                                   "note": "FK 132 J_rev + 21 per sphere + 9 per sphere-voxel test"}}
 
 
 def edit_regions_benchmark(regions=2048, per=32, steps=9, voxels=64, reps=20):

@@ -139,8 +139,14 @@ def test_voxels_vs_oracle(gpu, csd, orc):
     assert_flags_match(got, ref, sd, "voxels")
     flags = csd.CheckCollisionFreeVoxelsWithoutSelfCollisionCuda(q[:, :200], csd.Voxels(vox), 0.02, mp, rg)
     ref2 = orc.voxel_flags(op, q[:, :200], vox, 0.02, rg, orc.RULE_GPU)
-    mism = np.argwhere(np.asarray(flags) != ref2)
-    assert len(mism) <= 2, mism[:5]  # only contact-band voxel/geometry pairs may differ
+    # a (configuration, voxel) flag may differ only where the robot is within
+    # 1e-5 m of that voxel's sphere: there a voxel radius 1e-5 m smaller and
+    # one 1e-5 m larger give the oracle different verdicts
+    lo = orc.voxel_flags(op, q[:, :200], vox, 0.02 - 1e-5, rg, orc.RULE_GPU)
+    hi = orc.voxel_flags(op, q[:, :200], vox, 0.02 + 1e-5, rg, orc.RULE_GPU)
+    mism = np.asarray(flags) != ref2
+    assert not np.any(mism & (lo == hi)), np.argwhere(mism & (lo == hi))[:5]
+    assert mism.sum() <= 2, np.argwhere(mism)[:5]
 
 
 def test_fast_path_voxels(gpu, csd, orc):

@@ -312,7 +312,15 @@ def test_occupancy_vs_brute_force(gpu, orc):
     vr = np.float32(np.sqrt(3.0) * res)
     flags = orc.voxel_flags(op, q, vx.T, vr, rg, orc.RULE_GPU)  # [N][V], 1 = free
     ref = (flags == 0).any(0).astype(np.uint8)
-    assert np.mean(got == ref) > 0.999 and ref.sum() > 50
+    assert ref.sum() > 50
+    # a voxel may differ only where some configuration's robot is within
+    # 1e-5 m of its sphere: a radius 1e-5 m smaller and one 1e-5 m larger
+    # then give the oracle different occupancies
+    lo = (orc.voxel_flags(op, q, vx.T, vr - np.float32(1e-5), rg, orc.RULE_GPU) == 0).any(0)
+    hi = (orc.voxel_flags(op, q, vx.T, vr + np.float32(1e-5), rg, orc.RULE_GPU) == 0).any(0)
+    diff = got != ref
+    assert not np.any(diff & (lo == hi)), np.argwhere(diff & (lo == hi))[:5]
+    assert diff.sum() <= 4, np.argwhere(diff)[:5]
 
 
 def test_knn_vs_brute_force(gpu):
