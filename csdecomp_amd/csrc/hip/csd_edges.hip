@@ -29,7 +29,24 @@ __device__ __forceinline__ float norm_diff_rn(const float *a, const float *b, in
   return sqrtf(s);
 }
 
-// One wave per edge (grid-stride over edges); lane k checks point k + 64 j.
+// Point k of an edge under each rule (see k_validate_edges), the float
+// expression the CPU restatement evaluates
+__device__ __forceinline__ float edge_point_t(int rule, int64_t k, int64_t npts, float tstep) {
+  if (rule == 0) return (float)((double)k / (1.0 * (double)npts));
+  if (rule == 2) return (float)((double)(k + 1) / (1.0 * (double)npts));
+  return k == 0 ? 0.f : (k == 1 ? 1.f : __fmul_rn(tstep, (float)(k - 1)));
+}
+
+// Edge validation in two phases per group of 64 edges, grid-stride over
+// groups.
+//  1. Lane i takes edge i of the group and checks its first point (k = 0).
+//     In a roadmap over uniform nodes most edges end here: the first point
+//     is a node (rules 0 and 1) and most nodes collide.
+//  2. The edges whose first point is free and that have more points are
+//     taken one at a time by the whole wave, lane l checking point
+//     1 + l + 64 j, until a pass finds a collision or the points run out.
+// Every point is checked with the arithmetic a single configuration gets,
+// so the verdict (free iff no point collides) does not depend on the split.
 // rule 0: checkEdgeCollisionFree, n = ceil(|q2-q1|/step), t = k/n, k < n
 //         (plant/collision_checker.cpp:41-53)
 // rule 1: VisibilityGraph, n = (uint)(|q2-q1|/step + 2) points: q1, q2 and
@@ -49,54 +66,84 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
   R.n = 0;
   extern __shared__ __attribute__((aligned(16))) float csd_smem[];
   const float *tab = stage_tables<VAR>(P, csd_smem);
-  for (int64_t e = wave; e < E; e += nwaves) {
-    const int a = edges[2 * e], b = edges[2 * e + 1];
-    if (a < 0 || a >= num_nodes || b < 0 || b >= num_nodes) {
-      if (lane == 0) out[e] = 0;  // invalid node index: reported as not free
-      continue;
+  const int64_t groups = (E + 63) / 64;
+  for (int64_t grp = wave; grp < groups; grp += nwaves) {
+    // ---- phase 1: one edge per lane, its first point ----
+    const int64_t e = grp * 64 + lane;
+    int a = 0, b = 0;
+    bool live = e < E;  // an edge whose first point is still to be checked
+    if (live) {
+      a = edges[2 * e];
+      b = edges[2 * e + 1];
+      if (a < 0 || a >= num_nodes || b < 0 || b >= num_nodes) {
+        out[e] = 0;  // invalid node index: reported as not free
+        live = false;
+      }
     }
     float q1[kEdgeDim], q2[kEdgeDim];
 #pragma unroll
     for (int d = 0; d < kEdgeDim; ++d) {
-      q1[d] = d < dim ? nodes[(size_t)a * dim + d] : 0.f;
-      q2[d] = d < dim ? nodes[(size_t)b * dim + d] : 0.f;
+      q1[d] = live && d < dim ? nodes[(size_t)a * dim + d] : 0.f;
+      q2[d] = live && d < dim ? nodes[(size_t)b * dim + d] : 0.f;
     }
-    const float dist = norm_diff_rn(q1, q2, dim);
-    if (!(dist / step < kMaxEdgePoints)) {  // non-finite or absurdly long edge
-      if (lane == 0) out[e] = 0;
-      continue;
-    }
-    int64_t npts;
+    int64_t npts = 0;
     float tstep = 0.f;
-    if (rule == 0) {
-      npts = (int64_t)ceilf(dist / step);
-    } else if (rule == 2) {
-      npts = max((int64_t)0, (int64_t)floorf(dist / step) - 2);
-    } else {
-      npts = (int64_t)(uint32_t)(dist / step + 2);
-      tstep = (float)(1.0 / (double)(npts - 1));
-    }
-    bool any = false;
-    for (int64_t base = 0; base < npts && !any; base += 64) {
-      // idle tail lanes re-check the last point: in "any collision" mode a
-      // lane that starts as collided would end the wave's checks early
-      const int64_t k = min(base + lane, npts - 1);
-      float t;
-      if (rule == 0) {
-        t = (float)((double)k / (1.0 * (double)npts));
+    if (live) {
+      const float dist = norm_diff_rn(q1, q2, dim);
+      if (!(dist / step < kMaxEdgePoints)) {  // non-finite or absurdly long edge
+        out[e] = 0;
+        live = false;
+      } else if (rule == 0) {
+        npts = (int64_t)ceilf(dist / step);
       } else if (rule == 2) {
-        t = (float)((double)(k + 1) / (1.0 * (double)npts));
+        npts = max((int64_t)0, (int64_t)floorf(dist / step) - 2);
       } else {
-        t = k == 0 ? 0.f : (k == 1 ? 1.f : __fmul_rn(tstep, (float)(k - 1)));
+        npts = (int64_t)(uint32_t)(dist / step + 2);
+        tstep = (float)(1.0 / (double)(npts - 1));
       }
-      float q[kEdgeDim];
+      if (live && npts == 0) {  // no point to check: free
+        out[e] = 1;
+        live = false;
+      }
+    }
+    float q[kEdgeDim];
+    {
+      const float t = edge_point_t(rule, 0, live ? npts : 1, tstep);
 #pragma unroll
       for (int d = 0; d < kEdgeDim; ++d)
         q[d] = __fadd_rn(__fmul_rn(t, q2[d]), __fmul_rn(__fsub_rn(1.f, t), q1[d]));
-      const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, false, tab);
-      any = __any(col);
     }
-    if (lane == 0) out[e] = any ? 0 : 1;
+    // lanes without a live edge start as collided, so the wave leaves the
+    // pair loop once every live lane has collided
+    const bool col0 = collides_variant<VAR>(P, QReg<kEdgeDim>{q}, V, R, !live, tab);
+    if (live && (col0 || npts == 1)) out[e] = col0 ? 0 : 1;
+    // ---- phase 2: the remaining points of each surviving edge ----
+    uint64_t rest = __ballot(live && !col0 && npts > 1);
+    while (rest) {
+      const int src = __builtin_ctzll(rest);
+      rest &= rest - 1;
+      const int64_t n = __shfl(npts, src);
+      const float ts = __shfl(tstep, src);
+      float p1[kEdgeDim], p2[kEdgeDim];
+#pragma unroll
+      for (int d = 0; d < kEdgeDim; ++d) {
+        p1[d] = __shfl(q1[d], src);
+        p2[d] = __shfl(q2[d], src);
+      }
+      bool any = false;
+      for (int64_t base = 1; base < n && !any; base += 64) {
+        // idle tail lanes re-check the last point: in "any collision" mode a
+        // lane that starts as collided would end the wave's checks early
+        const int64_t k = min(base + lane, n - 1);
+        const float t = edge_point_t(rule, k, n, ts);
+#pragma unroll
+        for (int d = 0; d < kEdgeDim; ++d)
+          q[d] = __fadd_rn(__fmul_rn(t, p2[d]), __fmul_rn(__fsub_rn(1.f, t), p1[d]));
+        const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, false, tab);
+        any = __any(col);
+      }
+      if (lane == src) out[e] = any ? 0 : 1;
+    }
   }
 }
 
@@ -420,8 +467,9 @@ int csd_validate_edges(const csd_plant *plant, const float *nodes, int32_t num_n
   if (num_edges == 0) return CSD_OK;
   if (!nodes || !edges || !out) return fail(CSD_ERR_INVALID_ARGUMENT, "null device pointer");
   const int var = check_variant(plant->img, false);
-  // persistent grid: enough waves to fill 256 CUs several times over
-  const int64_t waves = std::min<int64_t>(num_edges, 256 * 64);
+  // persistent grid over groups of 64 edges: enough waves to fill 256 CUs
+  // several times over
+  const int64_t waves = std::min<int64_t>((num_edges + 63) / 64, 256 * 32);
   dim3 g((unsigned)((waves * 64 + kEdgeBlock - 1) / kEdgeBlock));
   CSD_DISPATCH_VARIANT(var, k_validate_edges, g, dim3(kEdgeBlock),
                        check_lds_bytes(plant->img, var), (hipStream_t)stream, plant->img, nodes,

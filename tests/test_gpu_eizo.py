@@ -278,6 +278,99 @@ def test_edges_vs_oracle(gpu, csd, orc):
     assert csd.CheckEdgeCollisionFree(nodes[:, 0], nodes[:, 0], mp)
 
 
+def _edge_points(q1, q2, step, rule):
+    """The points of one edge under each rule, in the float32 arithmetic the
+    edge kernel uses (sequential sum of squares, t per rule, q = t q2 +
+    (1 - t) q1 rounded per operation)."""
+    d = np.float32(0)
+    for k in range(len(q1)):
+        e = np.float32(q2[k] - q1[k])
+        d = np.float32(d + np.float32(e * e))
+    dist = np.float32(np.sqrt(d))
+    r = np.float32(dist / np.float32(step))
+    if rule == 0:
+        n = int(np.ceil(r))
+        t = [np.float32(k / (1.0 * n)) for k in range(n)]
+    elif rule == 2:
+        n = max(0, int(np.floor(r)) - 2)
+        t = [np.float32((k + 1) / (1.0 * n)) for k in range(n)]
+    else:
+        n = int(np.uint32(np.float32(r + np.float32(2))))
+        ts = np.float32(1.0 / (n - 1))
+        t = [np.float32(0), np.float32(1)] + [np.float32(ts * np.float32(k - 1)) for k in range(2, n)]
+    t = np.asarray(t, np.float32)[:, None]
+    return (t * q2[None, :] + (np.float32(1) - t) * q1[None, :]).astype(np.float32)
+
+
+@pytest.mark.parametrize("rule", [0, 1, 2])
+def test_edges_match_their_point_checks(gpu, rule):
+    """Every rule: an edge's flag is the AND of the checker's flags at its
+    points, bit for bit (the same device arithmetic), over 5,013 edges (not
+    a multiple of 64) between free and colliding nodes, with zero-length
+    edges, invalid node indices and an absurdly long edge."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    pl = scenes.arm_scene(1001, 48, 16)
+    mp = pl.getMinimalPlant()
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    dp = capi.DevicePlant(mp)
+    cand = scenes.uniform_configs(77, 20000, lo, hi).T.copy()  # [n][7]
+    dc = torch.from_numpy(cand).to(gpu)
+    fl = torch.empty(len(cand), dtype=torch.uint8, device=gpu)
+    capi.check_collision_free(dp, dc, len(cand), 7, 1, fl)
+    torch.cuda.synchronize()
+    fl = fl.cpu().numpy()
+    free_nodes, col_nodes = cand[fl == 1][:300], cand[fl == 0][:300]
+    assert len(free_nodes) >= 100
+    nodes = np.concatenate([free_nodes, col_nodes]).astype(np.float32)
+    nodes = np.concatenate([nodes, (nodes[:1] + np.float32(1e7))])  # far away: too many points
+    N = len(nodes)
+    rng = np.random.default_rng(rule)
+    nf = len(free_nodes)
+    E = 5013
+    a = np.where(rng.random(E) < 0.7, rng.integers(0, nf, E), rng.integers(0, N - 1, E))
+    # short edges between free nodes: nearest other free node
+    b = rng.integers(0, N - 1, E)
+    near = rng.random(E) < 0.5
+    fa = a < nf
+    for i in np.nonzero(near & fa)[0]:
+        d2 = ((free_nodes - nodes[a[i]]) ** 2).sum(1)
+        d2[a[i]] = np.inf
+        b[i] = int(np.argmin(d2))
+    edges = np.stack([a, b], 1).astype(np.int32)
+    edges[10] = (3, 3)  # zero length
+    edges[11] = (-1, 4)  # invalid
+    edges[12] = (5, N)  # invalid
+    edges[13] = (0, N - 1)  # too long to discretise
+    step = 0.05
+    de = torch.from_numpy(edges).to(gpu)
+    dn = torch.from_numpy(nodes).to(gpu)
+    out = torch.full((E,), 7, dtype=torch.uint8, device=gpu)
+    capi.validate_edges(dp, dn, N, de, E, step, rule, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert got[11] == 0 and got[12] == 0 and got[13] == 0
+    pts, owner = [], []
+    for e in range(E):
+        if e in (11, 12, 13):
+            continue
+        p = _edge_points(nodes[edges[e, 0]], nodes[edges[e, 1]], step, rule)
+        pts.append(p)
+        owner.append(np.full(len(p), e))
+    pts = np.concatenate(pts)
+    owner = np.concatenate(owner)
+    dq = torch.from_numpy(np.ascontiguousarray(pts)).to(gpu)
+    pf = torch.empty(len(pts), dtype=torch.uint8, device=gpu)
+    capi.check_collision_free(dp, dq, len(pts), 7, 1, pf)
+    torch.cuda.synchronize()
+    pf = pf.cpu().numpy()
+    ref = np.ones(E, np.uint8)
+    np.minimum.at(ref, owner, pf)
+    ref[[11, 12, 13]] = 0
+    np.testing.assert_array_equal(got, ref)
+    assert 0.1 < ref.mean() < 0.9, ref.mean()  # both verdicts well represented
+
+
 def test_visibility_graph_reference_pattern(gpu, csd):
     # cuda_visibility_graph_test.cpp:132-155
     p, mp, _ = _boxes()
