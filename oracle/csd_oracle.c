@@ -605,6 +605,7 @@ static int check_one(const orc_plant *p, const float *q, const float *voxels,
 
 static int set_threads(int nthreads) {
 #ifdef _OPENMP
+  if (omp_in_parallel()) return 1;  /* inside orc_eizo_batch's region loop */
   if (nthreads > 0) return nthreads;
   return omp_get_max_threads();
 #else
@@ -1205,4 +1206,31 @@ int32_t orc_eizo(const orc_plant *p, const int32_t *robot_geoms,
     info->faces_added += added;
   }
   return 0;
+}
+
+/* Regions of orc_eizo side by side: region r inflates ls[r]-le[r] (rows of
+ * [R][dim]) from the domain (A_dom, b_dom, F faces) into A_out[r] ([cap][dim])
+ * and b_out[r], with stream r; one region per OpenMP thread (dynamic
+ * schedule), each region's checks on its own thread. status[r] is orc_eizo's
+ * return code. No reference counterpart: the reference has no CPU EI-ZO;
+ * this is the CPU restatement run on every host core. */
+void orc_eizo_batch(const orc_plant *p, const int32_t *robot_geoms,
+                    int32_t num_robot_geoms, const float *A_dom, const float *b_dom,
+                    int32_t F, int32_t cap, int32_t R, const float *ls, const float *le,
+                    const orc_eizo_options *opt, int rule, int spread_starts,
+                    float *A_out, float *b_out, int32_t *nf_out, int32_t *status,
+                    orc_eizo_info *infos, int nthreads) {
+  const int dim = p->num_config_vars;
+  const int nt = set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+  for (int32_t r = 0; r < R; ++r) {
+    float *A = A_out + (size_t)r * cap * dim;
+    float *b = b_out + (size_t)r * cap;
+    memcpy(A, A_dom, sizeof(float) * (size_t)F * dim);
+    memcpy(b, b_dom, sizeof(float) * (size_t)F);
+    nf_out[r] = F;
+    status[r] = orc_eizo(p, robot_geoms, num_robot_geoms, NULL, 0, 0.f, A, b, &nf_out[r], cap,
+                         ls + (size_t)r * dim, le + (size_t)r * dim, opt, (uint32_t)r, rule,
+                         spread_starts, &infos[r]);
+  }
 }

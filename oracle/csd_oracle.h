@@ -176,6 +176,15 @@ int32_t orc_eizo(const orc_plant *p, const int32_t *robot_geoms,
                  const float *le, const orc_eizo_options *opt, uint32_t stream,
                  int rule, int spread_starts, orc_eizo_info *info);
 
+/* orc_eizo over R regions, one per OpenMP thread (ls/le rows [R][dim],
+ * outputs [R][cap][dim], [R][cap], [R]; region r uses stream r). */
+void orc_eizo_batch(const orc_plant *p, const int32_t *robot_geoms,
+                    int32_t num_robot_geoms, const float *A_dom, const float *b_dom,
+                    int32_t F, int32_t cap, int32_t R, const float *ls, const float *le,
+                    const orc_eizo_options *opt, int rule, int spread_starts,
+                    float *A_out, float *b_out, int32_t *nf_out, int32_t *status,
+                    orc_eizo_info *infos, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "build", "libcsd_oracle.so")
+# the same source built for speed (FMA contraction): CPU baseline timing only
+LIB_FAST = os.path.join(HERE, "build", "libcsd_oracle_fast.so")
 
 RULE_CPU, RULE_GPU = 0, 1
 
@@ -47,10 +49,10 @@ class EizoInfo(C.Structure):
                 ("samples_drawn", C.c_int64), ("faces_added", C.c_int32)]
 
 
-def _load():
-    if not os.path.exists(LIB):
-        raise RuntimeError(f"oracle not built: {LIB} (run `make -C oracle`)")
-    lib = C.CDLL(LIB)
+def _load(path=LIB):
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+    lib = C.CDLL(path)
     P = C.POINTER(OrcPlant)
     sig = {
         "orc_fk_f32": (None, [P, _f32p, _f32p]),
@@ -92,6 +94,9 @@ def _load():
                                  _f32p, C.POINTER(C.c_int32), C.c_int32, _f32p, _f32p,
                                  C.POINTER(EizoOptions), C.c_uint32, C.c_int, C.c_int,
                                  C.POINTER(EizoInfo)]),
+        "orc_eizo_batch": (None, [P, _i32p, C.c_int32, _f32p, _f32p, C.c_int32, C.c_int32,
+                                  C.c_int32, _f32p, _f32p, C.POINTER(EizoOptions), C.c_int,
+                                  C.c_int, _f32p, _f32p, _i32p, _i32p, C.c_void_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -101,6 +106,7 @@ def _load():
 
 
 _LIB = None
+_LIB_FAST = None
 
 
 def lib():
@@ -108,6 +114,14 @@ def lib():
     if _LIB is None:
         _LIB = _load()
     return _LIB
+
+
+def lib_fast():
+    """The speed build (CPU baseline timing; results are not checked)."""
+    global _LIB_FAST
+    if _LIB_FAST is None:
+        _LIB_FAST = _load(LIB_FAST)
+    return _LIB_FAST
 
 
 class Plant:
@@ -179,10 +193,11 @@ def fk(plant, q, double=False):
     return out.reshape(L, 4, 4).transpose(0, 2, 1)
 
 
-def check(plant, q, rule=RULE_GPU, threads=0):
+def check(plant, q, rule=RULE_GPU, threads=0, fast=False):
+    """Flags (1 = free); fast=True runs the speed build (baseline timing)."""
     qa = _aos(q, plant.dof)
     out = np.zeros(qa.shape[0], np.uint8)
-    lib().orc_check_batch(plant.ref, qa, qa.shape[0], rule, out, threads)
+    (lib_fast() if fast else lib()).orc_check_batch(plant.ref, qa, qa.shape[0], rule, out, threads)
     return out
 
 
@@ -394,6 +409,29 @@ def eizo(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, stream=0, rule
     if rc != 0:
         raise RuntimeError("oracle EI-ZO face capacity exceeded")
     return A[:nf.value], b[:nf.value], info
+
+
+def eizo_batch(plant, robot_geoms, A_dom, b_dom, ls, le, opts, cap=512, rule=RULE_GPU,
+               spread_starts=False, threads=0, fast=False):
+    """orc_eizo over the columns of ls / le (dim x R), one region per OpenMP
+    thread. Returns (A [R][cap][dim], b [R][cap], faces [R], status [R],
+    infos [R])."""
+    dim = plant.dof
+    R = ls.shape[1]
+    A = np.zeros((R, cap, dim), np.float32)
+    b = np.zeros((R, cap), np.float32)
+    nf = np.zeros(R, np.int32)
+    st = np.zeros(R, np.int32)
+    infos = (EizoInfo * R)()
+    rg = np.ascontiguousarray(robot_geoms, np.int32)
+    o = EizoOptions(*opts)
+    L = lib_fast() if fast else lib()
+    L.orc_eizo_batch(plant.ref, rg, len(rg), np.ascontiguousarray(A_dom, np.float32),
+                     np.ascontiguousarray(b_dom, np.float32), A_dom.shape[0], cap, R,
+                     np.ascontiguousarray(ls.T, np.float32), np.ascontiguousarray(le.T, np.float32),
+                     C.byref(o), rule, int(spread_starts), A.reshape(-1), b.reshape(-1), nf, st,
+                     C.cast(infos, C.c_void_p), threads)
+    return A, b, nf, st, list(infos)
 
 
 FAR = 0x3fffffff

@@ -326,3 +326,36 @@ def test_oracle_eizo_boxes_in_corners(orc):
     s = orc.hit_and_run(Ar, br, (ls + le) / 2, 4000, 50, seed=99, stream=5)
     frac = 1 - orc.check(pl, s.T, 1).mean()
     assert frac < 0.011
+def test_oracle_eizo_batch_equals_serial(orc):
+    """orc_eizo_batch (regions side by side, one per OpenMP thread, the CPU
+    baseline's path) returns, per region, exactly what orc_eizo returns for
+    that region with stream r, in both the checking and the speed builds'
+    wrappers (the speed build only where its arithmetic agrees: faces count
+    and certification)."""
+    from csdecomp_amd import scenes
+    p = scenes.boxes_in_corners()
+    mp = p.get_minimal_plant()
+    pl = orc.Plant(mp)
+    t = p.get_kinematic_tree()
+    lo, hi = t.get_position_lower_limits(), t.get_position_upper_limits()
+    A = np.zeros((4, 2), np.float32)
+    bdom = np.zeros(4, np.float32)
+    for i in range(2):
+        A[2 * i, i], A[2 * i + 1, i] = -1, 1
+        bdom[2 * i], bdom[2 * i + 1] = -lo[i], hi[i]
+    opts = (0.01, 0.5, 0.05, 0.01, 9, 500, 10, 500, 30, 1337)
+    ls = np.array([[0, 0], [0.2, -0.3], [-0.5, 0.1], [0.3, 0.3], [1.0, -0.2]], np.float32).T
+    le = np.array([[1, 0], [0.2, 0.3], [-0.2, 0.1], [0.4, 0.2], [1.2, -0.1]], np.float32).T
+    rg = list(p.get_scene_inspector().robot_geometry_ids)
+    Ab, bb, nf, st, infos = orc.eizo_batch(pl, rg, A, bdom, ls, le, opts, cap=64, threads=3)
+    for r in range(ls.shape[1]):
+        Ar, br, info = orc.eizo(pl, rg, A, bdom, ls[:, r], le[:, r], opts, cap=64, stream=r)
+        assert st[r] == 0 and nf[r] == len(br)
+        np.testing.assert_array_equal(Ab[r, :nf[r]], Ar)
+        np.testing.assert_array_equal(bb[r, :nf[r]], br)
+        assert (infos[r].iterations, infos[r].converged, infos[r].samples_drawn) == \
+            (info.iterations, info.converged, info.samples_drawn)
+    _, _, nf2, st2, infos2 = orc.eizo_batch(pl, rg, A, bdom, ls, le, opts, cap=64, fast=True)
+    assert (st2 == 0).all() and all(i.converged for i in infos2)
+
+
