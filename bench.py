@@ -156,9 +156,9 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
     faces = {}
     for mode in modes:
         inf = csd.CudaEdgeInflator(mp, list(plant.getRobotGeometryIds()), opt, dom, R, 10, nd + face_budget)
-        if mode == "plain_graph":
+        if mode in ("plain_graph", "guided_graph"):
             inf.set_graph(1)  # one graph-captured round per launch, status read each round
-        if mode in ("guided", "guided_fallback"):
+        if mode in ("guided", "guided_graph", "guided_fallback"):
             inf.set_guidance(pilot_samples=pilot, pilot_factor=10.0, surrogate=model,
                              keep_particles=keep, line_points=4,
                              plain_fallback=mode == "guided_fallback")
@@ -174,7 +174,7 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         regions, infos = inf.inflateEdges(starts, ends, None, 0.0, budgets)
         torch.cuda.synchronize()
         infl = time.perf_counter() - t0
-        el = infl + (train_s if mode in ("guided", "guided_fallback") else 0.0)
+        el = infl + (train_s if mode in ("guided", "guided_graph", "guided_fallback") else 0.0)
         conv = np.array([i.converged for i in infos])
         faces[mode] = [(r.A(), r.b()) for r in regions]
         flops = sum(eizo_region_flops(i, starts.shape[0], opt.mixing_steps, opt.bisection_steps,
@@ -197,10 +197,11 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         print(f"eizo {name} R={R} faces<={face_budget} {mode}: {out[mode]['regions_per_s']} "
               f"regions/s, certified {out[mode]['certified_fraction']}", file=sys.stderr,
               flush=True)
-    if "plain" in faces and "plain_graph" in faces:
-        out["plain_graph_identical"] = all(
-            np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1])
-            for a, g in zip(faces["plain"], faces["plain_graph"]))
+    for host, dev in (("plain", "plain_graph"), ("guided", "guided_graph")):
+        if host in faces and dev in faces:
+            out[dev + "_identical"] = all(
+                np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1])
+                for a, g in zip(faces[host], faces[dev]))
     if "plain" in out and "guided" in out:
         out["guided_vs_plain_regions_per_s"] = round(out["guided"]["regions_per_s"] / out["plain"]["regions_per_s"], 3)
         out["guided_vs_plain_samples"] = round(out["guided"]["samples"] / out["plain"]["samples"], 3)
@@ -564,7 +565,7 @@ def eizo_benchmarks(segments=2048, cpu=True):
     arm = scenes.arm_scene(1001, 48, 16)
     s1, e1 = scenes.free_segments(arm, 256, 3001, (0.2, 1.5))
     m7, tr7 = surrogate.train_surrogate(arm, s1, e1)
-    modes = ("plain", "plain_graph", "guided", "guided_untrained", "guided_fallback")
+    modes = ("plain", "plain_graph", "guided", "guided_graph", "guided_untrained", "guided_fallback")
     res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, modes, m7,
                                        train_s=tr7["label_s"] + tr7["train_s"])
     res["single_edge_arm7"]["surrogate"] = tr7

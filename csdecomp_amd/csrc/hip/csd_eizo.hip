@@ -228,6 +228,10 @@ __global__ __launch_bounds__(kRankBlock) void k_eizo_rank(EizoDev D, const float
                                                           int keep, float *__restrict__ tmp,
                                                           uint32_t round) {
   const int slot = blockIdx.x;
+  if (D.ctl) {  // device-resident round: slots past the active count idle
+    if (slot >= (int)D.ctl[1]) return;
+    round = D.ctl[2];
+  }
   const int r = D.active[slot];
   if (D.status[r] != 0) return;
   const int u = D.u[r];
@@ -345,13 +349,15 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
 
 // Device-resident round start (one workgroup): the regions still running
 // (status 0), in region order, become this round's slots, with this round's
-// sample count max(M_k, budget) and round count; ctl takes the round number,
-// M_k, the test threshold and the slot count for the round's other stages.
-// Past the last round the slot count is 0 and every stage idles.
+// sample count -- max(M_k, budget), or `pilot` when the round starts with a
+// pilot -- and round count; ctl takes the round number, M_k, the test
+// threshold and the slot count for the round's other stages. Past the last
+// round the slot count is 0 and every stage idles.
 constexpr int kBeginBlock = 1024;
 
 __global__ __launch_bounds__(kBeginBlock) void k_eizo_round_begin(EizoDev D, int Rn,
-                                                                  uint32_t rounds) {
+                                                                  uint32_t rounds,
+                                                                  uint32_t pilot) {
   __shared__ uint32_t s_w[kBeginBlock / 64];
   __shared__ uint32_t s_base;
   const uint32_t it = D.ctl[0];
@@ -370,7 +376,7 @@ __global__ __launch_bounds__(kBeginBlock) void k_eizo_round_begin(EizoDev D, int
     for (int w = 0; w < wave; ++w) off += s_w[w];
     if (act) {
       D.active[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = r;
-      D.nsamp[r] = max(M, D.budget[r]);
+      D.nsamp[r] = pilot ? pilot : max(M, D.budget[r]);
       D.iters[r] += 1;
     }
     __syncthreads();
@@ -387,6 +393,49 @@ __global__ __launch_bounds__(kBeginBlock) void k_eizo_round_begin(EizoDev D, int
     D.ctl[3] = M;
     D.ctl[4] = live ? __float_as_uint(D.ttab[it]) : 0u;
     if (live) D.ctl[0] = it + 1;
+  }
+}
+
+// Device-resident pilot decision (one workgroup), after the pilot phase:
+// the round's slots whose pilot collision count is at most `limit` keep the
+// unadaptive test; they become the slots of T (T.active, in slot order) with
+// the sample count max(M_k, budget), and T.ctl takes the slot count and the
+// round's number, M_k and threshold. The others take their particles from
+// the pilot. The host loop's rule, with the same float comparison.
+__global__ __launch_bounds__(kBeginBlock) void k_eizo_pilot_select(EizoDev D, EizoDev T,
+                                                                   float limit) {
+  __shared__ uint32_t s_w[kBeginBlock / 64];
+  __shared__ uint32_t s_base;
+  const uint32_t na = D.ctl[1], M = D.ctl[3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  for (uint32_t s0 = 0; s0 < na; s0 += kBeginBlock) {
+    const uint32_t s = s0 + threadIdx.x;
+    const int r = s < na ? D.active[s] : 0;
+    const bool keep = s < na && (float)D.pilot_col[r] <= limit;
+    const uint64_t m = __ballot(keep);
+    if (lane == 0) s_w[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = s_base;
+    for (int w = 0; w < wave; ++w) off += s_w[w];
+    if (keep) {
+      T.active[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = r;
+      D.nsamp[r] = max(M, D.budget[r]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < kBeginBlock / 64; ++w) t += s_w[w];
+      s_base += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    T.ctl[1] = s_base;
+    T.ctl[2] = D.ctl[2];
+    T.ctl[3] = M;
+    T.ctl[4] = D.ctl[4];
   }
 }
 
@@ -424,6 +473,9 @@ struct csd_eizo {
   hipStream_t gs = nullptr;
   uint32_t *ctl = nullptr, *mtab = nullptr;
   float *ttab = nullptr;
+  // the test phase of a device-resident guided round: its slots and control block
+  int32_t *active2 = nullptr;
+  uint32_t *ctl2 = nullptr;
   // the last csd_eizo_run: regions, rounds per region, wall time per round
   int last_regions = 0;
   std::vector<int32_t> last_rounds;
@@ -528,6 +580,8 @@ int csd_eizo_create(const csd_plant *plant, const int32_t *robot_geoms, int32_t 
   if (!rc) rc = dalloc(w, &D.collisions, R);
   if (!rc) rc = dalloc(w, &w->vox, std::max(max_voxels, 1) * 3);
   if (!rc) rc = dalloc(w, &w->ctl, 8);
+  if (!rc) rc = dalloc(w, &w->ctl2, 8);
+  if (!rc) rc = dalloc(w, &w->active2, R);
   if (!rc) rc = dalloc(w, &w->mtab, std::max<uint32_t>(options->max_iterations, 1));
   if (!rc) rc = dalloc(w, &w->ttab, std::max<uint32_t>(options->max_iterations, 1));
   D.T = (int)std::max<uint32_t>(options->max_iterations, 1);
@@ -763,13 +817,14 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
   };
 
   if (w->graph_sync) {
-    // Device-resident rounds: one round (start, sampling, check, test and
-    // gather, bisection, faces) is captured as a graph and replayed; the
-    // host reads the region status every graph_sync rounds only. Rounds are
-    // the host-driven ones launched with every region as a slot.
-    if (w->pilot || w->surrogate)
-      return fail(CSD_ERR_INVALID_ARGUMENT,
-                  "device-resident rounds run plain EI-ZO: clear the guidance first");
+    // Device-resident rounds: one round is captured as a graph and replayed;
+    // the host reads the region status every graph_sync rounds only. A round
+    // is the host loop's, stage for stage: start (slots, sample counts), the
+    // pilot (sampling, surrogate predictions when pruned, check, count) and
+    // its decision, the test phase (sampling, check, unadaptive test and
+    // gather), the surrogate ranking, bisection and faces. Every stage
+    // launches with every region as a slot and idles past the slot count its
+    // control block holds.
     const uint32_t T = o.max_iterations;
     std::vector<uint32_t> mt(std::max<uint32_t>(T, 1), 0u);
     std::vector<float> tt(mt.size(), 0.f);
@@ -779,10 +834,17 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     }
     const uint32_t zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     CSD_HIP(hipMemcpy(w->ctl, zero8, sizeof(zero8), hipMemcpyHostToDevice));
+    CSD_HIP(hipMemcpy(w->ctl2, zero8, sizeof(zero8), hipMemcpyHostToDevice));
     CSD_HIP(hipMemcpy(w->mtab, mt.data(), mt.size() * 4, hipMemcpyHostToDevice));
     CSD_HIP(hipMemcpy(w->ttab, tt.data(), tt.size() * 4, hipMemcpyHostToDevice));
     const hipStream_t s = w->gs;
-    const int ublk = (int)((max_budget + kSampleBlock - 1) / kSampleBlock);
+    const bool ranked = w->surrogate != nullptr;
+    const bool pruned = w->pilot && w->prune && w->surrogate;
+    const uint32_t bmax = ranked ? (uint32_t)w->keep : max_budget;
+    const int ublk = (int)((bmax + kSampleBlock - 1) / kSampleBlock);
+    const int pblk = (int)((w->pilot + kSampleBlock - 1) / kSampleBlock);
+    const float limit = w->pilot_factor * o.epsilon * (float)w->pilot;
+    const int L = w->line_points;
     // the sampler stages the largest polytope the workspace holds
     const int svar = dim <= 8 && polytope_lds_bytes<8>(cap) <= kLdsLimit    ? 17
                      : polytope_lds_bytes<16>(cap) <= kLdsLimit ? 33
@@ -790,6 +852,54 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     const size_t slds = svar == 17 ? polytope_lds_bytes<8>(cap)
                         : svar == 33 ? polytope_lds_bytes<16>(cap)
                                      : 0;
+    // buffers for every region as a slot, sized outside the capture
+    if (ranked) {
+      const size_t need = (size_t)Rn * kRankMax * L * sizeof(float);
+      if (need > w->pred_bytes) {
+        if (w->pred) hipFree(w->pred);
+        w->pred = nullptr;
+        w->pred_bytes = 0;
+        CSD_HIP(hipMalloc(&w->pred, need));
+        w->pred_bytes = need;
+      }
+    }
+    if (pruned) {
+      const size_t need = (size_t)Rn * pblk * kSampleBlock * sizeof(float);
+      if (need > w->ppred_bytes) {
+        if (w->ppred) hipFree(w->ppred);
+        w->ppred = nullptr;
+        w->ppred_bytes = 0;
+        CSD_HIP(hipMalloc(&w->ppred, need));
+        w->ppred_bytes = need;
+      }
+    }
+    auto sample = [&](const EizoDev &E, int nb, uint32_t off) {
+      const dim3 sg(nb, Rn), sb(kSampleBlock);
+      if (svar == 17 && dim <= 7)
+        hipLaunchKernelGGL((k_eizo_sample<17, true>), sg, sb, slds, s, E, dim, cap, nb,
+                           (int)o.mixing_steps, k0, k1, 0u, off);
+      else if (svar == 17)
+        hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, slds, s, E, dim, cap, nb,
+                           (int)o.mixing_steps, k0, k1, 0u, off);
+      else if (svar == 33 && dim <= 15)
+        hipLaunchKernelGGL((k_eizo_sample<33, true>), sg, sb, slds, s, E, dim, cap, nb,
+                           (int)o.mixing_steps, k0, k1, 0u, off);
+      else if (svar == 33)
+        hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, slds, s, E, dim, cap, nb,
+                           (int)o.mixing_steps, k0, k1, 0u, off);
+      else
+        hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, s, E, dim, cap, nb,
+                           (int)o.mixing_steps, k0, k1, 0u, off);
+    };
+    auto check = [&](const EizoDev &E, int nb, const float *pr, float below) {
+      const dim3 sg(nb, Rn), sb(kSampleBlock);
+      if (dp == 8)
+        CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 8, sg, sb, tab_bytes, s, P, V, RL, E, dim, nb,
+                               0u, pr, below);
+      else
+        CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, s, P, V, RL, E, dim,
+                               nb, 0u, pr, below);
+    };
     bool done = false;
     uint32_t it = 0;
     while (it < T && !done) {
@@ -797,40 +907,55 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       const uint32_t end = std::min<uint32_t>(T, std::max<uint32_t>(16u, it * 4u));
       const int nblk = (int)((std::max(mt[end - 1], max_budget) + kSampleBlock - 1) / kSampleBlock);
       if (nblk > 1024) return fail(CSD_ERR_CAPACITY, "more than 262144 samples in one round");
-      rc = grow_stage(w, Rn, nblk);
+      rc = grow_stage(w, Rn, std::max(nblk, pblk));
       if (rc) return rc;
+      if (ranked) {  // the ranking reorders through the staging buffer
+        const size_t need = (size_t)Rn * kRankMax * dim * sizeof(float);
+        if (need > w->stage_bytes) {
+          if (D.stage) hipFree(D.stage);
+          D.stage = nullptr;
+          w->stage_bytes = 0;
+          CSD_HIP(hipMalloc(&D.stage, need));
+          w->stage_bytes = need;
+        }
+      }
       EizoDev G = D;
       G.ctl = w->ctl;
       G.mtab = w->mtab;
       G.ttab = w->ttab;
+      EizoDev G2 = G;  // the test phase after a pilot: its own slots and control block
+      G2.active = w->active2;
+      G2.ctl = w->ctl2;
+      const EizoDev &GT = w->pilot ? G2 : G;
       hipGraph_t graph = nullptr;
       hipGraphExec_t exec = nullptr;
       CSD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      hipLaunchKernelGGL(k_eizo_round_begin, dim3(1), dim3(kBeginBlock), 0, s, G, Rn, T);
-      const dim3 sg(nblk, Rn), sb(kSampleBlock);
-      if (svar == 17 && dim <= 7)
-        hipLaunchKernelGGL((k_eizo_sample<17, true>), sg, sb, slds, s, G, dim, cap, nblk,
-                           (int)o.mixing_steps, k0, k1, 0u, 0u);
-      else if (svar == 17)
-        hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, slds, s, G, dim, cap, nblk,
-                           (int)o.mixing_steps, k0, k1, 0u, 0u);
-      else if (svar == 33 && dim <= 15)
-        hipLaunchKernelGGL((k_eizo_sample<33, true>), sg, sb, slds, s, G, dim, cap, nblk,
-                           (int)o.mixing_steps, k0, k1, 0u, 0u);
-      else if (svar == 33)
-        hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, slds, s, G, dim, cap, nblk,
-                           (int)o.mixing_steps, k0, k1, 0u, 0u);
-      else
-        hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, s, G, dim, cap, nblk,
-                           (int)o.mixing_steps, k0, k1, 0u, 0u);
-      if (dp == 8)
-        CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 8, sg, sb, tab_bytes, s, P, V, RL, G, dim, nblk,
-                               0u, (const float *)nullptr, 0.f);
-      else
-        CSD_DISPATCH_VARIANT_X(var, k_eizo_check, 16, sg, sb, tab_bytes, s, P, V, RL, G, dim,
-                               nblk, 0u, (const float *)nullptr, 0.f);
-      hipLaunchKernelGGL(k_eizo_compact, dim3(Rn), dim3(256), 0, s, G, dim, nblk, 0u, 0.f, U, 1,
+      int crc = CSD_OK;
+      hipLaunchKernelGGL(k_eizo_round_begin, dim3(1), dim3(kBeginBlock), 0, s, G, Rn, T,
+                         w->pilot);
+      if (w->pilot) {
+        sample(G, pblk, 0x40000000u);
+        const float *pr = nullptr;
+        if (pruned) {
+          crc = mlp_predict_staged(w->surrogate, G.stage, Rn * pblk, dim, w->ppred, s, w->ctl + 1,
+                                   pblk);
+          pr = w->ppred;
+        }
+        check(G, pblk, pr, w->prune_below);
+        hipLaunchKernelGGL(k_eizo_compact, dim3(Rn), dim3(256), 0, s, G, dim, pblk, 0u, 0.f, U,
+                           pruned ? 2 : 0, 0u);
+        hipLaunchKernelGGL(k_eizo_pilot_select, dim3(1), dim3(kBeginBlock), 0, s, G, G2, limit);
+      }
+      sample(GT, nblk, 0u);
+      check(GT, nblk, nullptr, 0.f);
+      hipLaunchKernelGGL(k_eizo_compact, dim3(Rn), dim3(256), 0, s, GT, dim, nblk, 0u, 0.f, U, 1,
                          0u);
+      if (ranked && crc == CSD_OK) {
+        SegmentPoints sp{G.part, G.ls, G.le, G.active, G.u, U, dim, L, kRankMax, w->ctl + 1};
+        crc = mlp_predict_segment_points(w->surrogate, sp, Rn, w->pred, s);
+        hipLaunchKernelGGL(k_eizo_rank, dim3(Rn), dim3(kRankBlock), 0, s, G, w->pred, dim, U, L,
+                           kRankMax, w->keep, G.stage, 0u);
+      }
       if (dp == 8)
         CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 8, dim3(ublk, Rn), dim3(kSampleBlock),
                                tab_bytes, s, P, V, RL, G, dim, U, (int)o.bisection_steps);
@@ -840,6 +965,10 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       hipLaunchKernelGGL(k_eizo_faces, dim3(Rn), dim3(kFaceBlock), 0, s, G, dim, U, cap,
                          o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1, 0u);
       hipError_t e = hipStreamEndCapture(s, &graph);
+      if (crc != CSD_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return crc;
+      }
       if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (graph) hipGraphDestroy(graph);
       while (e == hipSuccess && it < end && !done) {

@@ -123,6 +123,7 @@ struct StridedInput {
   const float *q;
   int64_t sn, sj;
   int in_dim;
+  __device__ __forceinline__ bool live(int64_t) const { return true; }
   __device__ __forceinline__ void load(int64_t r, float (&x)[kIn]) const {
 #pragma unroll
     for (int k = 0; k < kIn; ++k) x[k] = k < in_dim ? q[r * sn + (int64_t)k * sj] : 0.f;
@@ -133,6 +134,10 @@ struct StridedInput {
 // onto its segment and the particle (projection as in the bisection)
 struct SegmentInput {
   SegmentPoints s;
+  // a block whose first row lies in a slot past the live count has nothing to do
+  __device__ __forceinline__ bool live(int64_t row0) const {
+    return !s.live_slots || row0 / ((int64_t)s.cand * s.L) < (int64_t)*s.live_slots;
+  }
   __device__ __forceinline__ void load(int64_t row, float (&x)[kIn]) const {
     const int64_t per = (int64_t)s.cand * s.L;
     const int slot = (int)(row / per);
@@ -167,6 +172,11 @@ struct SegmentInput {
 struct StagedInput {
   const float *stage;
   int dim;
+  const uint32_t *live_slots;  // null: every slot
+  int64_t rows_per_slot;
+  __device__ __forceinline__ bool live(int64_t row0) const {
+    return !live_slots || row0 / rows_per_slot < (int64_t)*live_slots;
+  }
   __device__ __forceinline__ void load(int64_t r, float (&x)[kIn]) const {
     const float *seg = stage + (size_t)(r >> 8) * 256 * dim + (r & 255);
 #pragma unroll
@@ -179,6 +189,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_mlp_infer(MlpDev M, In in, int64_
                                                       float *__restrict__ out) {
   __shared__ __attribute__((aligned(16))) uint16_t sw[kW * kChunkPad];
   __shared__ __attribute__((aligned(16))) uint16_t st[4][32 * kTilePad];
+  if (!in.live((int64_t)blockIdx.x * (4 * kRowsWave))) return;  // block-uniform
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t row0 = (int64_t)blockIdx.x * (4 * kRowsWave) + wave * kRowsWave;
   const int hr = lane >> 5, c = lane & 31;
@@ -1117,14 +1128,16 @@ int csd_mlp_train_step(csd_mlp *m, const float *q, const float *target, int32_t 
 }  // extern "C"
 
 int csd::mlp_predict_staged(const csd_mlp *m, const float *stage, int num_blocks, int dim,
-                            float *out, hipStream_t st) {
+                            float *out, hipStream_t st, const uint32_t *live_slots,
+                            int blocks_per_slot) {
   if (!m) return fail(CSD_ERR_INVALID_ARGUMENT, "null surrogate");
   if (dim != m->in_dim) return fail(CSD_ERR_INVALID_ARGUMENT, "surrogate input dimension differs from the plant's");
   const int64_t n = (int64_t)num_blocks * 256;
   if (n == 0) return CSD_OK;
   const int64_t blocks = (n + 4 * kRowsWave - 1) / (4 * kRowsWave);
   hipLaunchKernelGGL((k_mlp_infer<StagedInput>), dim3((unsigned)blocks), dim3(kBlock), 0, st,
-                     m->dev(), StagedInput{stage, dim}, n, out);
+                     m->dev(), StagedInput{stage, dim, live_slots, (int64_t)blocks_per_slot * 256},
+                     n, out);
   CSD_HIP(hipGetLastError());
   return CSD_OK;
 }

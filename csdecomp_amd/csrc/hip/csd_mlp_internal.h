@@ -15,6 +15,9 @@ struct SegmentPoints {
   const float *part, *ls, *le;  // [R][U][dim], [R][dim], [R][dim]
   const int32_t *active, *u;    // [slots], [R]
   int U, dim, L, cand;
+  // device-resident rounds: the number of live slots, read on the device
+  // (rows of later slots are skipped); null: every slot is live
+  const uint32_t *live_slots = nullptr;
 };
 
 // out [num_slots][cand][L]: the surrogate's output at each point
@@ -23,8 +26,11 @@ int mlp_predict_segment_points(const csd_mlp *m, const SegmentPoints &s, int num
 
 // out [num_blocks * 256]: the surrogate's output at the samples of an EI-ZO
 // staging buffer, num_blocks segments of 256 samples as [dim][256]
+// live_slots (device, may be null): only the first *live_slots slots of
+// blocks_per_slot blocks each are predicted
 int mlp_predict_staged(const csd_mlp *m, const float *stage, int num_blocks, int dim, float *out,
-                       hipStream_t st);
+                       hipStream_t st, const uint32_t *live_slots = nullptr,
+                       int blocks_per_slot = 1);
 
 // EI-ZO workspaces that hold the surrogate register here (delta +1 on
 // attach, -1 on detach); csd_mlp_destroy refuses while the count is positive,

@@ -340,9 +340,12 @@ int csd_eizo_set_metric(csd_eizo *ws, const float *metric);
  * counters back between them, cuda_edge_inflation_zero_order.cu:109-306).
  * rounds_per_sync 1..64: a kernel builds each round's region list on the
  * device, one round is captured as a hipGraph and replayed, and the host
- * reads the region status every rounds_per_sync rounds. Results are those of
- * host-driven rounds bit for bit. 0 (the default) restores host-driven
- * rounds. Plain EI-ZO only: csd_eizo_run fails while guidance is set. */
+ * reads the region status every rounds_per_sync rounds. Guided rounds
+ * (csd_eizo_set_guidance, csd_eizo_set_pruning) are device-resident too: the
+ * pilot's decision picks the test phase's regions on the device, and the
+ * surrogate's predictions and ranking run over the device-built slot list.
+ * Results are those of host-driven rounds bit for bit. 0 (the default)
+ * restores host-driven rounds. */
 int csd_eizo_set_graph(csd_eizo *ws, int32_t rounds_per_sync);
 
 /* EI-ZO stage kernels exposed for stage-wise parity tests (device arrays).
