@@ -24,7 +24,7 @@ OBJ_DIR = os.path.join(ROOT, "_build", "obj")
 HIP_SOURCES = ["csd_plant.hip", "csd_check.hip", "csd_eizo.hip", "csd_stages.hip", "csd_edges.hip",
                "csd_mlp.hip", "csd_field.hip", "csd_screen.hip", "csd_bisect8.hip",
                "csd_bisect16.hip"]
-HOST_SOURCES = ["csdecomp.cpp", "bindings.cpp"]
+HOST_SOURCES = ["csdecomp.cpp", "cpu_checker.cpp", "bindings.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 in generated code.

@@ -662,13 +662,6 @@ std::vector<uint8_t> checkCollisionFreeVoxelsWithoutSelfCollisionCuda(
   return out;
 }
 
-namespace {
-MinimalPlant plant_of_tree(const MinimalKinematicTree &tree) {
-  MinimalPlant mp;
-  mp.kin_tree = tree;
-  return mp;
-}
-}  // namespace
 
 MatrixXf computeForwardKinematicsCuda(const MatrixXf &configurations,
                                       const MinimalKinematicTree &tree) {
@@ -687,87 +680,6 @@ MatrixXf computeForwardKinematicsCuda(const MatrixXf &configurations,
   check_status(csd_memcpy_d2h(out.data.data(), dt, bytes, nullptr), "result download");
   check_status(csd_stream_synchronize(nullptr), "synchronize");
   return out;
-}
-
-std::vector<Matrix4f> computeLinkFrameToWorldTransformsMinimal(const std::vector<float> &q,
-                                                                const MinimalKinematicTree &tree) {
-  MatrixXf c(tree.num_configuration_variables, 1);
-  require((int)q.size() == tree.num_configuration_variables,
-          "configuration has the wrong number of variables");
-  c.data = q;
-  MatrixXf tf = computeForwardKinematicsCuda(c, tree);
-  std::vector<Matrix4f> out(tree.num_links);
-  for (int l = 0; l < tree.num_links; ++l) std::memcpy(out[l].data(), &tf.data[16 * l], 64);
-  return out;
-}
-
-bool checkCollisionFree(const std::vector<float> &q, const MinimalPlant &plant) {
-  MatrixXf c((int)q.size(), 1);
-  c.data = q;
-  return checkCollisionFreeCuda(c, plant)[0] != 0;
-}
-
-bool checkCollisionFreeVoxels(const std::vector<float> &q, const MatrixXf &voxels,
-                              float voxel_radius, const MinimalPlant &plant,
-                              const std::vector<GeometryIndex> &robot_geoms) {
-  MatrixXf c((int)q.size(), 1);
-  c.data = q;
-  return checkCollisionFreeVoxelsCuda(c, voxels, voxel_radius, plant, robot_geoms)[0] != 0;
-}
-
-bool checkEdgeCollisionFree(const std::vector<float> &q1, const std::vector<float> &q2,
-                            const MinimalPlant &plant, float step_size) {
-  csd_plant *p = plant.device();
-  const int dof = plant.kin_tree.num_configuration_variables;
-  require((int)q1.size() == dof && (int)q2.size() == dof, "edge endpoints have the wrong size");
-  std::vector<float> nodes(q1);
-  nodes.insert(nodes.end(), q2.begin(), q2.end());
-  float *dn = (float *)scratch().aux.get(nodes.size() * 4);
-  int32_t e[2] = {0, 1};
-  int32_t *de = (int32_t *)scratch().aux2.get(8);
-  uint8_t *dout = (uint8_t *)scratch().out.get(1);
-  check_status(csd_memcpy_h2d(dn, nodes.data(), nodes.size() * 4, nullptr), "upload");
-  check_status(csd_memcpy_h2d(de, e, 8, nullptr), "upload");
-  check_status(csd_validate_edges(p, dn, 2, de, 1, step_size, 0, dout, nullptr), "edge check");
-  uint8_t r = 0;
-  check_status(csd_memcpy_d2h(&r, dout, 1, nullptr), "download");
-  check_status(csd_stream_synchronize(nullptr), "synchronize");
-  return r != 0;
-}
-
-bool pairCollisionFree(const CollisionGeometry &a, const Matrix4f &X_W_LA,
-                       const CollisionGeometry &b, const Matrix4f &X_W_LB) {
-  // two world-fixed links posed by X_W_LA / X_W_LB, one prismatic joint so
-  // the plant has a moving body: geometry A rides on it at q = 0
-  MinimalPlant mp;
-  MinimalKinematicTree &t = mp.kin_tree;
-  t.num_links = 3;
-  t.num_joints = 2;
-  t.num_configuration_variables = 1;
-  t.links.resize(3);
-  MinimalJoint j0, j1;
-  j0.type = PRISMATIC;
-  j0.X_PL_J = X_W_LA;
-  j0.parent_link = 0;
-  j0.child_link = 1;
-  j0.axis = {1.f, 0.f, 0.f};
-  j0.index = 0;
-  j1.type = FIXED;
-  j1.X_PL_J = X_W_LB;
-  j1.parent_link = 0;
-  j1.child_link = 2;
-  j1.index = 1;
-  t.joints = {j0, j1};
-  t.joint_traversal_sequence = {1, 0};
-  t.joint_idx_to_config_idx = {0, -1};
-  CollisionGeometry ga = a, gb = b;
-  ga.link_index = 1;
-  gb.link_index = 2;
-  mp.scene_collision_geometries = {ga, gb};
-  mp.num_scene_geometries = 2;
-  mp.collision_pairs_flat = {0, 1};
-  mp.num_collision_pairs = 1;
-  return checkCollisionFree({0.f}, mp);
 }
 
 std::vector<MatrixXf> UniformSampleInHPolyhedronCuda(const std::vector<HPolyhedron> &polys,

@@ -1,8 +1,12 @@
 // csdecomp.h -- C++ host API mirroring csdecomp's plant, collision-checker,
 // sampler and EI-ZO entry points (same class / function names and argument
-// meaning), Eigen-free. All compute goes through the C-ABI of
-// include/csdecomp_amd.h to the gfx950 kernels; nothing here computes a
-// collision, a transform or a sample on the CPU.
+// meaning), Eigen-free. The batched names (the *Cuda functions, EI-ZO, the
+// sampler, the visibility graph) go through the C-ABI of
+// include/csdecomp_amd.h to the gfx950 kernels. The reference's CPU names for
+// one configuration (checkCollisionFree, checkCollisionFreeVoxels,
+// checkEdgeCollisionFree, pairCollisionFree, geomToVoxelCollisionFree,
+// computeLinkFrameToWorldTransformsMinimal) are CPU functions, as in the
+// reference (cpu_checker.cpp).
 #pragma once
 #include <array>
 #include <cmath>
@@ -105,7 +109,7 @@ class KinematicTree {
   const std::vector<Joint> &getJoints() const { return joints_; }
   std::vector<float> getPositionLowerLimits() const { return lower_; }
   std::vector<float> getPositionUpperLimits() const { return upper_; }
-  // kinematic_tree.cpp:195-252, evaluated by the GPU FK kernel
+  // kinematic_tree.cpp:195-252, evaluated on the CPU (cpu_checker.cpp)
   std::vector<Matrix4f> computeLinkFrameToWorldTransforms(const std::vector<float> &q) const;
   MinimalKinematicTree getMyMinimalKinematicTreeStruct() const;
   int numLinks() const { return (int)links_.size(); }
@@ -251,7 +255,8 @@ std::vector<MatrixXf> UniformSampleInHPolyhedronCuda(const std::vector<HPolyhedr
                                                      int num_samples_per_hpolyhedron,
                                                      int mixing_steps, uint64_t seed = 1337);
 // single-configuration names of the reference's CPU API
-// (plant/collision_checker.h:15-75), evaluated on the GPU
+// (plant/collision_checker.h:15-75, minimal_kinematic_tree.h), evaluated on
+// the calling CPU thread with the reference's CPU comparison rules
 bool checkCollisionFree(const std::vector<float> &q, const MinimalPlant &plant);
 bool checkCollisionFreeVoxels(const std::vector<float> &q, const MatrixXf &voxels,
                               float voxel_radius, const MinimalPlant &plant,
@@ -262,6 +267,9 @@ std::vector<Matrix4f> computeLinkFrameToWorldTransformsMinimal(const std::vector
                                                                 const MinimalKinematicTree &tree);
 bool pairCollisionFree(const CollisionGeometry &a, const Matrix4f &X_W_LA,
                        const CollisionGeometry &b, const Matrix4f &X_W_LB);
+// collision_checker.cpp:205-253: robot geometry vs a voxel sphere
+bool geomToVoxelCollisionFree(const CollisionGeometry &geom, const Matrix4f &X_W_L,
+                              const Vector3f &p_W_voxel, float voxel_radius);
 
 // Visibility graph (planning/cuda_visibility_graph.h): upper-triangular edge
 // list (i < j) of mutually visible configurations, step rule of
