@@ -284,31 +284,32 @@ __global__ __launch_bounds__(kBlock, 2) void k_mlp_infer(MlpDev M, In in, int64_
 }
 
 // ---------------------------------------------------------------------------
-// Training, 10 launches per step at any batch B (a multiple of 128):
-//   k_train_prep            x0 = bf16(normalised q) as [B][32] and [32][B];
-//                           clears the small gradient accumulators
-//   k_train_fwd<kFwd> x3    layer l = 1..3: z = a_{l-1} W_l^T + b_l,
-//                           a_l = bf16(SiLU(z)) as [B][256] and [256][B],
-//                           s_l = bf16(SiLU'(z)) [B][256]
-//   k_train_fwd<kHead>      layer 4, then per row y = w5 . a_4 + b5, the
-//                           loss, dy = 2 (y - t) / B and delta_4 =
-//                           dy w5 SiLU'(z_4) (bf16, both layouts), with the
-//                           sums of db4, dw5, db5 and the loss
-//   k_train_back x4         l = 4..1: dW_l = delta_l^T a_{l-1}, each block a
-//                           tile of dW_l over a share of the batch (partial
-//                           sums, no atomics); for l > 1, other blocks of the
-//                           same launch compute delta_{l-1} =
-//                           (delta_l W_l) * s_{l-1} and the sums of db_{l-1}
-//   k_train_update          the gradient (partials summed in a fixed order),
-//                           then Adam and the bf16 weight copies (or only
-//                           the gradient, for csd_mlp_gradients)
-// GEMM operands are bf16 [rows][k] with k contiguous, staged through LDS in
-// K chunks of 32 with the next chunk's global loads in flight while the
-// current one feeds the MFMAs.
+// Training, 3 launches per step at any batch B (a multiple of 128):
+//   k_train_rows    one block per 32 rows, everything that is row-local:
+//                   x0 = bf16(normalised q); layers l = 1..4 with
+//                   z = a_{l-1} W_l^T + b_l, a_l = bf16(SiLU(z)) and
+//                   s_l = bf16(SiLU'(z)); y = w5 . a_4 + b5, the loss,
+//                   dy = 2 (y - t) / B and delta_4 = dy w5 SiLU'(z_4); then
+//                   delta_{l-1} = (delta_l W_l) * s_{l-1} for l = 4..2.
+//                   Activations, deltas and s_l stay in LDS; the transposes
+//                   a_l^T [K][B] (l = 0..3) and delta_l^T [256][B]
+//                   (l = 1..4) go to HBM for the weight gradients; the
+//                   sums of db_l, dw5, db5 and the loss go to accumulators.
+//   k_train_wgrad   dW_l = delta_l^T a_{l-1} for l = 1..4, each block a tile
+//                   of one dW_l over a share of the batch (partial sums in
+//                   a fixed order, no atomics)
+//   k_train_update  the gradient (partials summed in a fixed order), then
+//                   Adam and the bf16 weight copies (or only the gradient,
+//                   for csd_mlp_gradients); clears the accumulators
+// GEMM operands are bf16 [rows][k] with k contiguous. The weights stream
+// through one LDS buffer in K chunks of 128 (layer 1: its 32), the next
+// chunk's global loads in flight in registers while the current one feeds
+// the MFMAs; the chunks of consecutive layers form one stream, so the next
+// layer's first chunk loads during the current layer's epilogue. At these
+// sizes the step is bound by load latency and launch count, not by MFMA.
 // ---------------------------------------------------------------------------
-constexpr int kKC = 32, kKP = kKC + 8;  // K chunk, LDS row pitch (bf16)
-constexpr int kRB = 32;                 // rows per block of the row GEMMs
-enum { kFwd = 0, kHead = 1 };
+constexpr int kKC = 128;  // K chunk (bf16 elements)
+constexpr int kRB = 32;   // rows per block of the row kernel
 
 // small gradient accumulators (summed with atomics): b1..b4, w5, b5, loss
 constexpr int kAccB = 0, kAccW5 = kHid * kW, kAccB5 = kAccW5 + kW, kAccLoss = kAccB5 + 1;
@@ -316,122 +317,81 @@ constexpr int kAccLen = kAccLoss + 1;
 
 struct TrainDev {
   int B;
-  const uint16_t *a[kHid + 1];   // a_0 = x0 [B][32], a_l [B][256] (l <= 3)
-  const uint16_t *at[kHid + 1];  // transposes [K][B] (l <= 3)
-  uint16_t *aw[kHid + 1], *atw[kHid + 1];
-  uint16_t *s[kHid + 1];         // bf16(SiLU'(z_l)) [B][256], l <= 3
-  uint16_t *d[2], *dt[2];        // delta_l (ping-pong), bf16 [B][256] and [256][B]
-  float *wparts[kHid + 1];       // dW_l partial sums [S_l][256][K_l]
-  int split[kHid + 1];           // S_l
-  float *acc;                    // [kAccLen]
-  const float *target;
+  uint16_t *at[kHid];       // a_l^T [K_l][B], l = 0..3 (a_0 = x0, K_0 = 32)
+  uint16_t *dt[kHid + 1];   // delta_l^T [256][B], l = 1..4
+  float *wparts[kHid + 1];  // dW_l partial sums [S_l][256][K_l]
+  int split[kHid + 1];      // S_l
+  float *acc;               // [kAccLen], zero between steps
 };
 
-// Arguments of one launch (plain pointers: a runtime index into the arrays
-// of TrainDev would put the kernel argument block in scratch memory).
-struct FwdArgs {
-  int B, K;
-  const uint16_t *A, *W;  // a_{l-1} [B][K], W_l [256][K]
-  const float *bias;      // b_l
-  uint16_t *a, *at;       // a_l [B][256], [256][B]        (kFwd)
-  uint16_t *s;            // bf16(SiLU'(z_l)) [B][256]      (kFwd)
-  uint16_t *d, *dt;       // delta_4 [B][256], [256][B]   (kHead)
+// Arguments of the row kernel (plain pointers: a runtime index into an
+// array of the argument block would put it in scratch memory).
+struct RowArgs {
+  int B;
+  const float *q, *target;
+  uint16_t *at0, *at1, *at2, *at3;
+  uint16_t *dt1, *dt2, *dt3, *dt4;
   float *acc;
-  const float *target;
-};
-struct BackArgs {
-  int B, K, split, nrow;
-  const uint16_t *dt, *at;    // delta_l^T [256][B], a_{l-1}^T [K][B]
-  float *wparts;              // [split][256][K]
-  const uint16_t *d, *wt;     // delta_l [B][256], W_l^T [256][256]
-  const uint16_t *sprev;      // bf16(SiLU'(z_{l-1})) [B][256]
-  uint16_t *dn, *dnt;         // delta_{l-1}, both layouts
-  float *accb;                // db_{l-1} accumulators [256]
 };
 
-// K-chunk pipeline of one block: rows [r0, r0 + NA) of A and [c0, c0 + NB)
-// of Bop (each [rows][ld] bf16), chunk k0 of 32 columns, 16-byte pieces,
-// thread t taking pieces t, t + 256, ... (at most five, in named registers:
-// an array here was kept in scratch memory)
+// K-chunk staging: rows [r0, r0 + NA) of A and [c0, c0 + NB) of Bop (each
+// [rows][ld] bf16), columns k0 .. k0 + KC, as 16-byte pieces; thread t takes
+// pieces t, t + 256, ... and holds them in registers until they are stored
+// to LDS rows of pitch KC + 8 (16-byte aligned rows; the 8 extra elements
+// shift consecutive rows by four banks). NA = 0 stages Bop alone.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int NA, int NB>
+template <int NA, int NB, int KC>
 struct ChunkLoader {
-  static constexpr int kPieces = (NA + NB) * 4;
+  static constexpr int kPerRow = KC / 8;
+  static constexpr int kPieces = (NA + NB) * kPerRow;
   static constexpr int kPer = (kPieces + 255) / 256;
-  static_assert(kPer <= 5, "at most five pieces per thread");
-  u32x4 v0, v1, v2, v3, v4;
-  template <int J>
-  __device__ __forceinline__ u32x4 &reg() {
-    if constexpr (J == 0) return v0;
-    else if constexpr (J == 1) return v1;
-    else if constexpr (J == 2) return v2;
-    else if constexpr (J == 3) return v3;
-    else return v4;
-  }
-  template <int J>
-  __device__ __forceinline__ void load1(const uint16_t *A, int lda, int r0, const uint16_t *Bm,
-                                        int ldb, int c0, int k0) {
-    if constexpr (J < kPer) {
-      const int pc = threadIdx.x + 256 * J;
-      if (J * 256 + 256 <= kPieces || pc < kPieces) {
-        const int row = pc >> 2, part = pc & 3;
+  static constexpr int kPitch = KC + 8;
+  static_assert(kPer <= 16, "at most 16 pieces per thread");
+  u32x4 v[kPer];
+  __device__ __forceinline__ void load(const uint16_t *A, int lda, int r0, const uint16_t *Bm,
+                                       int ldb, int c0, int k0) {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int pc = (int)threadIdx.x + 256 * j;
+      if ((j + 1) * 256 <= kPieces || pc < kPieces) {
+        const int row = pc / kPerRow, part = pc % kPerRow;
         const uint16_t *src = row < NA ? A + (size_t)(r0 + row) * lda
                                        : Bm + (size_t)(c0 + row - NA) * ldb;
-        reg<J>() = *reinterpret_cast<const u32x4 *>(src + k0 + part * 8);
+        v[j] = *reinterpret_cast<const u32x4 *>(src + k0 + part * 8);
       }
     }
   }
-  template <int J>
-  __device__ __forceinline__ void store1(uint16_t *sA) {
-    if constexpr (J < kPer) {
-      const int pc = threadIdx.x + 256 * J;
-      if (J * 256 + 256 <= kPieces || pc < kPieces)
-        *reinterpret_cast<u32x4 *>(sA + (pc >> 2) * kKP + (pc & 3) * 8) = reg<J>();
+  __device__ __forceinline__ void store(uint16_t *sA) const {  // sA: NA + NB rows
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int pc = (int)threadIdx.x + 256 * j;
+      if ((j + 1) * 256 <= kPieces || pc < kPieces)
+        *reinterpret_cast<u32x4 *>(sA + (pc / kPerRow) * kPitch + (pc % kPerRow) * 8) = v[j];
     }
-  }
-  __device__ __forceinline__ void load(const uint16_t *A, int lda, int r0, const uint16_t *Bm,
-                                       int ldb, int c0, int k0) {
-    load1<0>(A, lda, r0, Bm, ldb, c0, k0);
-    load1<1>(A, lda, r0, Bm, ldb, c0, k0);
-    load1<2>(A, lda, r0, Bm, ldb, c0, k0);
-    load1<3>(A, lda, r0, Bm, ldb, c0, k0);
-    load1<4>(A, lda, r0, Bm, ldb, c0, k0);
-  }
-  __device__ __forceinline__ void store(uint16_t *sA) {  // sA: NA + NB rows
-    store1<0>(sA);
-    store1<1>(sA);
-    store1<2>(sA);
-    store1<3>(sA);
-    store1<4>(sA);
   }
 };
 
-// Row GEMM of one block: C[32][256] = A[r0.. r0+32][K] Bop[256][K]^T; wave w
-// holds columns 64 w .. 64 w + 63 as two 32x32 MFMA tiles.
-__device__ __forceinline__ void rows_gemm(const uint16_t *A, const uint16_t *Bm, int K, int r0,
-                                          uint16_t *lds, f32x16 (&acc)[2]) {
+// LDS of the row kernel (bf16 elements unless noted)
+constexpr int kApitch = kW + 8;                  // activations / deltas [32][264]
+constexpr int kWpitch = kKC + 8;                 // weight chunk [256][136]
+constexpr int kLdsA = kRB * kApitch;             // 8,448
+constexpr int kLdsW = kW * kWpitch;              // 34,816
+constexpr int kLdsS = kRB * kW;                  // one s_l [32][256]: 8,192
+
+// One 128-deep chunk of a layer's row GEMM: C[32][256] += sA[.][k0 .. k0+128]
+// sW^T, wave w holding columns 64 w .. 64 w + 63 as two 32x32 MFMA tiles.
+template <int KC, int PITCH>
+__device__ __forceinline__ void chunk_mfma(const uint16_t *sA, int k0, const uint16_t *sW,
+                                           f32x16 (&acc)[2]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hr = lane >> 5, c = lane & 31;
-  constexpr int kRows = kRB + kW;
-  ChunkLoader<kRB, kW> ld;
-  acc[0] = acc[1] = f32x16{};
-  const int nk = K / kKC;
-  ld.load(A, K, r0, Bm, K, 0, 0);
-  ld.store(lds);
-  __syncthreads();
-  for (int kc = 0; kc < nk; ++kc) {
-    uint16_t *cur = lds + (kc & 1) * kRows * kKP;
-    if (kc + 1 < nk) ld.load(A, K, r0, Bm, K, 0, (kc + 1) * kKC);
 #pragma unroll
-    for (int ks = 0; ks < kKC; ks += 16) {
-      const bf16x8 af = ld8(cur + c * kKP + ks + 8 * hr);
+  for (int ks = 0; ks < KC; ks += 16) {
+    const bf16x8 af = ld8(sA + c * kApitch + k0 + ks + 8 * hr);
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[j] = mfma(af, ld8(cur + (kRB + 64 * wave + 32 * j + c) * kKP + ks + 8 * hr), acc[j]);
-    }
-    if (kc + 1 < nk) ld.store(lds + ((kc + 1) & 1) * kRows * kKP);
-    __syncthreads();
+    for (int j = 0; j < 2; ++j)
+      acc[j] = mfma(af, ld8(sW + (64 * wave + 32 * j + c) * PITCH + ks + 8 * hr), acc[j]);
   }
 }
 
@@ -444,156 +404,258 @@ __device__ __forceinline__ void store_t4(uint16_t *t, int B, int col, int row,
   *reinterpret_cast<uint2 *>(t + (size_t)col * B + row) = v;
 }
 
-constexpr size_t kRowsLds = 2 * (size_t)(kRB + kW) * kKP;  // bf16 elements
+// sigmoid from the hardware exp2 and reciprocal (~1 ulp): the activation is
+// rounded to bf16 at once and SiLU' only scales the deltas (for z -> -inf,
+// exp2 -> inf and sigmoid -> 0)
+__device__ __forceinline__ float sigmoid_hw(float z) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+}
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_train_fwd(FwdArgs T, MlpDev M) {
-  __shared__ __attribute__((aligned(16))) uint16_t lds[kRowsLds];
+// The chunk stream of the row kernel: W1 (one chunk of 32), then two chunks
+// of 128 for each of W2, W3, W4 (forward) and W4^T, W3^T, W2^T (deltas).
+__device__ __forceinline__ const uint16_t *stream_matrix(const MlpDev &M, const uint16_t *const *wt,
+                                                         int n) {
+  // n = 1..12 -> chunks of the 256-deep matrices; n / 2 picks the matrix
+  const int m = (n - 1) >> 1;
+  return m == 0 ? M.wb[2] : m == 1 ? M.wb[3] : m == 2 ? M.wb[4] : m == 3 ? wt[4] : m == 4 ? wt[3]
+                                                                                      : wt[2];
+}
+
+__global__ __launch_bounds__(256) void k_train_rows(RowArgs T, MlpDev M, const uint16_t *wt2,
+                                                    const uint16_t *wt3, const uint16_t *wt4) {
+  __shared__ __attribute__((aligned(16))) uint16_t sA[kLdsA];
+  __shared__ __attribute__((aligned(16))) uint16_t sW[kLdsW];
+  __shared__ __attribute__((aligned(16))) uint16_t sS[3 * kLdsS];  // s_1, s_2, s_3
   __shared__ float s_row[4][kRB];
   __shared__ float s_dy[kRB];
+  const uint16_t *wt[kHid + 1] = {nullptr, nullptr, wt2, wt3, wt4};
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hr = lane >> 5, c = lane & 31;
-  const int K = T.K, B = T.B;
-  const int r0 = blockIdx.x * kRB;
-  f32x16 acc[2];
-  rows_gemm(T.A, T.W, K, r0, lds, acc);
-  const float *bias = T.bias;
-  float av[2][16], sv[2][16];
+  const int B = T.B, r0 = blockIdx.x * kRB;
+
+  // layer 1's weights (256 x 32) and x0 of the block's rows
+  {
+    ChunkLoader<0, kW, kIn> l1;
+    l1.load(nullptr, 0, 0, M.wb[1], kIn, 0, 0);
+    // x0: thread t normalises row t / 8, features 4 (t % 8) .. + 3
+    const int row = threadIdx.x >> 3, k0 = (threadIdx.x & 7) * 4;
+    uint16_t h[4];
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int col = 64 * wave + 32 * j + c;
-    const float bc = bias[col];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      uint16_t h[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = 4 * q + e;
-        // sigmoid from the hardware exp2 and reciprocal (~1 ulp): the
-        // activation is rounded to bf16 at once and SiLU' only scales the
-        // deltas (for z -> -inf, exp2 -> inf and sigmoid -> 0)
-        const float z = acc[j][i] + bc;
-        const float sg = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
-        h[e] = f2bf(z * sg);
-        av[j][i] = bf2f(h[e]);
-        sv[j][i] = sg * (1.f + z * (1.f - sg));
-      }
-      const int row = r0 + 8 * q + 4 * hr;
-      if (MODE == kFwd) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          T.a[(size_t)(row + e) * kW + col] = h[e];
-          T.s[(size_t)(row + e) * kW + col] = f2bf(sv[j][4 * q + e]);
-        }
-        store_t4(T.at, B, col, row, h);
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + e;
+      const float x = k < M.in_dim
+                          ? (T.q[(size_t)(r0 + row) * M.in_dim + k] - M.lo[k]) * M.scale[k] - 1.f
+                          : 0.f;
+      h[e] = f2bf(x);
+      sA[row * kApitch + k] = h[e];
+      T.at0[(size_t)k * B + r0 + row] = h[e];
     }
+    l1.store(sW);
   }
-  if (MODE == kHead) {
-    // y = w5 . a4 + b5 per row: lanes of a half-wave hold 32 columns of the
-    // same 16 rows; then the four waves' column ranges
-    float part[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) part[i] = 0.f;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const float w = bf2f(M.w5b[64 * wave + 32 * j + c]);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) part[i] = fmaf(av[j][i], w, part[i]);
+  ChunkLoader<0, kW, kKC> ld;
+  __syncthreads();
+  ld.load(nullptr, 0, 0, M.wb[2], kW, 0, 0);  // first chunk of W2, in flight
+  f32x16 acc[2];
+  acc[0] = acc[1] = f32x16{};
+  chunk_mfma<kIn, kIn + 8>(sA, 0, sW, acc);
+
+  // 12 chunks of 128: n = 1, 2 -> W2; 3, 4 -> W3; 5, 6 -> W4; 7, 8 -> W4^T;
+  // 9, 10 -> W3^T; 11, 12 -> W2^T. A layer's epilogue follows its chunk
+  // n = 0 (layer 1) or its even chunk.
+  float av[2][16], sv[2][16];
+  for (int n = 0; n <= 12; ++n) {
+    if (n > 0) {
+      chunk_mfma<kKC, kWpitch>(sA, ((n - 1) & 1) * kKC, sW, acc);
     }
+    const bool end_of_layer = n == 0 || (n & 1) == 0;
+    __syncthreads();  // every wave is done reading sA and sW
+    if (end_of_layer) {
+      // n = 0, 2, 4: forward layers 1..3; 6: layer 4 and the head;
+      // 8, 10, 12: delta_3, delta_2, delta_1
+      const int l = n / 2 + 1;  // forward layer, or 4 + (backward step)
+      if (l <= 3) {
+        const float *bias = M.p + MlpLayout::b(l);
+        uint16_t *atl = l == 1 ? T.at1 : l == 2 ? T.at2 : T.at3;
+        uint16_t *sl = sS + (l - 1) * kLdsS;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+        for (int j = 0; j < 2; ++j) {
+          const int col = 64 * wave + 32 * j + c;
+          const float bc = bias[col];
 #pragma unroll
-      for (int o = 16; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 32);
-    }
-    if (c == 0) {
+          for (int qd = 0; qd < 4; ++qd) {
+            uint16_t h[4];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) s_row[wave][8 * (i >> 2) + 4 * hr + (i & 3)] = part[i];
-    }
-    __syncthreads();
-    if (threadIdx.x < kRB) {  // lanes 0..31 of wave 0, one row each
-      const int r = threadIdx.x;
-      const float y = s_row[0][r] + s_row[1][r] + s_row[2][r] + s_row[3][r] +
-                      M.p[MlpLayout::kB5];
-      const float e = y - T.target[r0 + r];
-      const float dy = 2.f * e / (float)B;
-      s_dy[r] = dy;
-      float sdy = dy, se = e * e;
-#pragma unroll
-      for (int o = 16; o > 0; o >>= 1) {
-        sdy += __shfl_xor(sdy, o, 32);
-        se += __shfl_xor(se, o, 32);
-      }
-      if (r == 0) {
-        atomicAdd(T.acc + kAccB5, sdy);
-        atomicAdd(T.acc + kAccLoss, se);
-      }
-    }
-    __syncthreads();
-    // delta_4 = dy w5 SiLU'(z4); column sums of delta_4 (db4) and dy a4 (dw5)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = 64 * wave + 32 * j + c;
-      const float w = bf2f(M.w5b[col]);
-      float sd = 0.f, sw = 0.f;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        uint16_t h[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int i = 4 * q + e, rr = 8 * q + 4 * hr + e;
-          const float dy = s_dy[rr];
-          const float dv = dy * w * sv[j][i];
-          sd += dv;
-          sw = fmaf(dy, av[j][i], sw);
-          h[e] = f2bf(dv);
-          T.d[(size_t)(r0 + rr) * kW + col] = h[e];
+            for (int e = 0; e < 4; ++e) {
+              const int rr = 8 * qd + 4 * hr + e;
+              const float z = acc[j][4 * qd + e] + bc;
+              const float sg = sigmoid_hw(z);
+              h[e] = f2bf(z * sg);
+              sA[rr * kApitch + col] = h[e];
+              sl[rr * kW + col] = f2bf(sg * (1.f + z * (1.f - sg)));
+            }
+            store_t4(atl, B, col, r0 + 8 * qd + 4 * hr, h);
+          }
         }
-        store_t4(T.dt, B, col, r0 + 8 * q + 4 * hr, h);
+      } else if (l == 4) {
+        // layer 4, y = w5 . a4 + b5 per row, loss, delta_4 = dy w5 SiLU'(z4)
+        const float *bias = M.p + MlpLayout::b(4);
+        float part[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) part[i] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int col = 64 * wave + 32 * j + c;
+          const float bc = bias[col];
+          const float w = bf2f(M.w5b[col]);
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const float z = acc[j][i] + bc;
+            const float sg = sigmoid_hw(z);
+            av[j][i] = bf2f(f2bf(z * sg));
+            sv[j][i] = sg * (1.f + z * (1.f - sg));
+            part[i] = fmaf(av[j][i], w, part[i]);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+#pragma unroll
+          for (int o = 16; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 32);
+        }
+        if (c == 0) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) s_row[wave][8 * (i >> 2) + 4 * hr + (i & 3)] = part[i];
+        }
+        __syncthreads();
+        if (threadIdx.x < kRB) {  // lanes 0..31 of wave 0, one row each
+          const int r = threadIdx.x;
+          const float y = s_row[0][r] + s_row[1][r] + s_row[2][r] + s_row[3][r] +
+                          M.p[MlpLayout::kB5];
+          const float e = y - T.target[r0 + r];
+          const float dy = 2.f * e / (float)B;
+          s_dy[r] = dy;
+          float sdy = dy, se = e * e;
+#pragma unroll
+          for (int o = 16; o > 0; o >>= 1) {
+            sdy += __shfl_xor(sdy, o, 32);
+            se += __shfl_xor(se, o, 32);
+          }
+          if (r == 0) {
+            atomicAdd(T.acc + kAccB5, sdy);
+            atomicAdd(T.acc + kAccLoss, se);
+          }
+        }
+        __syncthreads();
+        // delta_4 (bf16) into sA and delta_4^T; column sums db4 and dw5
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int col = 64 * wave + 32 * j + c;
+          const float w = bf2f(M.w5b[col]);
+          float sd = 0.f, sw = 0.f;
+#pragma unroll
+          for (int qd = 0; qd < 4; ++qd) {
+            uint16_t h[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int i = 4 * qd + e, rr = 8 * qd + 4 * hr + e;
+              const float dy = s_dy[rr];
+              const float dv = dy * w * sv[j][i];
+              sd += dv;
+              sw = fmaf(dy, av[j][i], sw);
+              h[e] = f2bf(dv);
+              sA[rr * kApitch + col] = h[e];
+            }
+            store_t4(T.dt4, B, col, r0 + 8 * qd + 4 * hr, h);
+          }
+          sd += __shfl_xor(sd, 32);
+          sw += __shfl_xor(sw, 32);
+          if (hr == 0) {
+            atomicAdd(T.acc + kAccB + (kHid - 1) * kW + col, sd);
+            atomicAdd(T.acc + kAccW5 + col, sw);
+          }
+        }
+      } else {
+        // delta_{m-1} = (delta_m W_m) * s_{m-1}, m = 4, 3, 2 for n = 8, 10, 12
+        const int m = 8 - n / 2;
+        const uint16_t *sl = sS + (m - 2) * kLdsS;
+        uint16_t *dtl = m == 4 ? T.dt3 : m == 3 ? T.dt2 : T.dt1;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int col = 64 * wave + 32 * j + c;
+          float sd = 0.f;
+#pragma unroll
+          for (int qd = 0; qd < 4; ++qd) {
+            uint16_t h[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int rr = 8 * qd + 4 * hr + e;
+              const float dv = acc[j][4 * qd + e] * bf2f(sl[rr * kW + col]);
+              sd += dv;
+              h[e] = f2bf(dv);
+              sA[rr * kApitch + col] = h[e];
+            }
+            store_t4(dtl, B, col, r0 + 8 * qd + 4 * hr, h);
+          }
+          sd += __shfl_xor(sd, 32);
+          if (hr == 0) atomicAdd(T.acc + kAccB + (m - 2) * kW + col, sd);
+        }
       }
-      sd += __shfl_xor(sd, 32);
-      sw += __shfl_xor(sw, 32);
-      if (hr == 0) {
-        atomicAdd(T.acc + kAccB + (kHid - 1) * kW + col, sd);
-        atomicAdd(T.acc + kAccW5 + col, sw);
+      acc[0] = acc[1] = f32x16{};
+    }
+    if (n < 12) {
+      ld.store(sW);  // chunk n + 1, loaded during chunk n and its epilogue
+      __syncthreads();
+      if (n + 2 <= 12) {
+        const int nx = n + 2;  // the chunk after next: k offset by its parity
+        ld.load(nullptr, 0, 0, stream_matrix(M, wt, nx), kW, 0, ((nx - 1) & 1) * kKC);
       }
     }
   }
 }
 
-// Backward launch for layer l. Blocks [0, nrow): delta_{l-1} (l > 1); blocks
-// [nrow, nrow + tiles * S): dW_l tile t of split s.
-// dW tiles are 32 WA x 32 WB (WA * WB = 4 waves of one 32x32 MFMA tile).
+// dW_l = delta_l^T a_{l-1}: a block takes one 32 WA x 32 WB tile of dW_l
+// (WA * WB = 4 waves of one 32x32 MFMA tile) over a share of the batch and
+// writes its partial sum (no atomics).
+struct WgradArgs {
+  int B;
+  int first[kHid + 1];       // first block of layer l (l = 1..4); first[0] = total
+  int split[kHid + 1];
+  const uint16_t *dt[kHid + 1], *at[kHid + 1];  // delta_l^T [256][B], a_{l-1}^T [K][B]
+  float *wparts[kHid + 1];
+};
+
 template <int WA, int WB>
-__device__ __forceinline__ void wgrad_block(const BackArgs &T, int blk, uint16_t *lds) {
+__device__ __forceinline__ void wgrad_block(const uint16_t *A, const uint16_t *Bm, int K, int B,
+                                            int S, int blk, float *wparts, uint16_t *lds) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hr = lane >> 5, c = lane & 31;
   constexpr int BM = 32 * WA, BN = 32 * WB;
-  const int K = T.K, B = T.B, S = T.split;
   const int tn = K / BN;
   const int tile = blk / S, sp = blk - tile * S;
   const int m0 = (tile / tn) * BM, n0 = (tile % tn) * BN;
   const int nch = B / kKC, cb = sp * nch / S, ce = (sp + 1) * nch / S;
-  const uint16_t *A = T.dt, *Bm = T.at;  // delta_l^T [256][B], a_{l-1}^T [K][B]
   const int wa = wave / WB, wb = wave % WB;
-  ChunkLoader<BM, BN> ld;
+  using L = ChunkLoader<BM, BN, kKC>;
+  L ld;
   f32x16 acc = f32x16{};
-  if (cb < ce) {
+  if (cb < ce) {  // block-uniform
     ld.load(A, B, m0, Bm, B, n0, cb * kKC);
     ld.store(lds);
   }
   __syncthreads();
   for (int kc = cb; kc < ce; ++kc) {
-    uint16_t *cur = lds + ((kc - cb) & 1) * (BM + BN) * kKP;
-    if (kc + 1 < ce) ld.load(A, B, m0, Bm, B, n0, (kc + 1) * kKC);
+    const bool more = kc + 1 < ce;
+    if (more) ld.load(A, B, m0, Bm, B, n0, (kc + 1) * kKC);
 #pragma unroll
     for (int ks = 0; ks < kKC; ks += 16)
-      acc = mfma(ld8(cur + (32 * wa + c) * kKP + ks + 8 * hr),
-                 ld8(cur + (BM + 32 * wb + c) * kKP + ks + 8 * hr), acc);
-    if (kc + 1 < ce) ld.store(lds + ((kc + 1 - cb) & 1) * (BM + BN) * kKP);
-    __syncthreads();
+      acc = mfma(ld8(lds + (32 * wa + c) * L::kPitch + ks + 8 * hr),
+                 ld8(lds + (BM + 32 * wb + c) * L::kPitch + ks + 8 * hr), acc);
+    if (more) {  // every wave has read the chunk before it is overwritten
+      __syncthreads();
+      ld.store(lds);
+      __syncthreads();
+    }
   }
-  float *out = T.wparts + (size_t)sp * kW * K;
+  float *out = wparts + (size_t)sp * kW * K;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const int o = m0 + 32 * wa + 8 * (i >> 2) + 4 * hr + (i & 3);
@@ -601,57 +663,19 @@ __device__ __forceinline__ void wgrad_block(const BackArgs &T, int blk, uint16_t
   }
 }
 
-__global__ __launch_bounds__(256) void k_train_back(BackArgs T) {
-  __shared__ __attribute__((aligned(16))) uint16_t lds[kRowsLds];
-  if ((int)blockIdx.x >= T.nrow) {
-    const int blk = blockIdx.x - T.nrow;
-    if (T.K == kIn) wgrad_block<4, 1>(T, blk, lds);
-    else wgrad_block<2, 2>(T, blk, lds);
-    return;
+__global__ __launch_bounds__(256) void k_train_wgrad(WgradArgs T) {
+  __shared__ __attribute__((aligned(16))) uint16_t lds[(128 + 32) * (kKC + 8)];
+  const int b = blockIdx.x;
+  // layers in order 4, 3, 2, 1 of the block index
+  if (b < T.first[3]) {
+    wgrad_block<2, 2>(T.dt[4], T.at[4], kW, T.B, T.split[4], b - T.first[4], T.wparts[4], lds);
+  } else if (b < T.first[2]) {
+    wgrad_block<2, 2>(T.dt[3], T.at[3], kW, T.B, T.split[3], b - T.first[3], T.wparts[3], lds);
+  } else if (b < T.first[1]) {
+    wgrad_block<2, 2>(T.dt[2], T.at[2], kW, T.B, T.split[2], b - T.first[2], T.wparts[2], lds);
+  } else {
+    wgrad_block<4, 1>(T.dt[1], T.at[1], kIn, T.B, T.split[1], b - T.first[1], T.wparts[1], lds);
   }
-  // delta_{l-1} = (delta_l W_l) * s_{l-1}: Bop = W_l^T [256][256]
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int hr = lane >> 5, c = lane & 31;
-  const int B = T.B, r0 = blockIdx.x * kRB;
-  f32x16 acc[2];
-  rows_gemm(T.d, T.wt, kW, r0, lds, acc);
-  const uint16_t *sprev = T.sprev;
-  uint16_t *dn = T.dn, *dnt = T.dnt;
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int col = 64 * wave + 32 * j + c;
-    float sd = 0.f;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      uint16_t h[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = r0 + 8 * q + 4 * hr + e;
-        const float dv = acc[j][4 * q + e] * bf2f(sprev[(size_t)row * kW + col]);
-        sd += dv;
-        h[e] = f2bf(dv);
-        dn[(size_t)row * kW + col] = h[e];
-      }
-      store_t4(dnt, B, col, r0 + 8 * q + 4 * hr, h);
-    }
-    sd += __shfl_xor(sd, 32);
-    if (hr == 0) atomicAdd(T.accb + col, sd);
-  }
-}
-
-// x0 [B][32] bf16 and its transpose [32][B]; clears the accumulators
-__global__ void k_train_prep(const float *__restrict__ q, int B, int in_dim,
-                             const float *__restrict__ lo, const float *__restrict__ scale,
-                             uint16_t *__restrict__ x0, uint16_t *__restrict__ x0t,
-                             float *__restrict__ acc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kAccLen) acc[i] = 0.f;
-  if (i >= B * kIn) return;
-  const int r = i / kIn, k = i % kIn;
-  const float x = k < in_dim ? (q[(size_t)r * in_dim + k] - lo[k]) * scale[k] - 1.f : 0.f;
-  const uint16_t h = f2bf(x);
-  x0[i] = h;
-  x0t[(size_t)k * B + r] = h;
 }
 
 // The gradient of each parameter (dW partials summed over the splits in a
@@ -662,7 +686,7 @@ struct UpdateArgs {
   int B;
   const float *wparts[kHid];  // dW_l partials, l = 1..4
   int split[kHid];
-  const float *acc;
+  float *acc;  // each entry is read by one thread, which then clears it
 };
 
 __global__ void k_train_update(UpdateArgs T, float *__restrict__ p, float *__restrict__ g,
@@ -671,7 +695,10 @@ __global__ void k_train_update(UpdateArgs T, float *__restrict__ p, float *__res
                                uint16_t *wb2, uint16_t *wb3, uint16_t *wb4, uint16_t *wt2,
                                uint16_t *wt3, uint16_t *wt4, uint16_t *w5b, float *loss) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0 && loss) *loss = T.acc[kAccLoss] / (float)T.B;
+  if (i == 0) {
+    if (loss) *loss = T.acc[kAccLoss] / (float)T.B;
+    T.acc[kAccLoss] = 0.f;
+  }
   if (i >= (int64_t)MlpLayout::kTotal) return;
   int l = 0, o = 0, k = 0;  // W_l [o][k], or l = 0
   float gi;
@@ -704,15 +731,20 @@ __global__ void k_train_update(UpdateArgs T, float *__restrict__ p, float *__res
     }
     for (; sp < S; ++sp) s0 += wp[(size_t)sp * stride];
     gi = (s0 + s1) + (s2 + s3);
-  } else if (i == (int64_t)MlpLayout::kB5) {
-    gi = T.acc[kAccB5];
-  } else if (i >= (int64_t)MlpLayout::kW5) {
-    k = (int)(i - (int64_t)MlpLayout::kW5);
-    gi = T.acc[kAccW5 + k];
   } else {
-    const int lb = i < (int64_t)MlpLayout::kW2 ? 1 : i < (int64_t)MlpLayout::kW3 ? 2
-                   : i < (int64_t)MlpLayout::kW4 ? 3 : 4;
-    gi = T.acc[kAccB + (lb - 1) * kW + (int)(i - (int64_t)MlpLayout::b(lb))];
+    int a;  // the accumulator of this bias, w5 entry or b5
+    if (i == (int64_t)MlpLayout::kB5) {
+      a = kAccB5;
+    } else if (i >= (int64_t)MlpLayout::kW5) {
+      k = (int)(i - (int64_t)MlpLayout::kW5);
+      a = kAccW5 + k;
+    } else {
+      const int lb = i < (int64_t)MlpLayout::kW2 ? 1 : i < (int64_t)MlpLayout::kW3 ? 2
+                     : i < (int64_t)MlpLayout::kW4 ? 3 : 4;
+      a = kAccB + (lb - 1) * kW + (int)(i - (int64_t)MlpLayout::b(lb));
+    }
+    gi = T.acc[a];
+    T.acc[a] = 0.f;  // zero for the next step's atomics
   }
   g[i] = gi;
   if (!(lr > 0.f)) return;
@@ -829,7 +861,10 @@ int refresh(csd_mlp *m, hipStream_t st) {
   return CSD_OK;
 }
 
-// training workspace for batch B (multiple of 128), carved from m->ws
+// training workspace for batch B (multiple of 128), carved from m->ws. The
+// accumulators and the loss come first, at the same offset for every batch:
+// they are zeroed when the workspace is allocated and cleared again by each
+// step's update kernel after it reads them.
 int train_split(int l, int B) { return std::min(l == 1 ? 32 : 16, B / kKC); }
 
 size_t carve(void *base, int B, TrainDev *T, float **loss) {
@@ -843,26 +878,14 @@ size_t carve(void *base, int B, TrainDev *T, float **loss) {
   TrainDev t;
   memset(&t, 0, sizeof(t));
   t.B = B;
-  t.aw[0] = (uint16_t *)take((size_t)B * kIn * 2);
-  t.atw[0] = (uint16_t *)take((size_t)B * kIn * 2);
-  for (int l = 1; l < kHid; ++l) {
-    t.aw[l] = (uint16_t *)take(act * 2);
-    t.atw[l] = (uint16_t *)take(act * 2);
-    t.s[l] = (uint16_t *)take(act * 2);
-  }
-  for (int k = 0; k < 2; ++k) {
-    t.d[k] = (uint16_t *)take(act * 2);
-    t.dt[k] = (uint16_t *)take(act * 2);
-  }
+  t.acc = (float *)take((kAccLen + 1) * 4);
+  float *lp = t.acc + kAccLen;
+  t.at[0] = (uint16_t *)take((size_t)B * kIn * 2);
+  for (int l = 1; l < kHid; ++l) t.at[l] = (uint16_t *)take(act * 2);
+  for (int l = 1; l <= kHid; ++l) t.dt[l] = (uint16_t *)take(act * 2);
   for (int l = 1; l <= kHid; ++l) {
     t.split[l] = train_split(l, B);
     t.wparts[l] = (float *)take((size_t)t.split[l] * kW * MlpLayout::K(l) * 4);
-  }
-  t.acc = (float *)take(kAccLen * 4);
-  float *lp = (float *)take(4);
-  for (int l = 0; l <= kHid; ++l) {
-    t.a[l] = t.aw[l];
-    t.at[l] = t.atw[l];
   }
   if (T) *T = t;
   if (loss) *loss = lp;
@@ -880,60 +903,46 @@ int train_launches(csd_mlp *m, const float *q, const float *target, int B, float
     const size_t bytes = carve(nullptr, B, nullptr, nullptr);
     hipError_t e = hipMalloc(&m->ws, bytes);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc (MLP training workspace)");
+    e = hipMemset(m->ws, 0, (kAccLen + 1) * 4);  // accumulators start at zero
+    if (e != hipSuccess) return hip_fail(e, "hipMemset (MLP training workspace)");
     m->cap = B;
     m->ws_bytes = bytes;
   }
   TrainDev T;
   float *loss = nullptr;
   carve(m->ws, B, &T, &loss);
-  T.target = target;
   *loss_dev = loss;
   const MlpDev M = m->dev();
-  const int nrow = B / kRB;
-  hipLaunchKernelGGL(k_train_prep, dim3((std::max(B * kIn, kAccLen) + 255) / 256), dim3(256), 0,
-                     st, q, B, m->in_dim, m->lo, m->scale, T.aw[0], T.atw[0], T.acc);
-  for (int l = 1; l <= kHid; ++l) {
-    FwdArgs F;
-    memset(&F, 0, sizeof(F));
-    F.B = B;
-    F.K = MlpLayout::K(l);
-    F.A = T.a[l - 1];
-    F.W = m->wb[l];
-    F.bias = m->p + MlpLayout::b(l);
-    F.acc = T.acc;
-    F.target = target;
-    if (l < kHid) {
-      F.a = T.aw[l];
-      F.at = T.atw[l];
-      F.s = T.s[l];
-      hipLaunchKernelGGL(k_train_fwd<kFwd>, dim3(nrow), dim3(256), 0, st, F, M);
-    } else {
-      F.d = T.d[kHid & 1];
-      F.dt = T.dt[kHid & 1];
-      hipLaunchKernelGGL(k_train_fwd<kHead>, dim3(nrow), dim3(256), 0, st, F, M);
-    }
-  }
-  for (int l = kHid; l >= 1; --l) {
-    BackArgs A;
-    memset(&A, 0, sizeof(A));
-    A.B = B;
-    A.K = MlpLayout::K(l);
-    A.split = T.split[l];
-    A.nrow = l > 1 ? nrow : 0;
-    A.dt = T.dt[l & 1];
-    A.at = T.at[l - 1];
-    A.wparts = T.wparts[l];
-    if (l > 1) {
-      A.d = T.d[l & 1];
-      A.wt = m->wt[l];
-      A.sprev = T.s[l - 1];
-      A.dn = T.d[(l - 1) & 1];
-      A.dnt = T.dt[(l - 1) & 1];
-      A.accb = T.acc + kAccB + (l - 2) * kW;
-    }
+  RowArgs R;
+  R.B = B;
+  R.q = q;
+  R.target = target;
+  R.at0 = T.at[0];
+  R.at1 = T.at[1];
+  R.at2 = T.at[2];
+  R.at3 = T.at[3];
+  R.dt1 = T.dt[1];
+  R.dt2 = T.dt[2];
+  R.dt3 = T.dt[3];
+  R.dt4 = T.dt[4];
+  R.acc = T.acc;
+  hipLaunchKernelGGL(k_train_rows, dim3(B / kRB), dim3(256), 0, st, R, M, m->wt[2], m->wt[3],
+                     m->wt[4]);
+  WgradArgs W;
+  memset(&W, 0, sizeof(W));
+  W.B = B;
+  int nb = 0;
+  for (int l = kHid; l >= 1; --l) {  // blocks of layers 4, 3, 2, 1 in that order
+    W.first[l] = nb;
+    W.split[l] = T.split[l];
+    W.dt[l] = T.dt[l];
+    W.at[l] = T.at[l - 1];
+    W.wparts[l] = T.wparts[l];
     const int tiles = l == 1 ? (kW / 128) * (kIn / 32) : (kW / 64) * (kW / 64);
-    hipLaunchKernelGGL(k_train_back, dim3(A.nrow + tiles * A.split), dim3(256), 0, st, A);
+    nb += tiles * T.split[l];
   }
+  W.first[0] = nb;
+  hipLaunchKernelGGL(k_train_wgrad, dim3(nb), dim3(256), 0, st, W);
   UpdateArgs U;
   U.B = B;
   for (int l = 1; l <= kHid; ++l) {

@@ -120,6 +120,37 @@ def test_gradients_match_oracle(gpu):
     assert l2 < l1
 
 
+def test_gradients_across_batch_sizes(gpu):
+    """Gradient calls in a row with batches 384, 128 (the workspace is carved
+    again for a smaller batch), 1024 (it grows) and 384 again, plus an Adam
+    step in between: each call's gradient and loss match the oracle's for its
+    own batch, so the gradient accumulators start every step at zero."""
+    import torch
+    in_dim = 7
+    m, lo, hi = _model(in_dim, 9)
+    rng = np.random.default_rng(21)
+    for i, B in enumerate((384, 128, 1024, 384)):
+        q = rng.uniform(-np.pi, np.pi, (B, in_dim)).astype(np.float32)
+        t = rng.uniform(-1, 1, B).astype(np.float32)
+        dq, dt = torch.from_numpy(q).to(gpu), torch.from_numpy(t).to(gpu)
+        grads = torch.empty(m.num_params, device=gpu)
+        loss = torch.empty(1, device=gpu)
+        m.gradients(dq, dt, B, grads, loss)
+        torch.cuda.synchronize()
+        p = m.get_params().astype(np.float64)
+        ref_loss, ref = omlp.loss_and_grads(p, omlp.normalise(q, lo, hi), t)
+        assert abs(loss.item() - ref_loss) < 1e-3 * (1 + ref_loss), (B, loss.item(), ref_loss)
+        g = grads.cpu().numpy().astype(np.float64)
+        for name, (o, s) in omlp.OFFSETS.items():
+            k = int(np.prod(s))
+            a, b = g[o:o + k], ref[o:o + k]
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+            assert rel < 3e-2, (B, name, rel)
+        if i == 1:
+            m.train_step(dq, dt, B, 1e-3)  # an Adam step between two gradient calls
+            torch.cuda.synchronize()
+
+
 def test_batch_argument_checks(gpu):
     import torch
     from csdecomp_amd import capi
