@@ -413,12 +413,160 @@ __device__ __forceinline__ float sigmoid_hw(float z) {
 
 // The chunk stream of the row kernel: W1 (one chunk of 32), then two chunks
 // of 128 for each of W2, W3, W4 (forward) and W4^T, W3^T, W2^T (deltas).
-__device__ __forceinline__ const uint16_t *stream_matrix(const MlpDev &M, const uint16_t *const *wt,
-                                                         int n) {
-  // n = 1..12 -> chunks of the 256-deep matrices; n / 2 picks the matrix
+// Chunk n = 1..12 is columns ((n - 1) % 2) * 128 .. + 128 of matrix (n - 1) / 2.
+__device__ __forceinline__ const uint16_t *stream_matrix(const MlpDev &M, const uint16_t *wt2,
+                                                         const uint16_t *wt3,
+                                                         const uint16_t *wt4, int n) {
   const int m = (n - 1) >> 1;
-  return m == 0 ? M.wb[2] : m == 1 ? M.wb[3] : m == 2 ? M.wb[4] : m == 3 ? wt[4] : m == 4 ? wt[3]
-                                                                                      : wt[2];
+  return m == 0 ? M.wb[2] : m == 1 ? M.wb[3] : m == 2 ? M.wb[4] : m == 3 ? wt4 : m == 4 ? wt3
+                                                                                      : wt2;
+}
+
+struct RowLds {
+  uint16_t *sA;  // [32][kApitch]: the next GEMM's A operand
+  uint16_t *sS;  // s_1, s_2, s_3, each [32][256]
+  float (*s_row)[kRB];
+  float *s_dy;
+};
+
+// The epilogue after chunk n (n = 0 or even): n = 0, 2, 4 forward layers
+// 1..3; 6 layer 4 and the head; 8, 10, 12 delta_3, delta_2, delta_1. bf16
+// rounding by the hardware conversion (round to nearest even, as f2bf, for
+// these finite values).
+__device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpDev &M,
+                                             const RowLds &L, f32x16 (&acc)[2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int hr = lane >> 5, c = lane & 31;
+  const int B = T.B, r0 = blockIdx.x * kRB;
+  const int l = n / 2 + 1;  // forward layer, or 4 + (backward step)
+  if (l <= 3) {
+    const float *bias = M.p + MlpLayout::b(l);
+    uint16_t *atl = l == 1 ? T.at1 : l == 2 ? T.at2 : T.at3;
+    uint16_t *sl = L.sS + (l - 1) * kLdsS;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 64 * wave + 32 * j + c;
+      const float bc = bias[col];
+#pragma unroll
+      for (int qd = 0; qd < 4; ++qd) {
+        uint16_t h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int rr = 8 * qd + 4 * hr + e;
+          const float z = acc[j][4 * qd + e] + bc;
+          const float sg = sigmoid_hw(z);
+          h[e] = f2bf_hw(z * sg);
+          L.sA[rr * kApitch + col] = h[e];
+          sl[rr * kW + col] = f2bf_hw(sg * (1.f + z * (1.f - sg)));
+        }
+        store_t4(atl, B, col, r0 + 8 * qd + 4 * hr, h);
+      }
+    }
+  } else if (l == 4) {
+    // layer 4, y = w5 . a4 + b5 per row, loss, delta_4 = dy w5 SiLU'(z4)
+    const float *bias = M.p + MlpLayout::b(4);
+    float av[2][16], sv[2][16];
+    float part[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) part[i] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 64 * wave + 32 * j + c;
+      const float bc = bias[col];
+      const float w = bf2f(M.w5b[col]);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const float z = acc[j][i] + bc;
+        const float sg = sigmoid_hw(z);
+        av[j][i] = bf2f(f2bf_hw(z * sg));
+        sv[j][i] = sg * (1.f + z * (1.f - sg));
+        part[i] = fmaf(av[j][i], w, part[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 32);
+    }
+    if (c == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) L.s_row[wave][8 * (i >> 2) + 4 * hr + (i & 3)] = part[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < kRB) {  // lanes 0..31 of wave 0, one row each
+      const int r = threadIdx.x;
+      const float y = L.s_row[0][r] + L.s_row[1][r] + L.s_row[2][r] + L.s_row[3][r] +
+                      M.p[MlpLayout::kB5];
+      const float e = y - T.target[r0 + r];
+      const float dy = 2.f * e / (float)B;
+      L.s_dy[r] = dy;
+      float sdy = dy, se = e * e;
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1) {
+        sdy += __shfl_xor(sdy, o, 32);
+        se += __shfl_xor(se, o, 32);
+      }
+      if (r == 0) {
+        atomicAdd(T.acc + kAccB5, sdy);
+        atomicAdd(T.acc + kAccLoss, se);
+      }
+    }
+    __syncthreads();
+    // delta_4 (bf16) into sA and delta_4^T; column sums db4 and dw5
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 64 * wave + 32 * j + c;
+      const float w = bf2f(M.w5b[col]);
+      float sd = 0.f, sw = 0.f;
+#pragma unroll
+      for (int qd = 0; qd < 4; ++qd) {
+        uint16_t h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = 4 * qd + e, rr = 8 * qd + 4 * hr + e;
+          const float dy = L.s_dy[rr];
+          const float dv = dy * w * sv[j][i];
+          sd += dv;
+          sw = fmaf(dy, av[j][i], sw);
+          h[e] = f2bf_hw(dv);
+          L.sA[rr * kApitch + col] = h[e];
+        }
+        store_t4(T.dt4, B, col, r0 + 8 * qd + 4 * hr, h);
+      }
+      sd += __shfl_xor(sd, 32);
+      sw += __shfl_xor(sw, 32);
+      if (hr == 0) {
+        atomicAdd(T.acc + kAccB + (kHid - 1) * kW + col, sd);
+        atomicAdd(T.acc + kAccW5 + col, sw);
+      }
+    }
+  } else {
+    // delta_{m-1} = (delta_m W_m) * s_{m-1}, m = 4, 3, 2 for n = 8, 10, 12
+    const int m = 8 - n / 2;
+    const uint16_t *sl = L.sS + (m - 2) * kLdsS;
+    uint16_t *dtl = m == 4 ? T.dt3 : m == 3 ? T.dt2 : T.dt1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 64 * wave + 32 * j + c;
+      float sd = 0.f;
+#pragma unroll
+      for (int qd = 0; qd < 4; ++qd) {
+        uint16_t h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int rr = 8 * qd + 4 * hr + e;
+          const float dv = acc[j][4 * qd + e] * bf2f(sl[rr * kW + col]);
+          sd += dv;
+          h[e] = f2bf_hw(dv);
+          L.sA[rr * kApitch + col] = h[e];
+        }
+        store_t4(dtl, B, col, r0 + 8 * qd + 4 * hr, h);
+      }
+      sd += __shfl_xor(sd, 32);
+      if (hr == 0) atomicAdd(T.acc + kAccB + (m - 2) * kW + col, sd);
+    }
+  }
+  acc[0] = acc[1] = f32x16{};
 }
 
 __global__ __launch_bounds__(256) void k_train_rows(RowArgs T, MlpDev M, const uint16_t *wt2,
@@ -428,9 +576,7 @@ __global__ __launch_bounds__(256) void k_train_rows(RowArgs T, MlpDev M, const u
   __shared__ __attribute__((aligned(16))) uint16_t sS[3 * kLdsS];  // s_1, s_2, s_3
   __shared__ float s_row[4][kRB];
   __shared__ float s_dy[kRB];
-  const uint16_t *wt[kHid + 1] = {nullptr, nullptr, wt2, wt3, wt4};
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int hr = lane >> 5, c = lane & 31;
+  const RowLds L{sA, sS, s_row, s_dy};
   const int B = T.B, r0 = blockIdx.x * kRB;
 
   // layer 1's weights (256 x 32) and x0 of the block's rows
@@ -452,162 +598,33 @@ __global__ __launch_bounds__(256) void k_train_rows(RowArgs T, MlpDev M, const u
     }
     l1.store(sW);
   }
-  ChunkLoader<0, kW, kKC> ld;
+  // Chunks n = 1..12 stream through sW, two in flight at a time in
+  // registers: odd chunks in `ldo`, even ones in `lde`.
+  ChunkLoader<0, kW, kKC> ldo, lde;
   __syncthreads();
-  ld.load(nullptr, 0, 0, M.wb[2], kW, 0, 0);  // first chunk of W2, in flight
+  ldo.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 1), kW, 0, 0);
+  lde.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 2), kW, 0, kKC);
   f32x16 acc[2];
   acc[0] = acc[1] = f32x16{};
   chunk_mfma<kIn, kIn + 8>(sA, 0, sW, acc);
-
-  // 12 chunks of 128: n = 1, 2 -> W2; 3, 4 -> W3; 5, 6 -> W4; 7, 8 -> W4^T;
-  // 9, 10 -> W3^T; 11, 12 -> W2^T. A layer's epilogue follows its chunk
-  // n = 0 (layer 1) or its even chunk.
-  float av[2][16], sv[2][16];
-  for (int n = 0; n <= 12; ++n) {
-    if (n > 0) {
-      chunk_mfma<kKC, kWpitch>(sA, ((n - 1) & 1) * kKC, sW, acc);
-    }
-    const bool end_of_layer = n == 0 || (n & 1) == 0;
-    __syncthreads();  // every wave is done reading sA and sW
-    if (end_of_layer) {
-      // n = 0, 2, 4: forward layers 1..3; 6: layer 4 and the head;
-      // 8, 10, 12: delta_3, delta_2, delta_1
-      const int l = n / 2 + 1;  // forward layer, or 4 + (backward step)
-      if (l <= 3) {
-        const float *bias = M.p + MlpLayout::b(l);
-        uint16_t *atl = l == 1 ? T.at1 : l == 2 ? T.at2 : T.at3;
-        uint16_t *sl = sS + (l - 1) * kLdsS;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int col = 64 * wave + 32 * j + c;
-          const float bc = bias[col];
-#pragma unroll
-          for (int qd = 0; qd < 4; ++qd) {
-            uint16_t h[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int rr = 8 * qd + 4 * hr + e;
-              const float z = acc[j][4 * qd + e] + bc;
-              const float sg = sigmoid_hw(z);
-              h[e] = f2bf(z * sg);
-              sA[rr * kApitch + col] = h[e];
-              sl[rr * kW + col] = f2bf(sg * (1.f + z * (1.f - sg)));
-            }
-            store_t4(atl, B, col, r0 + 8 * qd + 4 * hr, h);
-          }
-        }
-      } else if (l == 4) {
-        // layer 4, y = w5 . a4 + b5 per row, loss, delta_4 = dy w5 SiLU'(z4)
-        const float *bias = M.p + MlpLayout::b(4);
-        float part[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) part[i] = 0.f;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int col = 64 * wave + 32 * j + c;
-          const float bc = bias[col];
-          const float w = bf2f(M.w5b[col]);
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const float z = acc[j][i] + bc;
-            const float sg = sigmoid_hw(z);
-            av[j][i] = bf2f(f2bf(z * sg));
-            sv[j][i] = sg * (1.f + z * (1.f - sg));
-            part[i] = fmaf(av[j][i], w, part[i]);
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-#pragma unroll
-          for (int o = 16; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 32);
-        }
-        if (c == 0) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) s_row[wave][8 * (i >> 2) + 4 * hr + (i & 3)] = part[i];
-        }
-        __syncthreads();
-        if (threadIdx.x < kRB) {  // lanes 0..31 of wave 0, one row each
-          const int r = threadIdx.x;
-          const float y = s_row[0][r] + s_row[1][r] + s_row[2][r] + s_row[3][r] +
-                          M.p[MlpLayout::kB5];
-          const float e = y - T.target[r0 + r];
-          const float dy = 2.f * e / (float)B;
-          s_dy[r] = dy;
-          float sdy = dy, se = e * e;
-#pragma unroll
-          for (int o = 16; o > 0; o >>= 1) {
-            sdy += __shfl_xor(sdy, o, 32);
-            se += __shfl_xor(se, o, 32);
-          }
-          if (r == 0) {
-            atomicAdd(T.acc + kAccB5, sdy);
-            atomicAdd(T.acc + kAccLoss, se);
-          }
-        }
-        __syncthreads();
-        // delta_4 (bf16) into sA and delta_4^T; column sums db4 and dw5
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int col = 64 * wave + 32 * j + c;
-          const float w = bf2f(M.w5b[col]);
-          float sd = 0.f, sw = 0.f;
-#pragma unroll
-          for (int qd = 0; qd < 4; ++qd) {
-            uint16_t h[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int i = 4 * qd + e, rr = 8 * qd + 4 * hr + e;
-              const float dy = s_dy[rr];
-              const float dv = dy * w * sv[j][i];
-              sd += dv;
-              sw = fmaf(dy, av[j][i], sw);
-              h[e] = f2bf(dv);
-              sA[rr * kApitch + col] = h[e];
-            }
-            store_t4(T.dt4, B, col, r0 + 8 * qd + 4 * hr, h);
-          }
-          sd += __shfl_xor(sd, 32);
-          sw += __shfl_xor(sw, 32);
-          if (hr == 0) {
-            atomicAdd(T.acc + kAccB + (kHid - 1) * kW + col, sd);
-            atomicAdd(T.acc + kAccW5 + col, sw);
-          }
-        }
-      } else {
-        // delta_{m-1} = (delta_m W_m) * s_{m-1}, m = 4, 3, 2 for n = 8, 10, 12
-        const int m = 8 - n / 2;
-        const uint16_t *sl = sS + (m - 2) * kLdsS;
-        uint16_t *dtl = m == 4 ? T.dt3 : m == 3 ? T.dt2 : T.dt1;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int col = 64 * wave + 32 * j + c;
-          float sd = 0.f;
-#pragma unroll
-          for (int qd = 0; qd < 4; ++qd) {
-            uint16_t h[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int rr = 8 * qd + 4 * hr + e;
-              const float dv = acc[j][4 * qd + e] * bf2f(sl[rr * kW + col]);
-              sd += dv;
-              h[e] = f2bf(dv);
-              sA[rr * kApitch + col] = h[e];
-            }
-            store_t4(dtl, B, col, r0 + 8 * qd + 4 * hr, h);
-          }
-          sd += __shfl_xor(sd, 32);
-          if (hr == 0) atomicAdd(T.acc + kAccB + (m - 2) * kW + col, sd);
-        }
-      }
-      acc[0] = acc[1] = f32x16{};
-    }
-    if (n < 12) {
-      ld.store(sW);  // chunk n + 1, loaded during chunk n and its epilogue
+  __syncthreads();  // every wave is done reading sA and sW
+  row_epilogue(0, T, M, L, acc);
+  ldo.store(sW);
+  __syncthreads();
+  ldo.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 3), kW, 0, 0);
+  for (int pr = 1; pr <= 6; ++pr) {
+    chunk_mfma<kKC, kWpitch>(sA, 0, sW, acc);  // chunk 2 pr - 1
+    __syncthreads();
+    lde.store(sW);
+    __syncthreads();
+    if (pr < 6) lde.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 2 * pr + 2), kW, 0, kKC);
+    chunk_mfma<kKC, kWpitch>(sA, kKC, sW, acc);  // chunk 2 pr
+    __syncthreads();
+    row_epilogue(2 * pr, T, M, L, acc);
+    if (pr < 6) {
+      ldo.store(sW);  // chunk 2 pr + 1
       __syncthreads();
-      if (n + 2 <= 12) {
-        const int nx = n + 2;  // the chunk after next: k offset by its parity
-        ld.load(nullptr, 0, 0, stream_matrix(M, wt, nx), kW, 0, ((nx - 1) & 1) * kKC);
-      }
+      if (pr < 5) ldo.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 2 * pr + 3), kW, 0, 0);
     }
   }
 }
