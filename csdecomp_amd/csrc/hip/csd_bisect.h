@@ -44,21 +44,20 @@ __global__ __launch_bounds__(kSampleBlock) void k_bisect_standalone(
   const bool valid = i < n;
   const int ii = valid ? i : n - 1;
   const uint32_t s_idx = seg[ii];
-  float p[DP], q1[DP], q2[DP], sc[DP], pr[DP];
+  float p[DP], q1[DP], q2[DP], sc[DP];
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
     p[d] = d < dim ? part[(size_t)ii * dim + d] : 0.f;
     q1[d] = d < dim ? ls[(size_t)s_idx * dim + d] : 0.f;
     q2[d] = d < dim ? le[(size_t)s_idx * dim + d] : 0.f;
   }
-  bisect_particle<VAR, DP, S>(P, V, R, p, q1, q2, steps, !valid, sc, pr, tab, sub, verdict);
-  if (!valid || sub != 0) return;
+  const bool owner = valid && sub == 0;  // writes the particle's results
+  bisect_particle<VAR, DP, S>(P, V, R, p, q1, q2, steps, !valid, sc,
+                              owner ? proj + (size_t)i * dim : nullptr, dim, tab, sub, verdict);
+  if (!owner) return;
 #pragma unroll
   for (int d = 0; d < DP; ++d)
-    if (d < dim) {
-      scol[(size_t)i * dim + d] = sc[d];
-      proj[(size_t)i * dim + d] = pr[d];
-    }
+    if (d < dim) scol[(size_t)i * dim + d] = sc[d];
 }
 
 

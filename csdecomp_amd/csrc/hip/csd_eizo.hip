@@ -325,22 +325,25 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
   const float *tab = stage_tables<VAR>(P, csd_smem);
   const bool valid = i < u;
   const int ii = valid ? i : u - 1;
-  float p[DP], q1[DP], q2[DP], sc[DP], pr[DP];
+  float p[DP], q1[DP], q2[DP], sc[DP];
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
     p[d] = d < dim ? D.part[((size_t)r * U + ii) * dim + d] : 0.f;
     q1[d] = d < dim ? D.ls[(size_t)r * dim + d] : 0.f;
     q2[d] = d < dim ? D.le[(size_t)r * dim + d] : 0.f;
   }
-  bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc, pr, tab);
+  // the projection goes to D.proj before the steps and is read back for the
+  // distance (this lane's own stores)
+  float *proj = D.proj + ((size_t)r * U + i) * dim;
+  bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc, valid ? proj : nullptr, dim,
+                           tab);
   if (!valid) return;
   double dd = 0.0;
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
     if (d < dim) {
       D.scol[((size_t)r * U + i) * dim + d] = sc[d];
-      D.proj[((size_t)r * U + i) * dim + d] = pr[d];
-      const double e = (double)sc[d] - (double)pr[d];
+      const double e = (double)sc[d] - (double)proj[d];
       dd += e * e;
     }
   }

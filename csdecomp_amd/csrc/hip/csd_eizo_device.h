@@ -234,11 +234,14 @@ __device__ __forceinline__ float seg_point(float t, float q1, float q2) {
   return (1.f - t) * q1 + t * q2;
 }
 
-// Projection of particle p onto the segment q1-q2 (written to pr) and
+// Projection of particle p onto the segment q1-q2 (its first `dim`
+// coordinates written to proj_out right away, unless proj_out is null) and
 // `steps` bisection steps between the projection (assumed free) and p
 // (colliding), with FK + collision check inline (cuda_set_builder_utils.cu
 // :25-67, 239-357). sc is the last colliding midpoint (p itself if none
-// collides). Idle lanes pass idle = true.
+// collides): it always equals the upper bound, so only the two bounds stay
+// live across the steps (the projection goes to memory before them).
+// Idle lanes pass idle = true.
 // With S > 1, S consecutive waves of the workgroup hold the same 64
 // particles; wave `sub` of the group checks its share of the pairs and the
 // verdicts are OR-ed through `part` (one byte per thread) at every step, so
@@ -258,16 +261,15 @@ __device__ __forceinline__ void bisect_particle(const PlantImage &P, const Voxel
                                                 const RobotGeomList &R, const float (&p)[DP],
                                                 const float (&q1)[DP], const float (&q2)[DP],
                                                 int steps, bool idle, float (&sc)[DP],
-                                                float (&pr)[DP], const float *tab, int sub = 0,
-                                                uint8_t *part = nullptr) {
+                                                float *proj_out, int dim, const float *tab,
+                                                int sub = 0, uint8_t *part = nullptr) {
   const float t = seg_param<DP>(p, q1, q2);
   float lo[DP], hi[DP], mid[DP];
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
-    pr[d] = seg_point(t, q1[d], q2[d]);
-    lo[d] = pr[d];
+    lo[d] = seg_point(t, q1[d], q2[d]);
     hi[d] = p[d];
-    sc[d] = p[d];
+    if (proj_out && d < dim) proj_out[d] = lo[d];
   }
   for (int s = 0; s < steps; ++s) {
 #pragma unroll
@@ -296,10 +298,11 @@ __device__ __forceinline__ void bisect_particle(const PlantImage &P, const Voxel
 #pragma unroll
     for (int d = 0; d < DP; ++d) {
       hi[d] = col ? mid[d] : hi[d];
-      sc[d] = col ? mid[d] : sc[d];
       lo[d] = col ? lo[d] : mid[d];
     }
   }
+#pragma unroll
+  for (int d = 0; d < DP; ++d) sc[d] = hi[d];
 }
 
 // Stage 4: greedy face placement in FP64 (maybePlaceFace +
