@@ -162,19 +162,27 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
             inf.set_guidance(pilot_samples=pilot, pilot_factor=10.0, surrogate=model,
                              keep_particles=keep, line_points=4,
                              plain_fallback=mode == "guided_fallback")
-        if mode == "guided_untrained":
+        if mode in ("guided_pruned", "guided_pruned_graph"):
+            inf.set_guidance(pilot_samples=pilot, pilot_factor=10.0, surrogate=model,
+                             keep_particles=keep, line_points=4, prune_below=PRUNE_BELOW)
+            if mode == "guided_pruned_graph":
+                inf.set_graph(1)
+        if mode in ("guided_untrained", "guided_pruned_untrained"):
             from csdecomp_amd import capi
             lo = np.asarray(plant.getPositionLowerLimits(), np.float32)
             hi = np.asarray(plant.getPositionUpperLimits(), np.float32)
             inf.set_guidance(pilot_samples=pilot, pilot_factor=10.0,
                              surrogate=capi.Mlp(len(lo), lo, hi, 7), keep_particles=keep,
-                             line_points=4)
+                             line_points=4,
+                             prune_below=PRUNE_BELOW if mode == "guided_pruned_untrained" else None)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         regions, infos = inf.inflateEdges(starts, ends, None, 0.0, budgets)
         torch.cuda.synchronize()
         infl = time.perf_counter() - t0
-        el = infl + (train_s if mode in ("guided", "guided_graph", "guided_fallback") else 0.0)
+        trained = mode in ("guided", "guided_graph", "guided_fallback", "guided_pruned",
+                           "guided_pruned_graph")
+        el = infl + (train_s if trained else 0.0)
         conv = np.array([i.converged for i in infos])
         faces[mode] = [(r.A(), r.b()) for r in regions]
         flops = sum(eizo_region_flops(i, starts.shape[0], opt.mixing_steps, opt.bisection_steps,
@@ -194,10 +202,16 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
                                   "f_cfg": f_cfg}}
         if mode == "guided_fallback":
             out[mode]["fallback_regions"] = int(sum(i.plain_fallback for i in infos))
+        rounds = [x for i in infos for x in i.iteration_info]
+        pilot_n = sum(x.num_pilot_samples for x in rounds)
+        if pilot_n:
+            out[mode]["pilot_checked_fraction"] = round(
+                sum(x.num_pilot_checked for x in rounds) / pilot_n, 4)
         print(f"eizo {name} R={R} faces<={face_budget} {mode}: {out[mode]['regions_per_s']} "
               f"regions/s, certified {out[mode]['certified_fraction']}", file=sys.stderr,
               flush=True)
-    for host, dev in (("plain", "plain_graph"), ("guided", "guided_graph")):
+    for host, dev in (("plain", "plain_graph"), ("guided", "guided_graph"),
+                      ("guided_pruned", "guided_pruned_graph")):
         if host in faces and dev in faces:
             out[dev + "_identical"] = all(
                 np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1])
@@ -212,10 +226,21 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
         out["trained_vs_untrained_inflation_regions_per_s"] = round(
             g["inflation_regions_per_s"] / u["inflation_regions_per_s"], 3)
         out["trained_vs_untrained_samples"] = round(g["samples"] / u["samples"], 3)
+    if "guided_pruned" in out and "guided_pruned_untrained" in out:
+        g, u = out["guided_pruned"], out["guided_pruned_untrained"]
+        out["pruned_trained_vs_untrained_inflation_regions_per_s"] = round(
+            g["inflation_regions_per_s"] / u["inflation_regions_per_s"], 3)
+    if "plain" in out and "guided_pruned" in out:
+        out["pruned_vs_plain_certified_per_s"] = round(
+            out["guided_pruned"]["certified_per_s"] / max(out["plain"]["certified_per_s"], 1e-9), 3)
     return out
 
 
 MLP_FLOPS_PER_ROW = 2 * (256 * 7 + 3 * 256 * 256 + 256)  # 7 inputs, 4 hidden layers, head
+# pruned pilots: samples the surrogate predicts above this value skip the
+# exact check (its output is the signed distance clipped to +-0.2 m and
+# scaled to +-1, so 0.1 is 2 cm clear)
+PRUNE_BELOW = 0.1
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense
 
 
@@ -565,7 +590,8 @@ def eizo_benchmarks(segments=2048, cpu=True):
     arm = scenes.arm_scene(1001, 48, 16)
     s1, e1 = scenes.free_segments(arm, 256, 3001, (0.2, 1.5))
     m7, tr7 = surrogate.train_surrogate(arm, s1, e1)
-    modes = ("plain", "plain_graph", "guided", "guided_graph", "guided_untrained", "guided_fallback")
+    modes = ("plain", "plain_graph", "guided", "guided_graph", "guided_untrained", "guided_fallback",
+             "guided_pruned", "guided_pruned_graph", "guided_pruned_untrained")
     res["single_edge_arm7"] = eizo_leg("single", arm, s1, e1, [], 4096, modes, m7,
                                        train_s=tr7["label_s"] + tr7["train_s"])
     res["single_edge_arm7"]["surrogate"] = tr7
