@@ -154,8 +154,15 @@ def eizo_leg(name, plant, starts, ends, budgets, particles, modes, model, face_b
     f_cfg = algorithmic_flops_per_config(mp, plant)
     out = {"segments": R, "dof": int(starts.shape[0])}
     faces = {}
+    wide_from = None
     for mode in modes:
         inf = csd.CudaEdgeInflator(mp, list(plant.getRobotGeometryIds()), opt, dom, R, 10, nd + face_budget)
+        # sampler tile: timed once per leg on synthetic polytopes (outside the timed region)
+        if wide_from is None:
+            wide_from = inf.tune_sampler()
+            out["sampler_wide_from_faces"] = wide_from
+        else:
+            inf.set_sampler_tile(wide_from)
         if mode in ("plain_graph", "guided_graph"):
             inf.set_graph(1)  # one graph-captured round per launch, status read each round
         if mode in ("guided", "guided_graph", "guided_fallback"):

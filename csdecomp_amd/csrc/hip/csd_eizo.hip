@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 #include "csd_eizo_device.h"
@@ -33,13 +34,15 @@ namespace csd {
 // unchecked into the block's staging segment as [dim][256] (coalesced).
 // Kept apart from the collision check so that the chains run at the high
 // occupancy their small register footprint allows.
-template <int DPL, bool BROW = false>
-__global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim, int cap,
-                                                              int nblk, int mixing, uint32_t k0,
-                                                              uint32_t k1, uint32_t round,
-                                                              uint32_t stream_off) {
+template <int DPL, bool BROW = false, int BLK = kSampleBlock>
+__global__ __launch_bounds__(BLK) void k_eizo_sample(EizoDev D, int dim, int cap, int nblk,
+                                                     int mixing, uint32_t k0, uint32_t k1,
+                                                     uint32_t round, uint32_t stream_off) {
   constexpr int DP = SamplerShape<DPL>::DP;
   constexpr bool LDS = SamplerShape<DPL>::LDS;
+  // a block of BLK threads fills BLK / 256 consecutive staging segments and
+  // shares one LDS copy of the polytope among them
+  constexpr int kSegs = BLK / kSampleBlock;
   extern __shared__ float csd_smem[];
   const int slot = blockIdx.y;
   if (D.ctl) {  // device-resident round: slots past the active count idle
@@ -48,7 +51,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
   }
   const int r = D.active[slot];
   const uint32_t N = D.nsamp[r];
-  if (blockIdx.x * kSampleBlock >= N) return;  // whole block past this region's samples
+  if (blockIdx.x * BLK >= N) return;  // whole block past this region's samples
   const int F = D.nf[r];
   const float *A = D.A + (size_t)r * cap * dim, *b = D.b + (size_t)r * cap;
   if constexpr (LDS) {
@@ -57,7 +60,12 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
     A = csd_smem;
     b = csd_smem + (size_t)F * DP;
   }
-  const uint32_t i = blockIdx.x * kSampleBlock + threadIdx.x;
+  const int sblk = (int)blockIdx.x * kSegs + (int)threadIdx.x / kSampleBlock;
+  const int lane = (int)threadIdx.x % kSampleBlock;
+  // a segment past the round's staging or this region's samples: nothing
+  // to draw (the same chains run whatever the block size)
+  if (sblk >= nblk || (uint32_t)sblk * kSampleBlock >= N) return;
+  const uint32_t i = (uint32_t)sblk * kSampleBlock + lane;
   // chain i starts at fraction (i + 1/2) / N of the segment, which lies in
   // the region: spread starts cover elongated regions better than starts at
   // line_start only
@@ -71,10 +79,87 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_sample(EizoDev D, int dim
   }
   hit_and_run<DP, LDS, BROW>(A, b, F, dim, x, i, (uint32_t)r + stream_off, round, mixing, k0,
                              k1);
-  float *seg = D.stage + ((size_t)slot * nblk + blockIdx.x) * kSampleBlock * dim;
+  float *seg = D.stage + ((size_t)slot * nblk + sblk) * kSampleBlock * dim;
 #pragma unroll
   for (int d = 0; d < DP; ++d)
-    if (d < dim) seg[(size_t)d * kSampleBlock + threadIdx.x] = x[d];
+    if (d < dim) seg[(size_t)d * kSampleBlock + lane] = x[d];
+}
+
+// Sampler tile for polytopes of up to `faces` faces: staged in LDS with
+// rows padded to 8 or 16 floats (b in the padding slot when dim < DP) by
+// 256-thread blocks up to 64 KB, or by 1024-thread blocks -- 16 waves
+// sharing one copy -- up to the 160 KB of a gfx950 workgroup; larger
+// polytopes are read from global memory. 1024-thread blocks are taken from
+// `wide_from` faces on (csd_eizo_tune_sampler measures it) and whenever
+// only they fit.
+struct SamplerLaunch {
+  int dpl = 32;  // SamplerShape: 2 DP + LDS
+  bool brow = false;
+  int blk = kSampleBlock;
+  size_t lds = 0;
+};
+
+SamplerLaunch pick_sampler(int dim, int faces, int wide_from) {
+  SamplerLaunch L;
+  for (int dp : {8, 16}) {
+    if (dim > dp) continue;
+    const size_t bytes = dp == 8 ? polytope_lds_bytes<8>(faces) : polytope_lds_bytes<16>(faces);
+    const bool narrow = bytes <= kLdsLimit, wide = bytes <= kLdsLimitWide;
+    if (!narrow && !wide) continue;
+    L.dpl = 2 * dp + 1;
+    L.brow = dim < dp;
+    L.blk = narrow && faces < wide_from ? kSampleBlock : kSampleBlockWide;
+    L.lds = bytes;
+    return L;
+  }
+  return L;  // global memory, 256-thread blocks
+}
+
+// k_eizo_sample for `slots` regions and `nblk` staging segments each
+void launch_sampler(const SamplerLaunch &L, int nblk, int slots, hipStream_t s,
+                    const EizoDev &E, int dim, int cap, int mixing, uint32_t k0, uint32_t k1,
+                    uint32_t round, uint32_t off) {
+  const int segs = L.blk / kSampleBlock;
+  const dim3 g((nblk + segs - 1) / segs, slots), bk(L.blk);
+#define CSD_SAMPLE(DPL, BR, BLK)                                                              \
+  hipLaunchKernelGGL((k_eizo_sample<DPL, BR, BLK>), g, bk, L.lds, s, E, dim, cap, nblk, mixing, \
+                     k0, k1, round, off)
+  if (L.dpl == 32) CSD_SAMPLE(32, false, kSampleBlock);
+  else if (L.blk == kSampleBlock) {
+    if (L.dpl == 17) {
+      if (L.brow) CSD_SAMPLE(17, true, kSampleBlock);
+      else CSD_SAMPLE(17, false, kSampleBlock);
+    } else {
+      if (L.brow) CSD_SAMPLE(33, true, kSampleBlock);
+      else CSD_SAMPLE(33, false, kSampleBlock);
+    }
+  } else {
+    if (L.dpl == 17) {
+      if (L.brow) CSD_SAMPLE(17, true, kSampleBlockWide);
+      else CSD_SAMPLE(17, false, kSampleBlockWide);
+    } else {
+      if (L.brow) CSD_SAMPLE(33, true, kSampleBlockWide);
+      else CSD_SAMPLE(33, false, kSampleBlockWide);
+    }
+  }
+#undef CSD_SAMPLE
+}
+
+// Allows the 1024-thread sampler tiles their dynamic LDS above 64 KB.
+int allow_wide_sampler_lds() {
+  static const int rc = [] {
+    const void *fns[] = {
+        reinterpret_cast<const void *>(&k_eizo_sample<17, true, kSampleBlockWide>),
+        reinterpret_cast<const void *>(&k_eizo_sample<17, false, kSampleBlockWide>),
+        reinterpret_cast<const void *>(&k_eizo_sample<33, true, kSampleBlockWide>),
+        reinterpret_cast<const void *>(&k_eizo_sample<33, false, kSampleBlockWide>)};
+    for (const void *f : fns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kLdsLimitWide) != hipSuccess)
+        return (int)CSD_ERR_HIP;
+    return (int)CSD_OK;
+  }();
+  return rc;
 }
 
 // Round stage 1b: collision check of the block's samples and compaction of
@@ -485,6 +570,9 @@ struct csd_eizo {
   std::vector<float> round_ms;
   // csd_eizo_set_metric
   double *metric_buf = nullptr;
+  // sampler tile: 1024-thread blocks from this many faces on (when 256-thread
+  // blocks also fit); csd_eizo_tune_sampler / csd_eizo_set_sampler_tile
+  int wide_from = std::numeric_limits<int>::max();
 };
 
 using namespace csd;
@@ -558,6 +646,10 @@ int csd_eizo_create(const csd_plant *plant, const int32_t *robot_geoms, int32_t 
   w->U = max_particles;
   w->Vmax = max_voxels;
   w->dim = dim;
+  if (allow_wide_sampler_lds() != CSD_OK) {
+    delete w;
+    return fail(CSD_ERR_HIP, "hipFuncSetAttribute (sampler LDS)");
+  }
   const size_t R = max_regions, C = face_capacity, U = max_particles;
   int rc = CSD_OK;
   EizoDev &D = w->D;
@@ -641,6 +733,118 @@ int csd_eizo_set_pruning(csd_eizo *w, int32_t enable, float predicted_below) {
     return fail(CSD_ERR_INVALID_ARGUMENT, "the pruning threshold must be finite");
   w->prune = enable != 0;
   w->prune_below = predicted_below;
+  return CSD_OK;
+}
+
+int csd_eizo_set_sampler_tile(csd_eizo *w, int32_t wide_from) {
+  if (!w) return fail(CSD_ERR_INVALID_ARGUMENT, "null workspace");
+  if (wide_from < 0) return fail(CSD_ERR_INVALID_ARGUMENT, "negative face count");
+  w->wide_from = wide_from;
+  return CSD_OK;
+}
+
+int csd_eizo_tune_sampler(csd_eizo *w, int32_t *wide_from) {
+  if (!w) return fail(CSD_ERR_INVALID_ARGUMENT, "null workspace");
+  const int dim = w->dim, cap = w->cap;
+  EizoDev &D = w->D;
+  // synthetic regions that fill the GPU: Rt slots of N samples, short chains
+  // (the time per step and face is what the tiles differ in)
+  const int Rt = std::min(w->R, 64);
+  const uint32_t N = 16384;
+  const int nblk = (int)(N / kSampleBlock), steps = 8;
+  int rc = grow_stage(w, Rt, nblk);
+  if (rc) return rc;
+  // face-count classes where both tiles fit: up to the largest polytope a
+  // 256-thread block stages
+  const int dp = dim <= 8 ? 8 : 16;
+  const int narrow_max = (int)(kLdsLimit / ((dp + 1) * sizeof(float)));
+  std::vector<int> classes;
+  for (int f : {2 * dim + 32, 256, 512, narrow_max})
+    if (f <= std::min(cap, narrow_max) && (classes.empty() || f > classes.back()))
+      classes.push_back(f);
+  // polytope: the box |x_i| <= 1 and unit normals n with b = 1 + u, u in
+  // [0, 1): the unit ball is inside; segment along x_0 through the origin
+  const int fmax = classes.empty() ? 0 : classes.back();
+  std::vector<float> A((size_t)Rt * cap * dim, 0.f), b((size_t)Rt * cap, 0.f), ls(Rt * dim, 0.f),
+      le(Rt * dim, 0.f);
+  uint64_t seed = 0x5eed;
+  auto uni = [&seed] {
+    seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+    return (float)((seed >> 40) & 0xffffff) / 16777216.f;
+  };
+  for (int r = 0; r < Rt; ++r) {
+    float *Ar = &A[(size_t)r * cap * dim], *br = &b[(size_t)r * cap];
+    for (int f = 0; f < fmax; ++f) {
+      float *a = Ar + (size_t)f * dim;
+      if (f < 2 * dim) {
+        a[f / 2] = (f & 1) ? 1.f : -1.f;
+        br[f] = 1.f;
+        continue;
+      }
+      float nn = 0.f;
+      for (int d = 0; d < dim; ++d) {
+        a[d] = uni() - 0.5f;
+        nn += a[d] * a[d];
+      }
+      nn = nn > 0.f ? 1.f / sqrtf(nn) : 0.f;
+      for (int d = 0; d < dim; ++d) a[d] *= nn;
+      br[f] = 1.f + uni();
+    }
+    ls[r * dim] = -0.1f;
+    le[r * dim] = 0.1f;
+  }
+  std::vector<int32_t> act(Rt);
+  std::vector<uint32_t> ns(w->R, N);
+  for (int r = 0; r < Rt; ++r) act[r] = r;
+  CSD_HIP(hipMemcpy(D.A, A.data(), A.size() * 4, hipMemcpyHostToDevice));
+  CSD_HIP(hipMemcpy(D.b, b.data(), b.size() * 4, hipMemcpyHostToDevice));
+  CSD_HIP(hipMemcpy(D.ls, ls.data(), ls.size() * 4, hipMemcpyHostToDevice));
+  CSD_HIP(hipMemcpy(D.le, le.data(), le.size() * 4, hipMemcpyHostToDevice));
+  CSD_HIP(hipMemcpy(D.active, act.data(), Rt * 4, hipMemcpyHostToDevice));
+  CSD_HIP(hipMemcpy(D.nsamp, ns.data(), (size_t)w->R * 4, hipMemcpyHostToDevice));
+  EizoDev E = D;
+  E.ctl = nullptr;
+  hipEvent_t e0, e1;
+  CSD_HIP(hipEventCreate(&e0));
+  CSD_HIP(hipEventCreate(&e1));
+  int from = std::numeric_limits<int>::max();
+  bool wide_wins_above = true;  // 1024 wins at every larger class tested so far
+  for (int k = (int)classes.size() - 1; k >= 0 && rc == CSD_OK; --k) {
+    const int F = classes[k];
+    std::vector<int32_t> nf(Rt, F);
+    if (hipMemcpy(D.nf, nf.data(), Rt * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = hip_fail(hipGetLastError(), "hipMemcpy (sampler tuning)");
+      break;
+    }
+    float best[2] = {0.f, 0.f};
+    for (int t = 0; t < 2 && rc == CSD_OK; ++t) {
+      const SamplerLaunch L = pick_sampler(dim, F, t == 0 ? std::numeric_limits<int>::max() : 0);
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 4; ++rep) {  // the first launch warms up
+        hipEventRecord(e0, 0);
+        launch_sampler(L, nblk, Rt, 0, E, dim, cap, steps, 0x1234u, 0x5678u, 0u, 0u);
+        hipEventRecord(e1, 0);
+        hipError_t he = hipEventSynchronize(e1);
+        if (he == hipSuccess) he = hipGetLastError();
+        if (he != hipSuccess) {
+          rc = hip_fail(he, "sampler tuning launch");
+          break;
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) ms_min = std::min(ms_min, ms);
+      }
+      best[t] = ms_min;
+    }
+    const bool wide = best[1] < 0.97f * best[0];
+    wide_wins_above = wide_wins_above && wide;
+    if (wide_wins_above) from = F;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  w->wide_from = from;
+  if (wide_from) *wide_from = from;
   return CSD_OK;
 }
 
@@ -778,21 +982,8 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     int nfmax = 0;
     for (int r : slots) nfmax = std::max(nfmax, nfh[r]);
     const dim3 sg(nblk, na), sb(kSampleBlock);
-    if (dim <= 7 && polytope_lds_bytes<8>(nfmax) <= kLdsLimit)
-      hipLaunchKernelGGL((k_eizo_sample<17, true>), sg, sb, polytope_lds_bytes<8>(nfmax), 0, D,
-                         dim, cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
-    else if (dim <= 8 && polytope_lds_bytes<8>(nfmax) <= kLdsLimit)
-      hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, polytope_lds_bytes<8>(nfmax), 0, D, dim,
-                         cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
-    else if (dim <= 15 && polytope_lds_bytes<16>(nfmax) <= kLdsLimit)
-      hipLaunchKernelGGL((k_eizo_sample<33, true>), sg, sb, polytope_lds_bytes<16>(nfmax), 0, D,
-                         dim, cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
-    else if (polytope_lds_bytes<16>(nfmax) <= kLdsLimit)
-      hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, polytope_lds_bytes<16>(nfmax), 0, D, dim,
-                         cap, nblk, (int)o.mixing_steps, k0, k1, it, stream_off);
-    else
-      hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, 0, D, dim, cap, nblk,
-                         (int)o.mixing_steps, k0, k1, it, stream_off);
+    launch_sampler(pick_sampler(dim, nfmax, w->wide_from), nblk, na, 0, D, dim, cap,
+                   (int)o.mixing_steps, k0, k1, it, stream_off);
     const float *pr = nullptr;
     if (prune) {
       const size_t need = (size_t)na * nblk * kSampleBlock * sizeof(float);
@@ -849,12 +1040,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     const float limit = w->pilot_factor * o.epsilon * (float)w->pilot;
     const int L = w->line_points;
     // the sampler stages the largest polytope the workspace holds
-    const int svar = dim <= 8 && polytope_lds_bytes<8>(cap) <= kLdsLimit    ? 17
-                     : polytope_lds_bytes<16>(cap) <= kLdsLimit ? 33
-                                                                : 32;
-    const size_t slds = svar == 17 ? polytope_lds_bytes<8>(cap)
-                        : svar == 33 ? polytope_lds_bytes<16>(cap)
-                                     : 0;
+    const SamplerLaunch sl = pick_sampler(dim, cap, w->wide_from);
     // buffers for every region as a slot, sized outside the capture
     if (ranked) {
       const size_t need = (size_t)Rn * kRankMax * L * sizeof(float);
@@ -877,22 +1063,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
       }
     }
     auto sample = [&](const EizoDev &E, int nb, uint32_t off) {
-      const dim3 sg(nb, Rn), sb(kSampleBlock);
-      if (svar == 17 && dim <= 7)
-        hipLaunchKernelGGL((k_eizo_sample<17, true>), sg, sb, slds, s, E, dim, cap, nb,
-                           (int)o.mixing_steps, k0, k1, 0u, off);
-      else if (svar == 17)
-        hipLaunchKernelGGL((k_eizo_sample<17>), sg, sb, slds, s, E, dim, cap, nb,
-                           (int)o.mixing_steps, k0, k1, 0u, off);
-      else if (svar == 33 && dim <= 15)
-        hipLaunchKernelGGL((k_eizo_sample<33, true>), sg, sb, slds, s, E, dim, cap, nb,
-                           (int)o.mixing_steps, k0, k1, 0u, off);
-      else if (svar == 33)
-        hipLaunchKernelGGL((k_eizo_sample<33>), sg, sb, slds, s, E, dim, cap, nb,
-                           (int)o.mixing_steps, k0, k1, 0u, off);
-      else
-        hipLaunchKernelGGL((k_eizo_sample<32>), sg, sb, 0, s, E, dim, cap, nb,
-                           (int)o.mixing_steps, k0, k1, 0u, off);
+      launch_sampler(sl, nb, Rn, s, E, dim, cap, (int)o.mixing_steps, k0, k1, 0u, off);
     };
     auto check = [&](const EizoDev &E, int nb, const float *pr, float below) {
       const dim3 sg(nb, Rn), sb(kSampleBlock);

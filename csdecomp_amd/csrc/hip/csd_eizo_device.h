@@ -12,6 +12,7 @@ namespace csd {
 
 constexpr int kDimMax = 16;     // configuration variables handled by EI-ZO
 constexpr int kSampleBlock = 256;
+constexpr int kSampleBlockWide = 1024;  // the sampler's wide tile (16 waves)
 constexpr int kFaceBlock = 1024;
 
 // ---------------------------------------------------------------------------
@@ -39,7 +40,8 @@ __device__ __forceinline__ float u01(uint32_t x) {
 // Zero padding adds exact zeros to the dot products (x and dir are 0 past
 // dim), so the arithmetic equals the unpadded loop. Larger polytopes are
 // read from global memory row by row.
-constexpr size_t kLdsLimit = 65536;
+constexpr size_t kLdsLimit = 65536;       // 256-thread sampler blocks
+constexpr size_t kLdsLimitWide = 163840;  // 1024-thread blocks: a gfx950 workgroup's LDS
 
 template <int DP>
 __host__ __device__ constexpr size_t polytope_lds_bytes(int faces) {

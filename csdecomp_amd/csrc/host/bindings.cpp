@@ -497,6 +497,13 @@ PYBIND11_MODULE(_pycsdecomp, m) {
           "ellipsoid metric of the face placement: a symmetric positive definite dim x dim "
           "matrix (faces tangent to M-ellipsoids around the segment), None for the Euclidean "
           "metric of the reference")
+      .def("set_sampler_tile", &CudaEdgeInflator::setSamplerTile, py::arg("wide_from"),
+           "hit-and-run in 1024-thread workgroups (16 waves sharing the polytope in LDS) from "
+           "wide_from faces on when 256-thread workgroups also fit (0: always; 2**31 - 1: only "
+           "above their 64 KB); the samples are the same either way")
+      .def("tune_sampler", &CudaEdgeInflator::tuneSampler,
+           "time both sampler tiles on synthetic polytopes and keep the faster per face count; "
+           "returns the face count from which 1024-thread workgroups are used")
       .def("set_graph", &CudaEdgeInflator::setGraph, py::arg("rounds_per_sync") = 1,
            "replay one graph-captured round per launch; the host reads the region status "
            "every rounds_per_sync rounds (0: host-driven rounds)")

@@ -883,6 +883,16 @@ void CudaEdgeInflator::setMetric(const MatrixXf &metric) {
   check_status(csd_eizo_set_metric(ws_, rm.data()), "EI-ZO metric");
 }
 
+void CudaEdgeInflator::setSamplerTile(int wide_from) {
+  check_status(csd_eizo_set_sampler_tile(ws_, wide_from), "EI-ZO sampler tile");
+}
+
+int CudaEdgeInflator::tuneSampler() {
+  int32_t from = 0;
+  check_status(csd_eizo_tune_sampler(ws_, &from), "EI-ZO sampler tuning");
+  return from;
+}
+
 void CudaEdgeInflator::setGraph(int rounds_per_sync) {
   check_status(csd_eizo_set_graph(ws_, rounds_per_sync), "EI-ZO device-resident rounds");
 }

@@ -375,6 +375,11 @@ class CudaEdgeInflator {
   // symmetric positive definite dim x dim matrix, or an empty one for the
   // Euclidean metric of the reference.
   void setMetric(const MatrixXf &metric);
+  // Sampler tile (csd_eizo_set_sampler_tile / csd_eizo_tune_sampler):
+  // 1024-thread workgroups from this many faces on; tuneSampler() times both
+  // tiles and returns the count it picked. Same polytopes either way.
+  void setSamplerTile(int wide_from);
+  int tuneSampler();
   const EizoInfo &lastInfo() const { return last_info_; }
   // region r of the last inflateEdge(s) call, in its last round
   EizoParticles lastParticles(int region) const;

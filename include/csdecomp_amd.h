@@ -335,6 +335,24 @@ int csd_eizo_set_pruning(csd_eizo *ws, int32_t enable, float predicted_below);
  * if the matrix is not symmetric positive definite. */
 int csd_eizo_set_metric(csd_eizo *ws, const float *metric);
 
+/* Sampler tile of later csd_eizo_run calls (no counterpart in the
+ * reference, whose uniformSampleInHPolyhedronKernel re-reads the polytope
+ * from global memory in every thread, cuda_hit_and_run_sampling.cu:17-112).
+ * One lane runs one hit-and-run chain; a workgroup stages its region's
+ * polytope in LDS once. 256-thread workgroups stage up to 64 KB, 1024-thread
+ * workgroups (16 waves sharing one copy) up to 160 KB; larger polytopes are
+ * read from global memory. The samples are the same bit for bit whatever
+ * the tile. csd_eizo_set_sampler_tile: 1024-thread workgroups from
+ * wide_from faces on when both tiles fit (0: always; the default,
+ * INT32_MAX: only when the polytope exceeds 64 KB).
+ * csd_eizo_tune_sampler: times both tiles on synthetic polytopes of this
+ * workspace's dimension at several face counts and sets wide_from to the
+ * smallest count from which the wide tile is faster at every larger count
+ * tested (returned in *wide_from, may be NULL). Synchronous; it overwrites
+ * the workspace's region state, so call it before csd_eizo_run. */
+int csd_eizo_set_sampler_tile(csd_eizo *ws, int32_t wide_from);
+int csd_eizo_tune_sampler(csd_eizo *ws, int32_t *wide_from);
+
 /* Device-resident rounds for later csd_eizo_run calls (no counterpart in the
  * reference, whose executeEizo launches every stage from the host and reads
  * counters back between them, cuda_edge_inflation_zero_order.cu:109-306).

@@ -547,6 +547,59 @@ def test_graph_rounds_match_host_rounds(gpu, csd, scene):
     assert max(i.iterations for i in runs[0][1]) >= 2  # several rounds ran
 
 
+@pytest.mark.parametrize("scene", ["arm7", "dual14"])
+def test_sampler_tiles_give_the_same_regions(gpu, csd, scene):
+    """The hit-and-run sampler's tiles draw the same samples, so the regions
+    match bit for bit: 256-thread workgroups (the default), 1024-thread
+    workgroups (set_sampler_tile(0)), the tuned choice, and graph-captured
+    rounds, which stage LDS for the whole face capacity: above 64 KB only
+    the 1024-thread tile fits (2,000 faces at 7 dims, 1,100 at 14), above
+    160 KB the polytope is read from global memory (4,600 and 2,500)."""
+    from csdecomp_amd import scenes
+    if scene == "arm7":
+        pl, R, caps = scenes.arm_scene(1001, 48, 16), 4, (200, 2000, 4600)
+    else:
+        pl, R, caps = scenes.dual_arm_scene(1002, 72, 24), 3, (200, 1100, 2500)
+    mp = pl.getMinimalPlant()
+    lo, hi = pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+    ls, le = scenes.free_segments(pl, R, 3200, (0.1, 0.6))
+    dom = csd.HPolyhedron()
+    dom.MakeBox(lo, hi)
+    opt = csd.EizoOptions()
+    opt.epsilon = 0.01
+    opt.num_particles = 1000
+    opt.bisection_steps = 9
+    opt.max_iterations = 8
+    opt.verbose = False
+    budgets = [400, 1000, 700, 900][:R]
+    ids = list(pl.getRobotGeometryIds())
+
+    def run(inf, graph):
+        inf.set_graph(graph)
+        regions, infos = inf.inflateEdges(ls, le, None, 0.0, budgets)
+        return [(r.A(), r.b()) for r in regions], [(i.iterations, i.samples_drawn) for i in infos]
+
+    def same(a, b, what):
+        for r, ((A1, b1), (A2, b2)) in enumerate(zip(a[0], b[0])):
+            np.testing.assert_array_equal(A1, A2, err_msg=f"{what} region {r}")
+            np.testing.assert_array_equal(b1, b2, err_msg=f"{what} region {r}")
+        assert a[1] == b[1], what
+
+    for cap in caps:
+        inf = csd.CudaEdgeInflator(mp, ids, opt, dom, R, 10, cap)
+        ref = run(inf, 0)  # host rounds: the tile of each round's largest polytope
+        inf.set_sampler_tile(0)
+        same(run(inf, 0), ref, f"cap {cap}: 1024-thread tile")
+        same(run(inf, 1), ref, f"cap {cap}: graph rounds, 1024-thread tile where it fits")
+        inf.set_sampler_tile(2 ** 31 - 1)
+        same(run(inf, 1), ref, f"cap {cap}: graph rounds, default tile")
+        if cap == caps[0]:
+            wide_from = inf.tune_sampler()
+            assert wide_from >= 0
+            same(run(inf, 0), ref, f"tuned (1024-thread tile from {wide_from} faces)")
+        assert max(i for i, _ in ref[1]) >= 2  # several rounds ran
+
+
 @pytest.mark.parametrize("mode", ["pilot", "ranked", "pruned", "ranked_no_pilot", "dual_pruned"])
 def test_guided_graph_rounds_match_host_rounds(gpu, csd, mode):
     """Guided rounds device-resident: the pilot's decision, the pruned
