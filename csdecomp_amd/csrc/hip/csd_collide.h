@@ -339,6 +339,191 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   return col;
 }
 
+// 64 sphere slots (tables in LDS) checked in two halves of the bodies, so
+// that only half of the sphere centres are live at a time: 96 registers
+// instead of 192, which lets two waves share a SIMD. Half 0 is bodies
+// [0, NB/2): its centres, its self pairs, then its obstacles and voxels.
+// Half 1 continues the chain FK into bodies [NB/2, NB): its centres and self
+// pairs; then the bodies of half 0 are walked again one at a time (the same
+// FK arithmetic, so the same centres) for the pairs across the halves; then
+// half 1's obstacles and voxels. Every test is collides_fast's own, so the
+// verdict is the same; only the order of the tests differs. (sub, nsub) and
+// PH as in collides_fast.
+template <int NB, int KM, bool ANY = false, int PH = 3, class Q>
+__device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const Q &qa,
+                                                     const VoxelArgs &V, bool col,
+                                                     const float *tab, int sub = 0,
+                                                     int nsub = 1) {
+  constexpr int NH = NB / 2, NSH = NH * KM, NS = NB * KM;
+  static_assert(NS == 64, "the LDS-table plants");
+  col = col || P.always_colliding;
+  const float *orow = tab + NB * NB * KM * KM;
+  int rr = 0;  // the part that takes the next body-pair block
+  auto mine = [&]() {
+    const bool m = rr == sub;
+    if (++rr == nsub) rr = 0;
+    return m;
+  };
+  float X[12];  // the chain's current body frame
+  aff_identity(X);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int b0 = h * NH;
+    float cx[NSH], cy[NSH], cz[NSH];
+#pragma unroll
+    for (int bl = 0; bl < NH; ++bl) {
+      const int b = b0 + bl;
+      if (b < P.num_bodies) {
+        const DevBody &bd = P.bodies[b];
+        float M[12];
+        joint_local(bd, P.q_identity ? qa(b) : qa(bd.q_idx), M);
+        if (bd.parent < 0) {
+#pragma unroll
+          for (int k = 0; k < 12; ++k) X[k] = M[k];
+        } else {
+          float Y[12];
+          aff_mul(X, M, Y);
+#pragma unroll
+          for (int k = 0; k < 12; ++k) X[k] = Y[k];
+        }
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const float4 sp = P.slot_sphere[b * KM + k];
+          aff_point(X, sp.x, sp.y, sp.z, cx[bl * KM + k], cy[bl * KM + k], cz[bl * KM + k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < KM; ++k) cx[bl * KM + k] = cy[bl * KM + k] = cz[bl * KM + k] = 0.f;
+      }
+    }
+    // self pairs inside this half
+    if constexpr (PH & 1) {
+#pragma unroll
+      for (int bla = 0; bla < NH; ++bla) {
+        const int ba = b0 + bla;
+        const uint64_t act = P.self_active[ba];
+        if (act == 0 || wave_done<ANY>(col)) continue;
+#pragma unroll
+        for (int blb = bla + 1; blb < NH; ++blb) {
+          const int bb = b0 + blb;
+          if (!((act >> bb) & 1) || !mine()) continue;
+          const float *r2 = tab + ((size_t)ba * NB + bb) * (KM * KM);
+#pragma unroll
+          for (int ka = 0; ka < KM; ++ka)
+#pragma unroll
+            for (int kb = 0; kb < KM; ++kb) {
+              const int sa = bla * KM + ka, sb = blb * KM + kb;
+              col |= hit_sphere_sphere(cx[sa], cy[sa], cz[sa], cx[sb], cy[sb], cz[sb],
+                                       r2[ka * KM + kb]);
+            }
+        }
+      }
+      // pairs across the halves: bodies of half 0 again, one at a time
+      if (h == 1) {
+        float W[12];
+        aff_identity(W);
+#pragma unroll
+        for (int ba = 0; ba < NH; ++ba) {
+          if (ba >= P.num_bodies) continue;
+          const DevBody &bd = P.bodies[ba];
+          float M[12];
+          joint_local(bd, P.q_identity ? qa(ba) : qa(bd.q_idx), M);
+          if (bd.parent < 0) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) W[k] = M[k];
+          } else {
+            float Y[12];
+            aff_mul(W, M, Y);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) W[k] = Y[k];
+          }
+          const uint64_t act = P.self_active[ba] >> NH;
+          if (act == 0 || wave_done<ANY>(col)) continue;
+          float ax[KM], ay[KM], az[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const float4 sp = P.slot_sphere[ba * KM + k];
+            aff_point(W, sp.x, sp.y, sp.z, ax[k], ay[k], az[k]);
+          }
+#pragma unroll
+          for (int blb = 0; blb < NH; ++blb) {
+            if (!((act >> blb) & 1) || !mine()) continue;
+            const float *r2 = tab + ((size_t)ba * NB + NH + blb) * (KM * KM);
+#pragma unroll
+            for (int ka = 0; ka < KM; ++ka)
+#pragma unroll
+              for (int kb = 0; kb < KM; ++kb) {
+                const int sb = blb * KM + kb;
+                col |= hit_sphere_sphere(ax[ka], ay[ka], az[ka], cx[sb], cy[sb], cz[sb],
+                                         r2[ka * KM + kb]);
+              }
+          }
+        }
+      }
+    }
+    // this half's spheres x world-fixed obstacles: the record in scalar
+    // registers, each body's row of contact distances from LDS
+    if constexpr (PH & 2) {
+      const int o0 = sub >= rr ? sub - rr : sub - rr + nsub;
+      for (int o = o0; o < P.num_obstacles; o += nsub) {
+        if (wave_done<ANY>(col)) break;
+        const int t = P.obstacle_type[o];
+        float ov[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+        const float *r2 = orow + (size_t)o * NS;
+#pragma unroll
+        for (int bl = 0; bl < NH; ++bl) {
+          const int b = b0 + bl;
+          if (b >= P.num_bodies) continue;
+          float rb[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k) rb[k] = r2[b * KM + k];
+          if (t == CSD_BOX) {
+#pragma unroll
+            for (int k = 0; k < KM; ++k) {
+              const int s = bl * KM + k;
+              col |= hit_sphere_obox(cx[s], cy[s], cz[s], rb[k], ov);
+            }
+          } else if (t == CSD_SPHERE) {
+#pragma unroll
+            for (int k = 0; k < KM; ++k) {
+              const int s = bl * KM + k;
+              col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], rb[k]);
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < KM; ++k) {
+              const int s = bl * KM + k;
+              col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                                  ov[6]) < rb[k];
+            }
+          }
+        }
+      }
+      // this half's spheres x voxel spheres
+      if (V.num > 0) {
+        float vr2[NSH];
+#pragma unroll
+        for (int sl = 0; sl < NSH; ++sl) {
+          const int s = b0 * KM + sl;
+          const bool on = s < 64 ? ((V.mask0 >> s) & 1) : ((V.mask1 >> (s - 64)) & 1);
+          const float rs = V.radius + P.slot_sphere[s].w;
+          vr2[sl] = on ? rs * rs : -1.f;
+        }
+        for (int v = sub; v < V.num; v += nsub) {
+          if (wave_done<ANY>(col)) break;
+          const float vx = V.vox[3 * v], vy = V.vox[3 * v + 1], vz = V.vox[3 * v + 2];
+#pragma unroll
+          for (int sl = 0; sl < NSH; ++sl)
+            col |= hit_sphere_sphere(vx, vy, vz, cx[sl], cy[sl], cz[sl], vr2[sl]);
+        }
+      }
+    }
+  }
+  return col;
+}
+
 // Smallest signed distance over the plant's collision pairs in FP32 (fast
 // path plants): sphere-sphere |c_a - c_b| - r_a - r_b, sphere-box distance of
 // the centre to the box (negative inside) - r, sphere-capsule distance to the
@@ -506,8 +691,8 @@ __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &q
                                                  bool col, const float *tab, int sub = 0,
                                                  int nsub = 1) {
   if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
-  else if constexpr (VAR == kFast8x8) return collides_fast<8, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
-  else if constexpr (VAR == kFast16x4) return collides_fast<16, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast8x8) return collides_fast_halves<8, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast16x4) return collides_fast_halves<16, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if (sub != 0) return col;
   else if constexpr (VAR == kGeneric8) return collides_generic<8, ANY>(P, qa, V, R, col);
