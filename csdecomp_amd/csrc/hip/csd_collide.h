@@ -339,23 +339,22 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   return col;
 }
 
-// 64 sphere slots (tables in LDS) checked in two halves of the bodies, so
-// that only half of the sphere centres are live at a time: 96 registers
-// instead of 192, which lets two waves share a SIMD. Half 0 is bodies
-// [0, NB/2): its centres, its self pairs, then its obstacles and voxels.
-// Half 1 continues the chain FK into bodies [NB/2, NB): its centres and self
-// pairs; then the bodies of half 0 are walked again one at a time (the same
-// FK arithmetic, so the same centres) for the pairs across the halves; then
-// half 1's obstacles and voxels. Every test is collides_fast's own, so the
-// verdict is the same; only the order of the tests differs. (sub, nsub) and
-// PH as in collides_fast.
-template <int NB, int KM, bool ANY = false, int PH = 3, class Q>
-__device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const Q &qa,
+// 64 sphere slots (tables in LDS) checked in G groups of NB / G bodies, so
+// that only one group's sphere centres are live at a time (192 / G
+// registers instead of 192). Group g continues the chain FK into its bodies
+// [g NB/G, (g+1) NB/G): their centres and the self pairs among them; then
+// the bodies of the earlier groups are walked again one at a time (the same
+// FK arithmetic, so the same centres) for the pairs between them and group
+// g; then group g's obstacles and voxels. Every test is collides_fast's own,
+// so the verdict is the same; only the order of the tests differs. (sub,
+// nsub) and PH as in collides_fast.
+template <int NB, int KM, int G, bool ANY = false, int PH = 3, class Q>
+__device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const Q &qa,
                                                      const VoxelArgs &V, bool col,
                                                      const float *tab, int sub = 0,
                                                      int nsub = 1) {
-  constexpr int NH = NB / 2, NSH = NH * KM, NS = NB * KM;
-  static_assert(NS == 64, "the LDS-table plants");
+  constexpr int KG = NB / G, NSG = KG * KM, NS = NB * KM;
+  static_assert(NS == 64 && KG * G == NB, "the LDS-table plants, whole groups");
   col = col || P.always_colliding;
   const float *orow = tab + NB * NB * KM * KM;
   int rr = 0;  // the part that takes the next body-pair block
@@ -367,11 +366,11 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
   float X[12];  // the chain's current body frame
   aff_identity(X);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int b0 = h * NH;
-    float cx[NSH], cy[NSH], cz[NSH];
+  for (int g = 0; g < G; ++g) {
+    const int b0 = g * KG;
+    float cx[NSG], cy[NSG], cz[NSG];
 #pragma unroll
-    for (int bl = 0; bl < NH; ++bl) {
+    for (int bl = 0; bl < KG; ++bl) {
       const int b = b0 + bl;
       if (b < P.num_bodies) {
         const DevBody &bd = P.bodies[b];
@@ -396,17 +395,18 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
         for (int k = 0; k < KM; ++k) cx[bl * KM + k] = cy[bl * KM + k] = cz[bl * KM + k] = 0.f;
       }
     }
-    // self pairs inside this half
     if constexpr (PH & 1) {
+      // self pairs inside this group
 #pragma unroll
-      for (int bla = 0; bla < NH; ++bla) {
+      for (int bla = 0; bla < KG; ++bla) {
         const int ba = b0 + bla;
         const uint64_t act = P.self_active[ba];
         if (act == 0 || wave_done<ANY>(col)) continue;
+        // constant trip counts (uniform guards), so that every loop unrolls
 #pragma unroll
-        for (int blb = bla + 1; blb < NH; ++blb) {
+        for (int blb = 0; blb < KG; ++blb) {
           const int bb = b0 + blb;
-          if (!((act >> bb) & 1) || !mine()) continue;
+          if (blb <= bla || !((act >> bb) & 1) || !mine()) continue;
           const float *r2 = tab + ((size_t)ba * NB + bb) * (KM * KM);
 #pragma unroll
           for (int ka = 0; ka < KM; ++ka)
@@ -418,13 +418,14 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
             }
         }
       }
-      // pairs across the halves: bodies of half 0 again, one at a time
-      if (h == 1) {
+      // pairs between the earlier groups and this one: those bodies again,
+      // one at a time
+      if (g > 0) {
         float W[12];
         aff_identity(W);
 #pragma unroll
-        for (int ba = 0; ba < NH; ++ba) {
-          if (ba >= P.num_bodies) continue;
+        for (int ba = 0; ba < NB - KG; ++ba) {
+          if (ba >= b0 || ba >= P.num_bodies) continue;
           const DevBody &bd = P.bodies[ba];
           float M[12];
           joint_local(bd, P.q_identity ? qa(ba) : qa(bd.q_idx), M);
@@ -437,7 +438,7 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
 #pragma unroll
             for (int k = 0; k < 12; ++k) W[k] = Y[k];
           }
-          const uint64_t act = P.self_active[ba] >> NH;
+          const uint64_t act = (P.self_active[ba] >> b0) & ((1ull << KG) - 1);
           if (act == 0 || wave_done<ANY>(col)) continue;
           float ax[KM], ay[KM], az[KM];
 #pragma unroll
@@ -446,9 +447,9 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
             aff_point(W, sp.x, sp.y, sp.z, ax[k], ay[k], az[k]);
           }
 #pragma unroll
-          for (int blb = 0; blb < NH; ++blb) {
+          for (int blb = 0; blb < KG; ++blb) {
             if (!((act >> blb) & 1) || !mine()) continue;
-            const float *r2 = tab + ((size_t)ba * NB + NH + blb) * (KM * KM);
+            const float *r2 = tab + ((size_t)ba * NB + b0 + blb) * (KM * KM);
 #pragma unroll
             for (int ka = 0; ka < KM; ++ka)
 #pragma unroll
@@ -461,7 +462,7 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
         }
       }
     }
-    // this half's spheres x world-fixed obstacles: the record in scalar
+    // this group's spheres x world-fixed obstacles: the record in scalar
     // registers, each body's row of contact distances from LDS
     if constexpr (PH & 2) {
       const int o0 = sub >= rr ? sub - rr : sub - rr + nsub;
@@ -473,7 +474,7 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
         for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
         const float *r2 = orow + (size_t)o * NS;
 #pragma unroll
-        for (int bl = 0; bl < NH; ++bl) {
+        for (int bl = 0; bl < KG; ++bl) {
           const int b = b0 + bl;
           if (b >= P.num_bodies) continue;
           float rb[KM];
@@ -501,13 +502,13 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
           }
         }
       }
-      // this half's spheres x voxel spheres
+      // this group's spheres x voxel spheres
       if (V.num > 0) {
-        float vr2[NSH];
+        float vr2[NSG];
 #pragma unroll
-        for (int sl = 0; sl < NSH; ++sl) {
+        for (int sl = 0; sl < NSG; ++sl) {
           const int s = b0 * KM + sl;
-          const bool on = s < 64 ? ((V.mask0 >> s) & 1) : ((V.mask1 >> (s - 64)) & 1);
+          const bool on = (V.mask0 >> s) & 1;
           const float rs = V.radius + P.slot_sphere[s].w;
           vr2[sl] = on ? rs * rs : -1.f;
         }
@@ -515,7 +516,7 @@ __device__ __forceinline__ bool collides_fast_halves(const PlantImage &P, const 
           if (wave_done<ANY>(col)) break;
           const float vx = V.vox[3 * v], vy = V.vox[3 * v + 1], vz = V.vox[3 * v + 2];
 #pragma unroll
-          for (int sl = 0; sl < NSH; ++sl)
+          for (int sl = 0; sl < NSG; ++sl)
             col |= hit_sphere_sphere(vx, vy, vz, cx[sl], cy[sl], cz[sl], vr2[sl]);
         }
       }
@@ -685,14 +686,22 @@ __device__ __forceinline__ const float *stage_tables(const PlantImage &P, float 
 // (sub, nsub), PH: see collides_fast; the generic checker ignores the split
 // (part 0 does all of the work, other parts report no collision) and PH
 // (it always tests every pair).
+// body groups of the 64-slot layouts (collides_fast_groups): two bodies
+// (8 or 16 spheres) per group. Plain dual-arm EI-ZO, 64 segments: 92.8
+// regions/s with all 64 centres live (one wave per SIMD), 144.9 with 2
+// groups; 256 segments: 157.3 / 156.9 / 160.4 regions/s with 2 / 4 / 8
+// groups of the 16 x 4 layout (check 203 / 144 / 120 VGPRs, bisection
+// 235 / 181 / 156).
+constexpr int kGroups16x4 = 8, kGroups8x8 = 4;
+
 template <int VAR, bool ANY = false, int PH = 3, class Q>
 __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &qa,
                                                  const VoxelArgs &V, const RobotGeomList &R,
                                                  bool col, const float *tab, int sub = 0,
                                                  int nsub = 1) {
   if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
-  else if constexpr (VAR == kFast8x8) return collides_fast_halves<8, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
-  else if constexpr (VAR == kFast16x4) return collides_fast_halves<16, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast8x8) return collides_fast_groups<8, 8, kGroups8x8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast16x4) return collides_fast_groups<16, 4, kGroups16x4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if (sub != 0) return col;
   else if constexpr (VAR == kGeneric8) return collides_generic<8, ANY>(P, qa, V, R, col);
