@@ -341,11 +341,11 @@ struct RowArgs {
 // shift consecutive rows by four banks). NA = 0 stages Bop alone.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int NA, int NB, int KC>
+template <int NA, int NB, int KC, int NT = 256>
 struct ChunkLoader {
   static constexpr int kPerRow = KC / 8;
   static constexpr int kPieces = (NA + NB) * kPerRow;
-  static constexpr int kPer = (kPieces + 255) / 256;
+  static constexpr int kPer = (kPieces + NT - 1) / NT;
   static constexpr int kPitch = KC + 8;
   static_assert(kPer <= 16, "at most 16 pieces per thread");
   u32x4 v[kPer];
@@ -353,8 +353,8 @@ struct ChunkLoader {
                                        int ldb, int c0, int k0) {
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-      const int pc = (int)threadIdx.x + 256 * j;
-      if ((j + 1) * 256 <= kPieces || pc < kPieces) {
+      const int pc = (int)threadIdx.x + NT * j;
+      if ((j + 1) * NT <= kPieces || pc < kPieces) {
         const int row = pc / kPerRow, part = pc % kPerRow;
         const uint16_t *src = row < NA ? A + (size_t)(r0 + row) * lda
                                        : Bm + (size_t)(c0 + row - NA) * ldb;
@@ -365,8 +365,8 @@ struct ChunkLoader {
   __device__ __forceinline__ void store(uint16_t *sA) const {  // sA: NA + NB rows
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-      const int pc = (int)threadIdx.x + 256 * j;
-      if ((j + 1) * 256 <= kPieces || pc < kPieces)
+      const int pc = (int)threadIdx.x + NT * j;
+      if ((j + 1) * NT <= kPieces || pc < kPieces)
         *reinterpret_cast<u32x4 *>(sA + (pc / kPerRow) * kPitch + (pc % kPerRow) * 8) = v[j];
     }
   }
@@ -379,19 +379,25 @@ constexpr int kLdsA = kRB * kApitch;             // 8,448
 constexpr int kLdsW = kW * kWpitch;              // 34,816
 constexpr int kLdsS = kRB * kW;                  // one s_l [32][256]: 8,192
 
+// The row kernel's waves: 8 of them, each holding one 32 x 32 tile (32
+// columns) of a layer's 32 x 256 output, two per SIMD.
+constexpr int kRowTiles = 1;                 // 32 x 32 MFMA tiles per wave
+constexpr int kRowWaves = 8 / kRowTiles;
+constexpr int kRowThreads = 64 * kRowWaves;
+
 // One 128-deep chunk of a layer's row GEMM: C[32][256] += sA[.][k0 .. k0+128]
-// sW^T, wave w holding columns 64 w .. 64 w + 63 as two 32x32 MFMA tiles.
-template <int KC, int PITCH>
+// sW^T, wave w holding columns 32 TPW w .. + 32 TPW as TPW 32x32 MFMA tiles.
+template <int KC, int PITCH, int TPW>
 __device__ __forceinline__ void chunk_mfma(const uint16_t *sA, int k0, const uint16_t *sW,
-                                           f32x16 (&acc)[2]) {
+                                           f32x16 (&acc)[TPW]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hr = lane >> 5, c = lane & 31;
 #pragma unroll
   for (int ks = 0; ks < KC; ks += 16) {
     const bf16x8 af = ld8(sA + c * kApitch + k0 + ks + 8 * hr);
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      acc[j] = mfma(af, ld8(sW + (64 * wave + 32 * j + c) * PITCH + ks + 8 * hr), acc[j]);
+    for (int j = 0; j < TPW; ++j)
+      acc[j] = mfma(af, ld8(sW + (32 * TPW * wave + 32 * j + c) * PITCH + ks + 8 * hr), acc[j]);
   }
 }
 
@@ -433,8 +439,9 @@ struct RowLds {
 // 1..3; 6 layer 4 and the head; 8, 10, 12 delta_3, delta_2, delta_1. bf16
 // rounding by the hardware conversion (round to nearest even, as f2bf, for
 // these finite values).
+template <int TPW>
 __device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpDev &M,
-                                             const RowLds &L, f32x16 (&acc)[2]) {
+                                             const RowLds &L, f32x16 (&acc)[TPW]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int hr = lane >> 5, c = lane & 31;
   const int B = T.B, r0 = blockIdx.x * kRB;
@@ -444,8 +451,8 @@ __device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpD
     uint16_t *atl = l == 1 ? T.at1 : l == 2 ? T.at2 : T.at3;
     uint16_t *sl = L.sS + (l - 1) * kLdsS;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = 64 * wave + 32 * j + c;
+    for (int j = 0; j < TPW; ++j) {
+      const int col = 32 * TPW * wave + 32 * j + c;
       const float bc = bias[col];
 #pragma unroll
       for (int qd = 0; qd < 4; ++qd) {
@@ -465,13 +472,13 @@ __device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpD
   } else if (l == 4) {
     // layer 4, y = w5 . a4 + b5 per row, loss, delta_4 = dy w5 SiLU'(z4)
     const float *bias = M.p + MlpLayout::b(4);
-    float av[2][16], sv[2][16];
+    float av[TPW][16], sv[TPW][16];
     float part[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) part[i] = 0.f;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = 64 * wave + 32 * j + c;
+    for (int j = 0; j < TPW; ++j) {
+      const int col = 32 * TPW * wave + 32 * j + c;
       const float bc = bias[col];
       const float w = bf2f(M.w5b[col]);
 #pragma unroll
@@ -495,8 +502,10 @@ __device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpD
     __syncthreads();
     if (threadIdx.x < kRB) {  // lanes 0..31 of wave 0, one row each
       const int r = threadIdx.x;
-      const float y = L.s_row[0][r] + L.s_row[1][r] + L.s_row[2][r] + L.s_row[3][r] +
-                      M.p[MlpLayout::kB5];
+      float y = L.s_row[0][r];
+#pragma unroll
+      for (int w = 1; w < kRowWaves; ++w) y += L.s_row[w][r];
+      y += M.p[MlpLayout::kB5];
       const float e = y - T.target[r0 + r];
       const float dy = 2.f * e / (float)B;
       L.s_dy[r] = dy;
@@ -514,8 +523,8 @@ __device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpD
     __syncthreads();
     // delta_4 (bf16) into sA and delta_4^T; column sums db4 and dw5
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = 64 * wave + 32 * j + c;
+    for (int j = 0; j < TPW; ++j) {
+      const int col = 32 * TPW * wave + 32 * j + c;
       const float w = bf2f(M.w5b[col]);
       float sd = 0.f, sw = 0.f;
 #pragma unroll
@@ -546,8 +555,8 @@ __device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpD
     const uint16_t *sl = L.sS + (m - 2) * kLdsS;
     uint16_t *dtl = m == 4 ? T.dt3 : m == 3 ? T.dt2 : T.dt1;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = 64 * wave + 32 * j + c;
+    for (int j = 0; j < TPW; ++j) {
+      const int col = 32 * TPW * wave + 32 * j + c;
       float sd = 0.f;
 #pragma unroll
       for (int qd = 0; qd < 4; ++qd) {
@@ -566,61 +575,64 @@ __device__ __forceinline__ void row_epilogue(int n, const RowArgs &T, const MlpD
       if (hr == 0) atomicAdd(T.acc + kAccB + (m - 2) * kW + col, sd);
     }
   }
-  acc[0] = acc[1] = f32x16{};
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) acc[j] = f32x16{};
 }
 
-__global__ __launch_bounds__(256) void k_train_rows(RowArgs T, MlpDev M, const uint16_t *wt2,
+__global__ __launch_bounds__(kRowThreads) void k_train_rows(RowArgs T, MlpDev M, const uint16_t *wt2,
                                                     const uint16_t *wt3, const uint16_t *wt4) {
   __shared__ __attribute__((aligned(16))) uint16_t sA[kLdsA];
   __shared__ __attribute__((aligned(16))) uint16_t sW[kLdsW];
   __shared__ __attribute__((aligned(16))) uint16_t sS[3 * kLdsS];  // s_1, s_2, s_3
-  __shared__ float s_row[4][kRB];
+  __shared__ float s_row[kRowWaves][kRB];
   __shared__ float s_dy[kRB];
   const RowLds L{sA, sS, s_row, s_dy};
   const int B = T.B, r0 = blockIdx.x * kRB;
 
   // layer 1's weights (256 x 32) and x0 of the block's rows
   {
-    ChunkLoader<0, kW, kIn> l1;
+    ChunkLoader<0, kW, kIn, kRowThreads> l1;
     l1.load(nullptr, 0, 0, M.wb[1], kIn, 0, 0);
-    // x0: thread t normalises row t / 8, features 4 (t % 8) .. + 3
-    const int row = threadIdx.x >> 3, k0 = (threadIdx.x & 7) * 4;
-    uint16_t h[4];
+    // x0: 32 rows x 32 features, thread t takes row t / (N / 32), features
+    // E (t % (N / 32)) .. + E - 1 for N threads, E = 1024 / N
+    constexpr int E = kRB * kIn / kRowThreads;
+    const int row = threadIdx.x / (kIn / E), k0 = (threadIdx.x % (kIn / E)) * E;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < E; ++e) {
       const int k = k0 + e;
       const float x = k < M.in_dim
                           ? (T.q[(size_t)(r0 + row) * M.in_dim + k] - M.lo[k]) * M.scale[k] - 1.f
                           : 0.f;
-      h[e] = f2bf(x);
-      sA[row * kApitch + k] = h[e];
-      T.at0[(size_t)k * B + r0 + row] = h[e];
+      const uint16_t h = f2bf(x);
+      sA[row * kApitch + k] = h;
+      T.at0[(size_t)k * B + r0 + row] = h;
     }
     l1.store(sW);
   }
   // Chunks n = 1..12 stream through sW, two in flight at a time in
   // registers: odd chunks in `ldo`, even ones in `lde`.
-  ChunkLoader<0, kW, kKC> ldo, lde;
+  ChunkLoader<0, kW, kKC, kRowThreads> ldo, lde;
   __syncthreads();
   ldo.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 1), kW, 0, 0);
   lde.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 2), kW, 0, kKC);
-  f32x16 acc[2];
-  acc[0] = acc[1] = f32x16{};
-  chunk_mfma<kIn, kIn + 8>(sA, 0, sW, acc);
+  f32x16 acc[kRowTiles];
+#pragma unroll
+  for (int j = 0; j < kRowTiles; ++j) acc[j] = f32x16{};
+  chunk_mfma<kIn, kIn + 8, kRowTiles>(sA, 0, sW, acc);
   __syncthreads();  // every wave is done reading sA and sW
-  row_epilogue(0, T, M, L, acc);
+  row_epilogue<kRowTiles>(0, T, M, L, acc);
   ldo.store(sW);
   __syncthreads();
   ldo.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 3), kW, 0, 0);
   for (int pr = 1; pr <= 6; ++pr) {
-    chunk_mfma<kKC, kWpitch>(sA, 0, sW, acc);  // chunk 2 pr - 1
+    chunk_mfma<kKC, kWpitch, kRowTiles>(sA, 0, sW, acc);  // chunk 2 pr - 1
     __syncthreads();
     lde.store(sW);
     __syncthreads();
     if (pr < 6) lde.load(nullptr, 0, 0, stream_matrix(M, wt2, wt3, wt4, 2 * pr + 2), kW, 0, kKC);
-    chunk_mfma<kKC, kWpitch>(sA, kKC, sW, acc);  // chunk 2 pr
+    chunk_mfma<kKC, kWpitch, kRowTiles>(sA, kKC, sW, acc);  // chunk 2 pr
     __syncthreads();
-    row_epilogue(2 * pr, T, M, L, acc);
+    row_epilogue<kRowTiles>(2 * pr, T, M, L, acc);
     if (pr < 6) {
       ldo.store(sW);  // chunk 2 pr + 1
       __syncthreads();
@@ -943,7 +955,7 @@ int train_launches(csd_mlp *m, const float *q, const float *target, int B, float
   R.dt3 = T.dt[3];
   R.dt4 = T.dt[4];
   R.acc = T.acc;
-  hipLaunchKernelGGL(k_train_rows, dim3(B / kRB), dim3(256), 0, st, R, M, m->wt[2], m->wt[3],
+  hipLaunchKernelGGL(k_train_rows, dim3(B / kRB), dim3(kRowThreads), 0, st, R, M, m->wt[2], m->wt[3],
                      m->wt[4]);
   WgradArgs W;
   memset(&W, 0, sizeof(W));
