@@ -40,6 +40,31 @@ __device__ __forceinline__ bool wave_done(bool col) {
   else return __all(col);
 }
 
+// Bounding-sphere culling: true if, in every lane still undecided (col
+// false), the centres a and b lie farther apart than rsum, the sum of two
+// bounding radii about them. A body's bound is taken about the centre of its
+// first sphere slot (PlantImage::body_bound), an obstacle's about its own
+// centre (PlantImage::obstacle_bound). No sphere of the body can then touch
+// the obstacle or the other body's spheres in any such lane -- both radii are
+// inflated far beyond FP32 rounding -- and the wave skips those exact tests,
+// so every verdict is unchanged. Lanes already colliding count as apart.
+__device__ __forceinline__ bool wave_apart(float ax, float ay, float az, float bx, float by,
+                                           float bz, float rsum, bool col) {
+  const float dx = ax - bx, dy = ay - by, dz = az - bz;
+  return __all(col || fmaf(dx, dx, fmaf(dy, dy, dz * dz)) > rsum * rsum);
+}
+
+// a body (first-slot centre x, y, z; bound rb) against an obstacle's bound
+__device__ __forceinline__ bool wave_clear(float x, float y, float z, float rb,
+                                           const float4 &ob, bool col) {
+  return wave_apart(x, y, z, ob.x, ob.y, ob.z, rb + ob.w, col);
+}
+
+// PH bit 2 of the collide functions: the wave's configurations are close
+// together in configuration space, so bounding-sphere culls are likely to
+// hold for the whole wave
+constexpr int kCoherent = 4;
+
 // Chain FK of the fast path fused with the sphere centres: only the current
 // body frame is live, and every register index is a compile-time constant.
 // RAD: also return the slot radii (kept in vector registers).
@@ -91,6 +116,11 @@ __device__ __forceinline__ void fast_spheres(const PlantImage &P, const Q &qa,
 // PH selects the pair classes tested: bit 0 the moving x moving (self)
 // pairs, bit 1 the obstacles and voxels; a two-pass checker runs the cheap
 // self pairs first and the obstacles only for configurations still free.
+// Bit 2 (kCoherent) says the wave's configurations lie close together
+// (samples of one region, points of one edge): body-pair blocks are then
+// culled too, and each body's contact distances are read just before its
+// obstacle tests instead of all of them per obstacle. For batches of
+// unrelated configurations both cost more than they save (measured).
 template <int NB, int KM, bool ANY = false, int PH = 3, class Q>
 __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
                                               const VoxelArgs &V, bool col, const float *tab,
@@ -104,6 +134,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   constexpr bool kTab = NS <= 32;
   constexpr bool kLds = NS == 64;
   constexpr bool kMask = !kTab && !kLds;
+  constexpr bool kCoh = (PH & kCoherent) != 0;
   float cx[NS], cy[NS], cz[NS], rad[NS];  // sphere centres (world) and radii
   fast_spheres<NB, KM, kMask>(P, qa, cx, cy, cz, rad);
   col = col || P.always_colliding;
@@ -130,6 +161,9 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
         const bool mine = rr == sub;
         if (++rr == nsub) rr = 0;
         if (!mine) continue;
+        if (kCoh && wave_apart(cx[ba * KM], cy[ba * KM], cz[ba * KM], cx[bb * KM], cy[bb * KM],
+                               cz[bb * KM], P.body_bound[ba] + P.body_bound[bb], col))
+          continue;
         if constexpr (kTab || kLds) {
           const float *r2 = (kLds ? tab : P.self_r2) + ((size_t)ba * NB + bb) * (KM * KM);
 #pragma unroll
@@ -165,45 +199,73 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // obstacle with a few wide loads (scalar loads for one part; with parts
   // derived from the wave index the compiler cannot prove o uniform and
   // issues vector loads, which all lanes share through the L1 -- forcing
-  // scalar loads there measured slower); the slot loop has no branch.
+  // scalar loads there measured slower); the slot loop has no branch. With
+  // kCoherent the row is read body by body, for the bodies not culled.
   if constexpr (kTab && (PH & 2)) {
     for (int o = o0; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
-      float ov[16], r2[NS];
+      const float4 ob = P.obstacle_bound[o];
+      float ov[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+      const float *orow = P.obstacle_r2 + (size_t)o * NS;
+      float r2[kCoh ? 1 : NS];
+      if constexpr (!kCoh) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) r2[s] = P.obstacle_r2[(size_t)o * NS + s];
+        for (int s = 0; s < NS; ++s) r2[s] = orow[s];
+      }
+      auto skip = [&](int b) {
+        return b >= P.num_bodies ||
+               wave_clear(cx[b * KM], cy[b * KM], cz[b * KM], P.body_bound[b], ob, col);
+      };
       if (t == CSD_BOX) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b >= P.num_bodies) continue;
+          if (skip(b)) continue;
+          float rb[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            if constexpr (kCoh) rb[k] = orow[b * KM + k];
+            else rb[k] = r2[b * KM + k];
+          }
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
-            col |= hit_sphere_obox(cx[s], cy[s], cz[s], r2[s], ov);
+            col |= hit_sphere_obox(cx[s], cy[s], cz[s], rb[k], ov);
           }
         }
       } else if (t == CSD_SPHERE) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b >= P.num_bodies) continue;
+          if (skip(b)) continue;
+          float rb[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            if constexpr (kCoh) rb[k] = orow[b * KM + k];
+            else rb[k] = r2[b * KM + k];
+          }
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
-            col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], r2[s]);
+            col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], rb[k]);
           }
         }
       } else {  // capsule
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b >= P.num_bodies) continue;
+          if (skip(b)) continue;
+          float rb[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            if constexpr (kCoh) rb[k] = orow[b * KM + k];
+            else rb[k] = r2[b * KM + k];
+          }
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
             col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
-                                ov[6]) < r2[s];
+                                ov[6]) < rb[k];
           }
         }
       }
@@ -218,13 +280,16 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
     for (int o = o0; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
+      const float4 ob = P.obstacle_bound[o];
       float ov[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
       const float *r2 = orow + (size_t)o * NS;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        if (b >= P.num_bodies) continue;
+        if (b >= P.num_bodies ||
+            wave_clear(cx[b * KM], cy[b * KM], cz[b * KM], P.body_bound[b], ob, col))
+          continue;
         float rb[KM];
 #pragma unroll
         for (int k = 0; k < KM; ++k) rb[k] = r2[b * KM + k];
@@ -278,10 +343,15 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
       m0n = P.obstacle_mask[2 * onext];
       m1n = P.obstacle_mask[2 * onext + 1];
       auto paired = [&](int s) { return s < 64 ? ((m0 >> s) & 1) : ((m1 >> (s - 64)) & 1); };
+      const float4 ob = P.obstacle_bound[o];
+      auto skip = [&](int b) {
+        return b >= P.num_bodies ||
+               wave_clear(cx[b * KM], cy[b * KM], cz[b * KM], P.body_bound[b], ob, col);
+      };
       if (t == CSD_BOX) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b >= P.num_bodies) continue;
+          if (skip(b)) continue;
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
@@ -292,7 +362,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
       } else if (t == CSD_SPHERE) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b >= P.num_bodies) continue;
+          if (skip(b)) continue;
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
@@ -304,7 +374,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
       } else {  // capsule
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b >= P.num_bodies) continue;
+          if (skip(b)) continue;
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
@@ -354,6 +424,7 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
                                                      const float *tab, int sub = 0,
                                                      int nsub = 1) {
   constexpr int KG = NB / G, NSG = KG * KM, NS = NB * KM;
+  constexpr bool kCoh = (PH & kCoherent) != 0;
   static_assert(NS == 64 && KG * G == NB, "the LDS-table plants, whole groups");
   col = col || P.always_colliding;
   const float *orow = tab + NB * NB * KM * KM;
@@ -407,6 +478,10 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
         for (int blb = 0; blb < KG; ++blb) {
           const int bb = b0 + blb;
           if (blb <= bla || !((act >> bb) & 1) || !mine()) continue;
+          if (kCoh && wave_apart(cx[bla * KM], cy[bla * KM], cz[bla * KM], cx[blb * KM],
+                                 cy[blb * KM], cz[blb * KM], P.body_bound[ba] + P.body_bound[bb],
+                                 col))
+            continue;
           const float *r2 = tab + ((size_t)ba * NB + bb) * (KM * KM);
 #pragma unroll
           for (int ka = 0; ka < KM; ++ka)
@@ -449,6 +524,9 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
 #pragma unroll
           for (int blb = 0; blb < KG; ++blb) {
             if (!((act >> blb) & 1) || !mine()) continue;
+            if (kCoh && wave_apart(ax[0], ay[0], az[0], cx[blb * KM], cy[blb * KM], cz[blb * KM],
+                                   P.body_bound[ba] + P.body_bound[b0 + blb], col))
+              continue;
             const float *r2 = tab + ((size_t)ba * NB + b0 + blb) * (KM * KM);
 #pragma unroll
             for (int ka = 0; ka < KM; ++ka)
@@ -469,6 +547,7 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
       for (int o = o0; o < P.num_obstacles; o += nsub) {
         if (wave_done<ANY>(col)) break;
         const int t = P.obstacle_type[o];
+        const float4 ob = P.obstacle_bound[o];
         float ov[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
@@ -476,7 +555,9 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
 #pragma unroll
         for (int bl = 0; bl < KG; ++bl) {
           const int b = b0 + bl;
-          if (b >= P.num_bodies) continue;
+          if (b >= P.num_bodies ||
+              wave_clear(cx[bl * KM], cy[bl * KM], cz[bl * KM], P.body_bound[b], ob, col))
+            continue;
           float rb[KM];
 #pragma unroll
           for (int k = 0; k < KM; ++k) rb[k] = r2[b * KM + k];

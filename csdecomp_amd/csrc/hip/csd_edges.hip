@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kEdgeBlock) void k_validate_edges(
 #pragma unroll
         for (int d = 0; d < kEdgeDim; ++d)
           q[d] = __fadd_rn(__fmul_rn(t, p2[d]), __fmul_rn(__fsub_rn(1.f, t), p1[d]));
-        const bool col = collides_variant<VAR, true>(P, QReg<kEdgeDim>{q}, V, R, false, tab);
+        const bool col = collides_variant<VAR, true, 3 | kCoherent>(P, QReg<kEdgeDim>{q}, V, R, false, tab);
         any = __any(col);
       }
       if (lane == src) out[e] = any ? 0 : 1;

@@ -99,7 +99,18 @@ struct PlantImage {
   const uint64_t *self_bits;      // [nbt][nbt]
   const float *self_r2;           // [nbt][nbt][kmax][kmax] (nslots <= 64), 0 if absent
   const int32_t *geom_slot;       // [G] slot of moving sphere geometry, -1 else
+  // bounds for culling obstacle tests. body_bound[b]: radius about the
+  // centre of body b's first sphere slot that encloses all of its spheres;
+  // obstacle_bound[o]: world centre + radius of a sphere enclosing obstacle
+  // o. Both radii are inflated by kBoundInflate (relative) + kBoundPad
+  // (metres), far beyond FP32 rounding of the centres and of the exact
+  // tests, so a culled test can never have reported a hit.
+  const float *body_bound;        // [nbt]
+  const float4 *obstacle_bound;   // [num_obstacles]
 };
+
+constexpr double kBoundInflate = 1e-3;
+constexpr double kBoundPad = 5e-4;
 
 // Screen of the fast checker: a subset of the plant's pair tests that finds
 // most collisions of a workload early. Configurations the screen leaves

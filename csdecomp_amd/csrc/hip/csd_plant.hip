@@ -414,6 +414,7 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   std::vector<float4> slot_sphere(std::max(nslots, 1), make_float4(0, 0, 0, -1));
   std::vector<int32_t> geom_slot(std::max(G, 1), -1);
   std::vector<DevObstacle> obstacles;
+  std::vector<float4> obstacle_bound;
   std::vector<int32_t> obstacle_type;
   std::vector<std::pair<int, int>> obstacle_slot;  // (obstacle, slot) pairs
   const int nbt_alloc = fast ? nbt : 1;
@@ -422,12 +423,41 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   const bool tables = fast && nslots <= 64;
   std::vector<float> self_r2(tables ? (size_t)nbt * nbt * kmax * kmax : 1, 0.f);
   int num_self_pairs = 0;
+  // body bounds for culling: per body, the sphere whose centre encloses all
+  // of the body's spheres in the smallest radius takes the body's first slot
+  // and body_bound[b] holds that radius, inflated; the kernels then skip a
+  // body's obstacle tests when, for every undecided lane, its first sphere
+  // centre is farther from the obstacle's bounding sphere than the sum.
+  std::vector<float> body_bound(nbt_alloc, 0.f);
   if (fast) {
-    std::vector<int> used(NB, 0);
+    std::vector<int> used(NB, 0), first(NB, -1);
+    std::vector<double> best(NB, 0);
+    for (int g = 0; g < G; ++g) {
+      if (gbody[g] < 0 || geoms[g].type != CSD_SPHERE) continue;
+      const int b = gbody[g];
+      double rad = 0;
+      for (int h = 0; h < G; ++h) {
+        if (gbody[h] != b || geoms[h].type != CSD_SPHERE) continue;
+        double d2 = 0;
+        for (int k = 0; k < 3; ++k) {
+          const double dk = (double)geoms[h].G[4 * k + 3] - geoms[g].G[4 * k + 3];
+          d2 += dk * dk;
+        }
+        rad = std::max(rad, std::sqrt(d2) + geoms[h].dims[0]);
+      }
+      if (first[b] < 0 || rad < best[b]) {
+        first[b] = g;
+        best[b] = rad;
+      }
+    }
+    for (int b = 0; b < NB; ++b) {
+      used[b] = first[b] >= 0 ? 1 : 0;
+      body_bound[b] = (float)(best[b] * (1.0 + kBoundInflate) + kBoundPad);
+    }
     for (int g = 0; g < G; ++g) {
       if (gbody[g] < 0 || geoms[g].type != CSD_SPHERE) continue;
       int b = gbody[g];
-      int slot = b * kmax + used[b]++;
+      int slot = b * kmax + (g == first[b] ? 0 : used[b]++);
       geom_slot[g] = slot;
       slot_sphere[slot] = make_float4(geoms[g].G[3], geoms[g].G[7], geoms[g].G[11], geoms[g].dims[0]);
     }
@@ -487,6 +517,18 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
           rec.v[6] = n2 > 0 ? (float)(1.0 / n2) : 0.f;
           rec.v[7] = (float)gdim[og][0];
         }
+        // bounding sphere: box centre and |half extents|, the sphere itself,
+        // capsule midpoint and half length + radius; inflated like the bodies'
+        double bc[3] = {X.m[3], X.m[7], X.m[11]}, br;
+        if (t == CSD_BOX)
+          br = std::sqrt(gdim[og][0] * gdim[og][0] + gdim[og][1] * gdim[og][1] +
+                         gdim[og][2] * gdim[og][2]);
+        else if (t == CSD_SPHERE)
+          br = gdim[og][0];
+        else
+          br = gdim[og][0] + gdim[og][1];
+        obstacle_bound.push_back(make_float4((float)bc[0], (float)bc[1], (float)bc[2],
+                                             (float)(br * (1.0 + kBoundInflate) + kBoundPad)));
         obstacles.push_back(rec);
         obstacle_type.push_back(t);
       }
@@ -537,6 +579,9 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   size_t o_sr2 = bld.add(self_r2.data(), self_r2.size());
   size_t o_or2 = bld.add(obstacle_r2.data(), obstacle_r2.size());
   size_t o_gslot = bld.add(geom_slot.data(), geom_slot.size());
+  if (obstacle_bound.empty()) obstacle_bound.push_back(make_float4(0, 0, 0, 0));
+  size_t o_obound = bld.add(obstacle_bound.data(), obstacle_bound.size());
+  size_t o_bbound = bld.add(body_bound.data(), body_bound.size());
 
   csd_plant *pl = new csd_plant();
   hipError_t e = hipGetDevice(&pl->device);
@@ -586,6 +631,8 @@ int csd_plant_create(const csd_plant_desc *d, csd_plant **out) {
   img.self_r2 = (const float *)(base + o_sr2);
   img.obstacle_r2 = (const float *)(base + o_or2);
   img.geom_slot = (const int32_t *)(base + o_gslot);
+  img.obstacle_bound = (const float4 *)(base + o_obound);
+  img.body_bound = (const float *)(base + o_bbound);
   pl->geom_type.assign(d->geom_type, d->geom_type + G);
   pl->geom_link.assign(d->geom_link, d->geom_link + G);
   pl->geom_body = gbody;

@@ -165,9 +165,9 @@ def test_fast_path_voxels(gpu, csd, orc):
     assert_flags_match(got, ref, sd, "fast voxels")
 
 
-def test_capsules_vs_oracle(gpu, csd, orc):
-    """Capsule obstacles (fast path) and a capsule link (generic path) against
-    the FP64 segment distances of the oracle."""
+def _capsule_scene():
+    """The 7-DOF arm with 8 boxes and 6 capsule obstacles paired with every
+    robot sphere: (scene, plant, geometries, pairs, index of the first capsule)."""
     from csdecomp_amd import scenes
     from csdecomp_amd import _pycsdecomp as c
     pl = scenes.arm_scene(1000, 8, 0)
@@ -194,6 +194,15 @@ def test_capsules_vs_oracle(gpu, csd, orc):
         pairs += [(r, gi) for r in robot]
     mp.set_scene_collision_geometries(geoms)
     mp.set_collision_pairs_from_matrix(np.asarray(pairs, np.int32).T)
+    return pl, mp, geoms, pairs, start
+
+
+def test_capsules_vs_oracle(gpu, csd, orc):
+    """Capsule obstacles (fast path) and a capsule link (generic path) against
+    the FP64 segment distances of the oracle."""
+    from csdecomp_amd import scenes
+    from csdecomp_amd import _pycsdecomp as c
+    pl, mp, geoms, pairs, start = _capsule_scene()
     q = scenes.uniform_configs(13, 6000, pl.getPositionLowerLimits(), pl.getPositionUpperLimits())
     op = orc.Plant(mp)
     sd = orc.min_sd(op, q)
@@ -215,6 +224,53 @@ def test_capsules_vs_oracle(gpu, csd, orc):
     sd = orc.min_sd(op, q)
     got = np.asarray(csd.CheckCollisionFreeCuda(q, mp), np.uint8)
     assert_flags_match(got, orc.check(op, q, orc.RULE_GPU), sd, "capsule link")
+
+
+@pytest.mark.parametrize("name", ["arm7_16box", "arm7_48box16sph", "dualarm14_72box24sph",
+                                  "capsules"])
+def test_bound_cull_whole_waves(gpu, csd, orc, name):
+    """Each wave of 64 lanes holds copies of one configuration, so the
+    bounding-sphere cull of the obstacle tests decides for the whole wave: a
+    body or obstacle bound that missed a sphere, a box corner or a capsule
+    end would drop that configuration's collisions. Configurations free of
+    self collision are taken, so that the obstacle tests decide them."""
+    import torch
+    from csdecomp_amd import capi, scenes
+    from csdecomp_amd import _pycsdecomp as c
+
+    def plant():
+        if name == "capsules":
+            pl, mp, *_ = _capsule_scene()
+            return mp, pl.getPositionLowerLimits(), pl.getPositionUpperLimits()
+        return _plant(name)
+
+    mp, lo, hi = plant()
+    # the same plant with only its robot x robot pairs
+    ms, _, _ = plant()
+    robot = {i for i, g in enumerate(ms.scene_collision_geometries)
+             if g.group == c.CollisionGroup.ROBOT}
+    pairs = np.asarray(ms.collision_pairs_flat).reshape(-1, 2)
+    ms.set_collision_pairs_from_matrix(
+        np.ascontiguousarray(pairs[[a in robot and b in robot for a, b in pairs]].T, np.int32))
+    op = orc.Plant(mp)
+    q = scenes.uniform_configs(77, 8000, lo, hi)
+    self_free = orc.check(orc.Plant(ms), q, orc.RULE_GPU).astype(bool)
+    q = np.ascontiguousarray(q[:, self_free][:, :1024])
+    ref = orc.check(op, q, orc.RULE_GPU)
+    sd = orc.min_sd(op, q)
+    m = q.shape[1]
+    assert m >= 256 and 0.05 < np.mean(ref) < 0.95, "want both outcomes among self-free configurations"
+    qq = torch.from_numpy(np.ascontiguousarray(np.repeat(q, 64, axis=1))).to(gpu)
+    out = torch.empty(m * 64, dtype=torch.uint8, device=gpu)
+    dp = capi.DevicePlant(mp)
+    for shape in (1, 2, 4, 11, 14):
+        out.fill_(7)
+        assert capi.tune_check(dp, qq, m * 64, 1, m * 64, out, force=shape) == shape
+        capi.check_collision_free(dp, qq, m * 64, 1, m * 64, out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().reshape(m, 64)
+        assert np.all(got == got[:, :1]), f"shape {shape}: copies of one configuration disagree"
+        assert_flags_match(got[:, 0], ref, sd, f"{name} whole waves, shape {shape}")
 
 
 def test_pair_collision_free(gpu, csd):
