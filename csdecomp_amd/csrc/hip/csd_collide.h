@@ -65,6 +65,90 @@ __device__ __forceinline__ bool wave_clear(float x, float y, float z, float rb,
 // hold for the whole wave
 constexpr int kCoherent = 4;
 
+// Obstacle tests of bodies b0 .. b0 + KB - 1 (sphere centres cx[bl * KM + k])
+// against obstacles o0, o0 + nsub, ..., culled in chunks of 32 obstacles:
+// first the bounding-sphere culls of the whole chunk -- independent of each
+// other, so their bound loads overlap instead of each obstacle waiting for
+// its own -- give a mask of the (body, obstacle) blocks left per body; then
+// only the obstacles with a block left have their records read and tested.
+// row(o, b, rb) fills rb[KM] with body b's contact distances against o.
+template <int KB, int KM, bool ANY, class Row>
+__device__ __forceinline__ bool obstacles_culled(const PlantImage &P, int b0,
+                                                 const float (&cx)[KB * KM],
+                                                 const float (&cy)[KB * KM],
+                                                 const float (&cz)[KB * KM], bool col, int o0,
+                                                 int nsub, const Row &row) {
+  for (int c0 = o0; c0 < P.num_obstacles; c0 += 32 * nsub) {
+    if (wave_done<ANY>(col)) break;
+    uint32_t need[KB];
+#pragma unroll
+    for (int bl = 0; bl < KB; ++bl) need[bl] = 0;
+#pragma unroll 8
+    for (int j = 0; j < 32; ++j) {
+      const int o = c0 + j * nsub;
+      if (o >= P.num_obstacles) continue;
+      const float4 ob = P.obstacle_bound[o];
+#pragma unroll
+      for (int bl = 0; bl < KB; ++bl)
+        if (b0 + bl < P.num_bodies &&
+            !wave_clear(cx[bl * KM], cy[bl * KM], cz[bl * KM], P.body_bound[b0 + bl], ob, col))
+          need[bl] |= 1u << j;
+    }
+    uint32_t todo = 0;
+#pragma unroll
+    for (int bl = 0; bl < KB; ++bl) todo |= need[bl];
+    while (todo != 0) {
+      if (wave_done<ANY>(col)) break;
+      const int j = __builtin_ctz(todo);
+      todo &= todo - 1;
+      const int o = c0 + j * nsub;
+      const int t = P.obstacle_type[o];
+      float ov[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+      if (t == CSD_BOX) {
+#pragma unroll
+        for (int bl = 0; bl < KB; ++bl) {
+          if (!((need[bl] >> j) & 1)) continue;
+          float rb[KM];
+          row(o, b0 + bl, rb);
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = bl * KM + k;
+            col |= hit_sphere_obox(cx[s], cy[s], cz[s], rb[k], ov);
+          }
+        }
+      } else if (t == CSD_SPHERE) {
+#pragma unroll
+        for (int bl = 0; bl < KB; ++bl) {
+          if (!((need[bl] >> j) & 1)) continue;
+          float rb[KM];
+          row(o, b0 + bl, rb);
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = bl * KM + k;
+            col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], rb[k]);
+          }
+        }
+      } else {  // capsule
+#pragma unroll
+        for (int bl = 0; bl < KB; ++bl) {
+          if (!((need[bl] >> j) & 1)) continue;
+          float rb[KM];
+          row(o, b0 + bl, rb);
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = bl * KM + k;
+            col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                                ov[6]) < rb[k];
+          }
+        }
+      }
+    }
+  }
+  return col;
+}
+
 // Chain FK of the fast path fused with the sphere centres: only the current
 // body frame is live, and every register index is a compile-time constant.
 // RAD: also return the slot radii (kept in vector registers).
@@ -200,8 +284,11 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
   // derived from the wave index the compiler cannot prove o uniform and
   // issues vector loads, which all lanes share through the L1 -- forcing
   // scalar loads there measured slower); the slot loop has no branch. With
-  // kCoherent the row is read body by body, for the bodies not culled.
-  if constexpr (kTab && (PH & 2)) {
+  // kCoherent the row is read body by body, for the bodies not culled. (The
+  // chunked culls of obstacles_culled take this path, with all 32 centres
+  // live, to 256 VGPRs and one wave per SIMD, so it culls obstacle by
+  // obstacle.)
+  if constexpr (kTab && (PH & 2) && kCoh) {
     for (int o = o0; o < P.num_obstacles; o += nsub) {
       if (wave_done<ANY>(col)) break;
       const int t = P.obstacle_type[o];
@@ -210,11 +297,6 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
 #pragma unroll
       for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
       const float *orow = P.obstacle_r2 + (size_t)o * NS;
-      float r2[kCoh ? 1 : NS];
-      if constexpr (!kCoh) {
-#pragma unroll
-        for (int s = 0; s < NS; ++s) r2[s] = orow[s];
-      }
       auto skip = [&](int b) {
         return b >= P.num_bodies ||
                wave_clear(cx[b * KM], cy[b * KM], cz[b * KM], P.body_bound[b], ob, col);
@@ -225,10 +307,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
           if (skip(b)) continue;
           float rb[KM];
 #pragma unroll
-          for (int k = 0; k < KM; ++k) {
-            if constexpr (kCoh) rb[k] = orow[b * KM + k];
-            else rb[k] = r2[b * KM + k];
-          }
+          for (int k = 0; k < KM; ++k) rb[k] = orow[b * KM + k];
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
@@ -241,10 +320,7 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
           if (skip(b)) continue;
           float rb[KM];
 #pragma unroll
-          for (int k = 0; k < KM; ++k) {
-            if constexpr (kCoh) rb[k] = orow[b * KM + k];
-            else rb[k] = r2[b * KM + k];
-          }
+          for (int k = 0; k < KM; ++k) rb[k] = orow[b * KM + k];
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
@@ -257,15 +333,60 @@ __device__ __forceinline__ bool collides_fast(const PlantImage &P, const Q &qa,
           if (skip(b)) continue;
           float rb[KM];
 #pragma unroll
-          for (int k = 0; k < KM; ++k) {
-            if constexpr (kCoh) rb[k] = orow[b * KM + k];
-            else rb[k] = r2[b * KM + k];
-          }
+          for (int k = 0; k < KM; ++k) rb[k] = orow[b * KM + k];
 #pragma unroll
           for (int k = 0; k < KM; ++k) {
             const int s = b * KM + k;
             col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
                                 ov[6]) < rb[k];
+          }
+        }
+      }
+    }
+  }
+  if constexpr (kTab && (PH & 2) && !kCoh) {
+    for (int o = o0; o < P.num_obstacles; o += nsub) {
+      if (wave_done<ANY>(col)) break;
+      const int t = P.obstacle_type[o];
+      const float4 ob = P.obstacle_bound[o];
+      float ov[16], r2[NS];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) r2[s] = P.obstacle_r2[(size_t)o * NS + s];
+      auto skip = [&](int b) {
+        return b >= P.num_bodies ||
+               wave_clear(cx[b * KM], cy[b * KM], cz[b * KM], P.body_bound[b], ob, col);
+      };
+      if (t == CSD_BOX) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (skip(b)) continue;
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= hit_sphere_obox(cx[s], cy[s], cz[s], r2[s], ov);
+          }
+        }
+      } else if (t == CSD_SPHERE) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (skip(b)) continue;
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], r2[s]);
+          }
+        }
+      } else {  // capsule
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (skip(b)) continue;
+#pragma unroll
+          for (int k = 0; k < KM; ++k) {
+            const int s = b * KM + k;
+            col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
+                                ov[6]) < r2[s];
           }
         }
       }
@@ -540,49 +661,19 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
         }
       }
     }
-    // this group's spheres x world-fixed obstacles: the record in scalar
-    // registers, each body's row of contact distances from LDS
+    // this group's spheres x world-fixed obstacles in culled chunks: the
+    // record in scalar registers, each body's row of contact distances from
+    // LDS. (Culling obstacle by obstacle, each iteration waited for its own
+    // bound and record loads: dual-arm bisection took 15.8 ms per call at 64
+    // segments, 11% VALU-busy; in chunks it takes 9.1 ms.)
     if constexpr (PH & 2) {
       const int o0 = sub >= rr ? sub - rr : sub - rr + nsub;
-      for (int o = o0; o < P.num_obstacles; o += nsub) {
-        if (wave_done<ANY>(col)) break;
-        const int t = P.obstacle_type[o];
-        const float4 ob = P.obstacle_bound[o];
-        float ov[16];
+      col = obstacles_culled<KG, KM, ANY>(P, b0, cx, cy, cz, col, o0, nsub,
+                                          [&](int o, int b, float *rb) {
+                                            const float *r2 = orow + (size_t)o * NS + b * KM;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) ov[k] = P.obstacles[o].v[k];
-        const float *r2 = orow + (size_t)o * NS;
-#pragma unroll
-        for (int bl = 0; bl < KG; ++bl) {
-          const int b = b0 + bl;
-          if (b >= P.num_bodies ||
-              wave_clear(cx[bl * KM], cy[bl * KM], cz[bl * KM], P.body_bound[b], ob, col))
-            continue;
-          float rb[KM];
-#pragma unroll
-          for (int k = 0; k < KM; ++k) rb[k] = r2[b * KM + k];
-          if (t == CSD_BOX) {
-#pragma unroll
-            for (int k = 0; k < KM; ++k) {
-              const int s = bl * KM + k;
-              col |= hit_sphere_obox(cx[s], cy[s], cz[s], rb[k], ov);
-            }
-          } else if (t == CSD_SPHERE) {
-#pragma unroll
-            for (int k = 0; k < KM; ++k) {
-              const int s = bl * KM + k;
-              col |= hit_sphere_sphere(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], rb[k]);
-            }
-          } else {
-#pragma unroll
-            for (int k = 0; k < KM; ++k) {
-              const int s = bl * KM + k;
-              col |= seg_point_d2(cx[s], cy[s], cz[s], ov[0], ov[1], ov[2], ov[3], ov[4], ov[5],
-                                  ov[6]) < rb[k];
-            }
-          }
-        }
-      }
+                                            for (int k = 0; k < KM; ++k) rb[k] = r2[k];
+                                          });
       // this group's spheres x voxel spheres
       if (V.num > 0) {
         float vr2[NSG];
