@@ -64,6 +64,11 @@ __device__ __forceinline__ bool wave_clear(float x, float y, float z, float rb,
 // together in configuration space, so bounding-sphere culls are likely to
 // hold for the whole wave
 constexpr int kCoherent = 4;
+// PH bit 3: check 32-slot plants in body groups (collides_fast_groups), which
+// keeps fewer centres live and culls obstacles in chunks. It pays for EI-ZO's
+// sample checks and single-wave bisection; edge points (any-collision exits)
+// and bisection split over several waves measured faster without it.
+constexpr int kGrouped = 8;
 
 // Obstacle tests of bodies b0 .. b0 + KB - 1 (sphere centres cx[bl * KM + k])
 // against obstacles o0, o0 + nsub, ..., culled in chunks of 32 obstacles:
@@ -546,9 +551,12 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
                                                      int nsub = 1) {
   constexpr int KG = NB / G, NSG = KG * KM, NS = NB * KM;
   constexpr bool kCoh = (PH & kCoherent) != 0;
-  static_assert(NS == 64 && KG * G == NB, "the LDS-table plants, whole groups");
+  static_assert(NS <= 64 && KG * G == NB, "the table plants, whole groups");
   col = col || P.always_colliding;
-  const float *orow = tab + NB * NB * KM * KM;
+  // contact-distance tables: staged in LDS (tab) for 64 slots, the plant's
+  // own arrays (same layouts) for 32
+  const float *stab = NS == 64 ? tab : P.self_r2;
+  const float *orow = NS == 64 ? tab + NB * NB * KM * KM : P.obstacle_r2;
   int rr = 0;  // the part that takes the next body-pair block
   auto mine = [&]() {
     const bool m = rr == sub;
@@ -603,7 +611,7 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
                                  cy[blb * KM], cz[blb * KM], P.body_bound[ba] + P.body_bound[bb],
                                  col))
             continue;
-          const float *r2 = tab + ((size_t)ba * NB + bb) * (KM * KM);
+          const float *r2 = stab + ((size_t)ba * NB + bb) * (KM * KM);
 #pragma unroll
           for (int ka = 0; ka < KM; ++ka)
 #pragma unroll
@@ -648,7 +656,7 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
             if (kCoh && wave_apart(ax[0], ay[0], az[0], cx[blb * KM], cy[blb * KM], cz[blb * KM],
                                    P.body_bound[ba] + P.body_bound[b0 + blb], col))
               continue;
-            const float *r2 = tab + ((size_t)ba * NB + b0 + blb) * (KM * KM);
+            const float *r2 = stab + ((size_t)ba * NB + b0 + blb) * (KM * KM);
 #pragma unroll
             for (int ka = 0; ka < KM; ++ka)
 #pragma unroll
@@ -865,13 +873,18 @@ __device__ __forceinline__ const float *stage_tables(const PlantImage &P, float 
 // groups of the 16 x 4 layout (check 203 / 144 / 120 VGPRs, bisection
 // 235 / 181 / 156).
 constexpr int kGroups16x4 = 8, kGroups8x8 = 4;
+// 32 slots with kGrouped: 4 groups of two bodies, so that the chunked
+// obstacle culls fit in registers
+constexpr int kGroups8x4 = 4;
 
 template <int VAR, bool ANY = false, int PH = 3, class Q>
 __device__ __forceinline__ bool collides_variant(const PlantImage &P, const Q &qa,
                                                  const VoxelArgs &V, const RobotGeomList &R,
                                                  bool col, const float *tab, int sub = 0,
                                                  int nsub = 1) {
-  if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  if constexpr (VAR == kFast8x4 && (PH & kGrouped))
+    return collides_fast_groups<8, 4, kGroups8x4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
+  else if constexpr (VAR == kFast8x4) return collides_fast<8, 4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if constexpr (VAR == kFast8x8) return collides_fast_groups<8, 8, kGroups8x8, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if constexpr (VAR == kFast16x4) return collides_fast_groups<16, 4, kGroups16x4, ANY, PH>(P, qa, V, col, tab, sub, nsub);
   else if constexpr (VAR == kFast16x8) return collides_fast<16, 8, ANY, PH>(P, qa, V, col, tab, sub, nsub);

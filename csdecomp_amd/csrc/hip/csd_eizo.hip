@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_check(PlantImage P, Voxel
       valid && (!pred || pred[((size_t)slot * nblk + blockIdx.x) * kSampleBlock + threadIdx.x] <
                              pred_below);
   // a wave with nothing to check skips the check (and its forward kinematics)
-  const bool col = __any(check) && collides_variant<VAR, false, 3 | kCoherent>(P, QReg<DP>{x}, V, R, !check, tab) && check;
+  const bool col = __any(check) && collides_variant<VAR, false, 3 | kCoherent | kGrouped>(P, QReg<DP>{x}, V, R, !check, tab) && check;
 
   __shared__ uint32_t wcnt[kSampleBlock / 64], wfirst[kSampleBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -287,7 +287,8 @@ __device__ __forceinline__ void bisect_particle(const PlantImage &P, const Voxel
       Ps.slot_sphere = opaque_uniform(P.slot_sphere);
       Ps.self_active = opaque_uniform(P.self_active);
     }
-    bool col = collides_variant<VAR, false, 3 | kCoherent>(Ps, QReg<DP>{mid}, V, R, idle, tab, sub, S);
+    bool col = collides_variant<VAR, false, 3 | kCoherent | (S == 1 ? kGrouped : 0)>(
+        Ps, QReg<DP>{mid}, V, R, idle, tab, sub, S);
     if constexpr (S > 1) {
       part[threadIdx.x] = col ? 1 : 0;
       __syncthreads();
