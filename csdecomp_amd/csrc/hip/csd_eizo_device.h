@@ -116,7 +116,17 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
         dir[2 * k] = dir[2 * k + 1] = 0.f;
       }
     }
-    float tmax = 3.402823466e+38f, tmin = -3.402823466e+38f;
+    // Chord of the line x + t dir: face f bounds t by its slack
+    // lb = b_f - a_f x over ad = a_f dir, from above if ad > 0 and from below
+    // if ad < 0. With the chain inside (lb > 0) the tightest bounds are
+    // 1 / max_f(ad / lb) and 1 / min_f(ad / lb), so a face costs one
+    // reciprocal (hardware, 1 ulp), a product and a max and a min, with no
+    // compare or select; the two reciprocals of the extremes follow the
+    // loop. A slack that rounding left at or below zero counts as 1e-30:
+    // the face then stops the chain from moving further out (bound ~0 on
+    // that side), as the reference's quotient lb / ad does. ad = 0 gives
+    // ad / lb = 0 and no bound.
+    float qmax = 0.f, qmin = 0.f;
 #pragma unroll 2
     for (int f = 0; f < F; ++f) {
       const float *a = A + (size_t)f * stride;
@@ -131,13 +141,12 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
         }
       }
       const float bf = BROW ? a[DP - 1] : b[f];
-      // one division per face, by the hardware reciprocal (1 ulp), selected
-      // into the bound it tightens; ad = 0 gives +-inf or NaN and no bound,
-      // as the IEEE quotient does
-      const float t = (bf - ax) * __builtin_amdgcn_rcpf(ad);
-      tmin = ad < 0.f ? fmaxf(tmin, t) : tmin;
-      tmax = ad > 0.f ? fminf(tmax, t) : tmax;
+      const float q = ad * __builtin_amdgcn_rcpf(fmaxf(bf - ax, 1e-30f));
+      qmax = fmaxf(qmax, q);
+      qmin = fminf(qmin, q);
     }
+    const float tmax = qmax > 0.f ? 1.f / qmax : 3.402823466e+38f;
+    const float tmin = qmin < 0.f ? 1.f / qmin : -3.402823466e+38f;
     if (isinf(tmax) || isinf(tmin) || tmax < tmin) continue;
     // theta = u (0.999999 tmax - tmin) + tmin, in double as the reference
     const float theta = (float)((double)ut * (0.999999 * (double)tmax - (double)tmin) +
