@@ -34,7 +34,7 @@ namespace csd {
 // unchecked into the block's staging segment as [dim][256] (coalesced).
 // Kept apart from the collision check so that the chains run at the high
 // occupancy their small register footprint allows.
-template <int DPL, bool BROW = false, int BLK = kSampleBlock>
+template <int DPL, bool BROW = false, int BLK = kSampleBlock, int DIMC = 0>
 __global__ __launch_bounds__(BLK) void k_eizo_sample(EizoDev D, int dim, int cap, int nblk,
                                                      int mixing, uint32_t k0, uint32_t k1,
                                                      uint32_t round, uint32_t stream_off) {
@@ -77,8 +77,8 @@ __global__ __launch_bounds__(BLK) void k_eizo_sample(EizoDev D, int dim, int cap
     const float e = d < dim ? D.le[(size_t)r * dim + d] : 0.f;
     x[d] = fmaf(f, e - a, a);
   }
-  hit_and_run<DP, LDS, BROW>(A, b, F, dim, x, i, (uint32_t)r + stream_off, round, mixing, k0,
-                             k1);
+  hit_and_run<DP, LDS, BROW, DIMC>(A, b, F, dim, x, i, (uint32_t)r + stream_off, round, mixing,
+                                   k0, k1);
   float *seg = D.stage + ((size_t)slot * nblk + sblk) * kSampleBlock * dim;
 #pragma unroll
   for (int d = 0; d < DP; ++d)
@@ -124,13 +124,19 @@ void launch_sampler(const SamplerLaunch &L, int nblk, int slots, hipStream_t s,
 #define CSD_SAMPLE(DPL, BR, BLK)                                                              \
   hipLaunchKernelGGL((k_eizo_sample<DPL, BR, BLK>), g, bk, L.lds, s, E, dim, cap, nblk, mixing, \
                      k0, k1, round, off)
+  // 14 dimensions (two 7-DOF arms): rows of 16 with b in slot 15, and the
+  // dot products skip the zero slot 14
+#define CSD_SAMPLE14(BLK)                                                                     \
+  hipLaunchKernelGGL((k_eizo_sample<33, true, BLK, 14>), g, bk, L.lds, s, E, dim, cap, nblk,   \
+                     mixing, k0, k1, round, off)
   if (L.dpl == 32) CSD_SAMPLE(32, false, kSampleBlock);
   else if (L.blk == kSampleBlock) {
     if (L.dpl == 17) {
       if (L.brow) CSD_SAMPLE(17, true, kSampleBlock);
       else CSD_SAMPLE(17, false, kSampleBlock);
     } else {
-      if (L.brow) CSD_SAMPLE(33, true, kSampleBlock);
+      if (L.brow && dim == 14) CSD_SAMPLE14(kSampleBlock);
+      else if (L.brow) CSD_SAMPLE(33, true, kSampleBlock);
       else CSD_SAMPLE(33, false, kSampleBlock);
     }
   } else {
@@ -138,10 +144,12 @@ void launch_sampler(const SamplerLaunch &L, int nblk, int slots, hipStream_t s,
       if (L.brow) CSD_SAMPLE(17, true, kSampleBlockWide);
       else CSD_SAMPLE(17, false, kSampleBlockWide);
     } else {
-      if (L.brow) CSD_SAMPLE(33, true, kSampleBlockWide);
+      if (L.brow && dim == 14) CSD_SAMPLE14(kSampleBlockWide);
+      else if (L.brow) CSD_SAMPLE(33, true, kSampleBlockWide);
       else CSD_SAMPLE(33, false, kSampleBlockWide);
     }
   }
+#undef CSD_SAMPLE14
 #undef CSD_SAMPLE
 }
 
@@ -152,7 +160,8 @@ int allow_wide_sampler_lds() {
         reinterpret_cast<const void *>(&k_eizo_sample<17, true, kSampleBlockWide>),
         reinterpret_cast<const void *>(&k_eizo_sample<17, false, kSampleBlockWide>),
         reinterpret_cast<const void *>(&k_eizo_sample<33, true, kSampleBlockWide>),
-        reinterpret_cast<const void *>(&k_eizo_sample<33, false, kSampleBlockWide>)};
+        reinterpret_cast<const void *>(&k_eizo_sample<33, false, kSampleBlockWide>),
+        reinterpret_cast<const void *>(&k_eizo_sample<33, true, kSampleBlockWide, 14>)};
     for (const void *f : fns)
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kLdsLimitWide) != hipSuccess)

@@ -65,8 +65,9 @@ __device__ __forceinline__ void stage_polytope(float *sA, float *sb, const float
 // steps from x, in the polytope (A row-major [F][stride], b). A and b are
 // wave-uniform: rows are LDS broadcasts (LDS = true, stride DP) or scalar
 // loads (stride dim). A failed step (empty chord) leaves the chain where it
-// is.
-template <int DP, bool LDS, bool BROW = false>
+// is. DIMC > 0: dim == DIMC, and the dot products stop there (the padding
+// slots they skip only added exact zeros).
+template <int DP, bool LDS, bool BROW = false, int DIMC = 0>
 __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
                                             const float *__restrict__ b, int F, int dim,
                                             float (&x)[DP], uint32_t chain, uint32_t stream,
@@ -134,7 +135,7 @@ __device__ __forceinline__ void hit_and_run(const float *__restrict__ A,
       // with b in the last slot that slot is skipped: x and dir are 0 there,
       // so the skipped terms were exact zeros
 #pragma unroll
-      for (int d = 0; d < (BROW ? DP - 1 : DP); ++d) {
+      for (int d = 0; d < (DIMC > 0 ? DIMC : BROW ? DP - 1 : DP); ++d) {
         if (LDS || d < dim) {
           ax = fmaf(a[d], x[d], ax);
           ad = fmaf(a[d], dir[d], ad);
