@@ -76,13 +76,15 @@ constexpr int kGrouped = 8;
 // other, so their bound loads overlap instead of each obstacle waiting for
 // its own -- give a mask of the (body, obstacle) blocks left per body; then
 // only the obstacles with a block left have their records read and tested.
-// row(o, b, rb) fills rb[KM] with body b's contact distances against o.
+// row(o, b, rb) fills rb[KM] with body b's contact distances against o;
+// bounds[o] is obstacle o's bounding sphere (staged in LDS or the plant's).
 template <int KB, int KM, bool ANY, class Row>
 __device__ __forceinline__ bool obstacles_culled(const PlantImage &P, int b0,
                                                  const float (&cx)[KB * KM],
                                                  const float (&cy)[KB * KM],
                                                  const float (&cz)[KB * KM], bool col, int o0,
-                                                 int nsub, const Row &row) {
+                                                 int nsub, const float4 *bounds,
+                                                 const Row &row) {
   for (int c0 = o0; c0 < P.num_obstacles; c0 += 32 * nsub) {
     if (wave_done<ANY>(col)) break;
     uint32_t need[KB];
@@ -92,7 +94,7 @@ __device__ __forceinline__ bool obstacles_culled(const PlantImage &P, int b0,
     for (int j = 0; j < 32; ++j) {
       const int o = c0 + j * nsub;
       if (o >= P.num_obstacles) continue;
-      const float4 ob = P.obstacle_bound[o];
+      const float4 ob = bounds[o];
 #pragma unroll
       for (int bl = 0; bl < KB; ++bl)
         if (b0 + bl < P.num_bodies &&
@@ -553,10 +555,12 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
   constexpr bool kCoh = (PH & kCoherent) != 0;
   static_assert(NS <= 64 && KG * G == NB, "the table plants, whole groups");
   col = col || P.always_colliding;
-  // contact-distance tables: staged in LDS (tab) for 64 slots, the plant's
-  // own arrays (same layouts) for 32
+  // contact-distance tables and obstacle bounds: staged in LDS (tab) for 64
+  // slots, the plant's own arrays (same layouts) for 32
   const float *stab = NS == 64 ? tab : P.self_r2;
   const float *orow = NS == 64 ? tab + NB * NB * KM * KM : P.obstacle_r2;
+  const float4 *obnd = NS == 64 ? reinterpret_cast<const float4 *>(orow + (size_t)P.num_obstacles * NS)
+                                : P.obstacle_bound;
   int rr = 0;  // the part that takes the next body-pair block
   auto mine = [&]() {
     const bool m = rr == sub;
@@ -676,7 +680,7 @@ __device__ __forceinline__ bool collides_fast_groups(const PlantImage &P, const 
     // segments, 11% VALU-busy; in chunks it takes 9.1 ms.)
     if constexpr (PH & 2) {
       const int o0 = sub >= rr ? sub - rr : sub - rr + nsub;
-      col = obstacles_culled<KG, KM, ANY>(P, b0, cx, cy, cz, col, o0, nsub,
+      col = obstacles_culled<KG, KM, ANY>(P, b0, cx, cy, cz, col, o0, nsub, obnd,
                                           [&](int o, int b, float *rb) {
                                             const float *r2 = orow + (size_t)o * NS + b * KM;
 #pragma unroll
@@ -824,7 +828,7 @@ constexpr size_t kCheckLdsMax = 56 * 1024;
 inline size_t check_lds_bytes(const PlantImage &P, int var) {
   if (variant_slots(var) != 64) return 0;
   const size_t nbt = P.nslots / P.kmax;
-  return (nbt * nbt * P.kmax * P.kmax + (size_t)P.num_obstacles * 64) * sizeof(float);
+  return (nbt * nbt * P.kmax * P.kmax + (size_t)P.num_obstacles * (64 + 4)) * sizeof(float);
 }
 
 // the fast shape of a fast-path plant
@@ -845,8 +849,9 @@ inline int check_variant(const PlantImage &P, bool need_generic) {
 }
 
 // Copies the contact-distance tables of a 64-slot variant into LDS (self
-// pairs, then obstacle rows) and returns them; every thread of the block
-// must call it. Other variants need nothing (nullptr).
+// pairs, then obstacle rows, then the obstacles' bounding spheres as float4)
+// and returns them; every thread of the block must call it. Other variants
+// need nothing (nullptr).
 template <int VAR>
 __device__ __forceinline__ const float *stage_tables(const PlantImage &P, float *smem) {
   if constexpr (variant_slots(VAR) == 64) {
@@ -855,6 +860,8 @@ __device__ __forceinline__ const float *stage_tables(const PlantImage &P, float 
     const int no = P.num_obstacles * 64;
     for (int e = threadIdx.x; e < ns; e += blockDim.x) smem[e] = P.self_r2[e];
     for (int e = threadIdx.x; e < no; e += blockDim.x) smem[ns + e] = P.obstacle_r2[e];
+    const float *ob = reinterpret_cast<const float *>(P.obstacle_bound);
+    for (int e = threadIdx.x; e < 4 * P.num_obstacles; e += blockDim.x) smem[ns + no + e] = ob[e];
     __syncthreads();
     return smem;
   }
