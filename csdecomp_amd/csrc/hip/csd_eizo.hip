@@ -404,19 +404,26 @@ __global__ __launch_bounds__(kRankBlock) void k_eizo_rank(EizoDev D, const float
 }
 
 // Stage 3: projection + all bisection steps per lane; FP64 distance from
-// the last colliding point to the (pre-bisection) projection.
-template <int VAR, int DP>
+// the last colliding point to the (pre-bisection) projection. DPS = 8 DP + S:
+// S waves share each 64 particles, each testing part of the pairs, when a
+// launch has few waves (eizo_bisect_split).
+template <int VAR, int DPS>
 __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, VoxelArgs V,
                                                               RobotGeomList R, EizoDev D,
                                                               int dim, int U, int steps) {
+  constexpr int DP = DPS / 8, S = DPS % 8;
+  constexpr int kPer = kSampleBlock / S;
   if (D.ctl && blockIdx.y >= D.ctl[1]) return;
   const int r = D.active[blockIdx.y];
   if (D.status[r] != 0) return;
   const int u = D.u[r];
-  const int i = blockIdx.x * kSampleBlock + threadIdx.x;
-  if ((int)(blockIdx.x * kSampleBlock) >= u) return;
+  if ((int)(blockIdx.x * kPer) >= u) return;  // whole block past this region's particles
   extern __shared__ __attribute__((aligned(16))) float csd_smem[];
+  __shared__ uint8_t verdict[kSampleBlock];  // the parts' verdicts (S > 1)
   const float *tab = stage_tables<VAR>(P, csd_smem);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = wave % S, grp = wave / S;
+  const int i = blockIdx.x * kPer + grp * 64 + lane;
   const bool valid = i < u;
   const int ii = valid ? i : u - 1;
   float p[DP], q1[DP], q2[DP], sc[DP];
@@ -427,11 +434,12 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
     q2[d] = d < dim ? D.le[(size_t)r * dim + d] : 0.f;
   }
   // the projection goes to D.proj before the steps and is read back for the
-  // distance (this lane's own stores)
+  // distance (this lane's own stores; part 0 of the particle's waves owns it)
+  const bool owner = valid && sub == 0;
   float *proj = D.proj + ((size_t)r * U + i) * dim;
-  bisect_particle<VAR, DP>(P, V, R, p, q1, q2, steps, !valid, sc, valid ? proj : nullptr, dim,
-                           tab);
-  if (!valid) return;
+  bisect_particle<VAR, DP, S>(P, V, R, p, q1, q2, steps, !valid, sc, owner ? proj : nullptr,
+                              dim, tab, sub, verdict);
+  if (!owner) return;
   double dd = 0.0;
 #pragma unroll
   for (int d = 0; d < DP; ++d) {
@@ -443,6 +451,42 @@ __global__ __launch_bounds__(kSampleBlock) void k_eizo_bisect(PlantImage P, Voxe
   }
   D.dist[(size_t)r * U + i] = sqrt(dd);
 }
+
+// Waves per 64 particles of k_eizo_bisect: 4 or 2 when the launch would
+// otherwise have fewer than 2,048 or 4,096 waves (two or four per SIMD), so
+// that each particle's pair tests are split over several waves and the
+// steps' latency overlaps; else 1 (the thresholds of csd_bisect's split).
+inline int eizo_bisect_split(int regions, uint32_t particles) {
+  const int64_t waves = (int64_t)regions * ((particles + 63) / 64);
+  return waves < 2048 ? 4 : waves < 4096 ? 2 : 1;
+}
+
+// k_eizo_bisect over `regions` slots of up to `particles` particles each
+#define CSD_EIZO_BISECT(regions, particles, stream, D_)                                        \
+  do {                                                                                         \
+    const int split_ = eizo_bisect_split((regions), (particles));                              \
+    const int per_ = kSampleBlock / split_;                                                    \
+    const dim3 g_((unsigned)(((particles) + per_ - 1) / per_), (unsigned)(regions));           \
+    const int st_ = (int)o.bisection_steps;                                                    \
+    if (dp == 8 && split_ == 4)                                                                \
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 68, g_, dim3(kSampleBlock), tab_bytes,        \
+                             stream, P, V, RL, D_, dim, U, st_);                               \
+    else if (dp == 8 && split_ == 2)                                                           \
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 66, g_, dim3(kSampleBlock), tab_bytes,        \
+                             stream, P, V, RL, D_, dim, U, st_);                               \
+    else if (dp == 8)                                                                          \
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 65, g_, dim3(kSampleBlock), tab_bytes,        \
+                             stream, P, V, RL, D_, dim, U, st_);                               \
+    else if (split_ == 4)                                                                      \
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 132, g_, dim3(kSampleBlock), tab_bytes,       \
+                             stream, P, V, RL, D_, dim, U, st_);                               \
+    else if (split_ == 2)                                                                      \
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 130, g_, dim3(kSampleBlock), tab_bytes,       \
+                             stream, P, V, RL, D_, dim, U, st_);                               \
+    else                                                                                       \
+      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 129, g_, dim3(kSampleBlock), tab_bytes,       \
+                             stream, P, V, RL, D_, dim, U, st_);                               \
+  } while (0)
 
 // Device-resident round start (one workgroup): the regions still running
 // (status 0), in region order, become this round's slots, with this round's
@@ -1044,7 +1088,6 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
     const bool ranked = w->surrogate != nullptr;
     const bool pruned = w->pilot && w->prune && w->surrogate;
     const uint32_t bmax = ranked ? (uint32_t)w->keep : max_budget;
-    const int ublk = (int)((bmax + kSampleBlock - 1) / kSampleBlock);
     const int pblk = (int)((w->pilot + kSampleBlock - 1) / kSampleBlock);
     const float limit = w->pilot_factor * o.epsilon * (float)w->pilot;
     const int L = w->line_points;
@@ -1139,12 +1182,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
         hipLaunchKernelGGL(k_eizo_rank, dim3(Rn), dim3(kRankBlock), 0, s, G, w->pred, dim, U, L,
                            kRankMax, w->keep, G.stage, 0u);
       }
-      if (dp == 8)
-        CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 8, dim3(ublk, Rn), dim3(kSampleBlock),
-                               tab_bytes, s, P, V, RL, G, dim, U, (int)o.bisection_steps);
-      else
-        CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 16, dim3(ublk, Rn), dim3(kSampleBlock),
-                               tab_bytes, s, P, V, RL, G, dim, U, (int)o.bisection_steps);
+      CSD_EIZO_BISECT(Rn, bmax, s, G);
       hipLaunchKernelGGL(k_eizo_faces, dim3(Rn), dim3(kFaceBlock), 0, s, G, dim, U, cap,
                          o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1, 0u);
       hipError_t e = hipStreamEndCapture(s, &graph);
@@ -1225,13 +1263,7 @@ int csd_eizo_run(csd_eizo *w, const float *domain_A, const float *domain_b, int3
                          cand, w->keep, D.stage, it);
     }
     const uint32_t bmax = w->surrogate ? (uint32_t)w->keep : max_budget;
-    const int ublk = (int)((bmax + kSampleBlock - 1) / kSampleBlock);
-    if (dp == 8)
-      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 8, dim3(ublk, na), dim3(kSampleBlock),
-                             tab_bytes, 0, P, V, RL, D, dim, U, (int)o.bisection_steps);
-    else
-      CSD_DISPATCH_VARIANT_X(var, k_eizo_bisect, 16, dim3(ublk, na), dim3(kSampleBlock),
-                             tab_bytes, 0, P, V, RL, D, dim, U, (int)o.bisection_steps);
+    CSD_EIZO_BISECT(na, bmax, 0, D);
     hipLaunchKernelGGL(k_eizo_faces, dim3(na), dim3(kFaceBlock), 0, 0, D, dim, U, cap,
                        o.configuration_margin, (int)o.max_hyperplanes_per_iteration, 1, it);
     CSD_HIP(hipGetLastError());
